@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Throughput of the per-iteration residual-and-update path on one MI355X.
+
+Workload (default): the viscous RANS configuration -- single-species air,
+k-omega SST, one 256x128x64 flat-plate block with tanh wall clustering,
+Mach 2.5, Re=1e7/m, T=220 K, MUSCL+minmod+AUSM+ with the 2nd-order central
+viscous flux, DPLUR with 4 sweeps, CFL 5; FP32 bulk storage/arithmetic.
+A "step" is one full steady iteration of the Driver loop (source, flux
+sweeps, local dt, DPLUR, update, limit_flow, BCs, properties; the residual
+every 100 steps), with the state resident in HBM.
+
+Prints ONE JSON line.  value = Mcell-iter/s; roofline = the dominant
+kernel's algorithmic bytes / its measured average duration (HIP events on the
+launch stream over the timed region) vs the 8 TB/s HBM peak; cpu_baseline =
+the FP64 C oracle (OpenMP) timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--profile-json", default=None, help="also write per-stage timings here")
+    return ap.parse_args()
+
+
+# Algorithmic scalars per cell (distinct values read + written, stencil
+# neighbours counted once) for the air / SST / DPLUR path, per launch.
+def alg_scalars(nv: int, rans: bool, dplur: bool):
+    nsc = 2 if rans else 0
+    turb = 5 if rans else 0  # mut, lambda_t, wall distance (+ rho, k, omega already in bv/sv)
+    flux = 6 + nsc + 9 + 1 + 2 + (turb - 2 if rans else 0) + 2 * nv  # bv, sv, metric, J, mu, lambda (+mut, lt, d), dq RMW
+    return {
+        "flux_i": flux, "flux_j": flux, "flux_k": flux,
+        "source": (4 + nsc + 9 + 1 + 1 + 1) + (2 + nv + 2) if rans else nv,
+        "local_dt": 9 + 3 + 3 + (1 if rans else 0) + 4,
+        "dplur_dq0": nv + 1 + 3 + (2 if rans else 0) + nv,
+        "dplur_sweep": nv + 5 + 1 + nsc + 9 + 3 + 1 + nv + (2 if rans else 0) + nv,
+        "update": nv + nv + 2 + nv + 6 + nsc + 1,
+        "limit_flow": 5 + nsc,
+        "properties": 6,
+    }
+
+
+def cpu_baseline(case_builder, steps: int):
+    """FP64 oracle, OpenMP over (k,j), same configuration, `steps` iterations."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ffi as O
+
+    case = case_builder()
+    sim = O.OracleSimulation(case)
+    blk, ob = case.blocks[0], sim.blocks[0]
+    if case.wall_points is not None:
+        # nearest wall node on the tensor-product wall grid (equal to the
+        # brute-force minimum of src/Driver.cu:461-482 for this grid)
+        xs = np.unique(blk.x[0, 0, :])
+        zs = np.unique(blk.z[:, 0, 0])
+        X, Y, Z = ob.arr["x"][0], ob.arr["y"][0], ob.arr["z"][0]
+        nx_ = xs[np.clip(np.searchsorted(xs, X), 0, len(xs) - 1)]
+        nx2 = xs[np.clip(np.searchsorted(xs, X) - 1, 0, len(xs) - 1)]
+        dx = np.minimum(np.abs(X - nx_), np.abs(X - nx2))
+        nz_ = zs[np.clip(np.searchsorted(zs, Z), 0, len(zs) - 1)]
+        nz2 = zs[np.clip(np.searchsorted(zs, Z) - 1, 0, len(zs) - 1)]
+        dz = np.minimum(np.abs(Z - nz_), np.abs(Z - nz2))
+        ob.arr["wall_distance"][0] = np.sqrt(Y * Y + dx * dx + dz * dz)
+    # initialisation as the Driver does (src/Driver.cu:43-104), wall distance set above
+    lib, p = sim.lib, C.byref(sim.param)
+    z0 = C.byref(sim.zones[0])
+    lib.orc_apply_bc(z0, p, C.byref(sim.bcs[0]))
+    lib.orc_compute_cv_from_bv(z0, p)
+    lib.orc_initialize_mut(z0, p)
+    lib.orc_update_physical_properties(z0, p)
+    sim.step()  # first iteration (residual scale) outside the timing
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(output_screen=case.output_screen)
+    dt = time.perf_counter() - t0
+    cells = sum(b.mx * b.my * b.mz for b in case.blocks)
+    return {
+        "value": cells * steps / dt / 1e6,
+        "unit": "Mcell-iter/s",
+        "cores": int(lib.orc_omp_threads()),
+        "kind": "port",
+        "sample": f"{steps} iterations of the same {case.name} block ({cells} cells), FP64 C oracle with OpenMP "
+                  f"over (k,j); setup excluded",
+        "seconds": dt,
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only (one block per GPU is the scale-out "
+                                   "design; multi-GPU halos are not built)", "n_gpus": a.gpus}))
+        sys.exit(2)
+    from openstc_amd import _lib as L
+    from openstc_amd import cases
+    from openstc_amd.runtime import Simulation
+
+    if a.config == 2:
+        build = lambda: cases.config2()  # noqa: E731
+    else:
+        build = lambda: cases.config1(mx=256, my=256, mz=64)  # noqa: E731
+    case = build()
+    cells = sum(b.mx * b.my * b.mz for b in case.blocks)
+    print(f"[bench] {case.name}: {cells} cells, setup", file=sys.stderr, flush=True)
+    sim = Simulation(case)
+    sim.initialize_computation()
+    lib = sim.ctx.lib
+    for _ in range(a.warmup):
+        sim.step(output_screen=case.output_screen)
+    sim.sync()
+    L.check(lib.ostc_profile(sim.ctx.h, 1), "profile")
+    print(f"[bench] timing {a.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sim.step(output_screen=case.output_screen)
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    nst = lib.ostc_profile_stage_count()
+    ms = (C.c_double * nst)()
+    cnt = (C.c_longlong * nst)()
+    L.check(lib.ostc_profile_read(sim.ctx.h, ms, cnt, nst, 1), "profile_read")
+    L.check(lib.ostc_profile(sim.ctx.h, 0), "profile off")
+    names = [lib.ostc_profile_stage_name(i).decode() for i in range(nst)]
+    p = case.param
+    rans = p.turb_method == L.RANS
+    esize = 4 if case.precision == L.FP32 else 8
+    scal = alg_scalars(p.n_var, rans, p.implicit_method == 1)
+    stages = {}
+    for i, n in enumerate(names):
+        if cnt[i] == 0:
+            continue
+        avg_ms = ms[i] / cnt[i]
+        st = {"total_ms": ms[i], "launches": int(cnt[i]), "avg_ms": avg_ms}
+        if n in scal:
+            byts = scal[n] * esize * cells
+            st["alg_bytes_per_launch"] = byts
+            st["achieved_GBs"] = byts / (avg_ms * 1e-3) / 1e9
+            st["frac_hbm_peak"] = st["achieved_GBs"] / HBM_PEAK_GBS
+        stages[n] = st
+    dom = max((n for n in stages if "alg_bytes_per_launch" in stages[n]), key=lambda n: stages[n]["total_ms"])
+    d = stages[dom]
+    ms_per_step = elapsed / a.steps * 1e3
+    value = cells * a.steps / elapsed / 1e6
+    out = {
+        "metric": "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem @1e-3",
+        "value": round(value, 3),
+        "unit": "Mcell-iter/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if esize == 4 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": case.name,
+            "block": [case.blocks[0].mx, case.blocks[0].my, case.blocks[0].mz],
+            "cells": cells,
+            "model": "air k-omega SST, MUSCL+minmod+AUSM+, central viscous, DPLUR x%d, CFL %g"
+                     % (p.dplur_inner_step, p.cfl) if rans else "air inviscid, explicit, CFL %g" % p.cfl,
+            "precision": "FP32 bulk" if esize == 4 else "FP64",
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(d["achieved_GBs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(d["frac_hbm_peak"], 4),
+            "traffic": None,
+            "alg_bytes_per_cell": scal[dom] * esize,
+            "avg_launch_ms": round(d["avg_ms"], 5),
+        },
+        "stages": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                   for k, v in stages.items()},
+    }
+    sim.close()
+    if not a.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cb = cpu_baseline(build, a.cpu_steps)
+        out["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cb.items()}
+    else:
+        out["cpu_baseline"] = None
+    if a.profile_json:
+        with open(a.profile_json, "w") as f:
+            json.dump(out, f, indent=1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

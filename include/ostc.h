@@ -1,0 +1,236 @@
+/*
+ * ostc.h -- C-ABI of the MI355X-native OpenSTC residual-and-update path.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns
+ * an int status (OSTC_OK == 0).  Streams are passed as void* (a hipStream_t,
+ * or NULL for the null stream).  The library owns all device memory behind
+ * ostc_ctx / ostc_zone handles; host data goes in and out through
+ * ostc_zone_upload / ostc_zone_download in FP64 regardless of the zone's
+ * storage precision.
+ *
+ * Each per-stage entry point replaces one stage of the reference's
+ * Driver<mix,turb>::steady_simulation loop (src/Driver.cu:218-289); the
+ * reference interface it stands in for is cited next to it.
+ */
+#ifndef OSTC_H
+#define OSTC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSTC_MAX_SPEC 20
+#define OSTC_MAX_REAC 50
+
+enum ostc_status {
+  OSTC_OK = 0,
+  OSTC_ERR_ARG = 1,       /* invalid argument / unsupported configuration */
+  OSTC_ERR_HIP = 2,       /* a HIP runtime call failed */
+  OSTC_ERR_ALLOC = 3,     /* device allocation failed */
+  OSTC_ERR_NAN = 4,       /* residual is NaN (reference aborts: src/Driver.cu:372-377) */
+  OSTC_ERR_STATE = 5      /* call out of order (e.g. zone not initialised) */
+};
+
+/* Mixture / turbulence models: src/Define.h:14-27 */
+enum { OSTC_AIR = 0, OSTC_MIXTURE = 1, OSTC_FR = 2 };
+enum { OSTC_LAMINAR = 0, OSTC_RANS = 1 };
+
+/* Model constants: the device copy of the reference's DParameter
+ * (src/DParameter.h:11-58, filled by src/DParameter.cu:4-101) plus the model
+ * switches the reference reads from Parameter (implicit_method,
+ * DPLUR_inner_step).  Species matrices use the fixed stride OSTC_MAX_SPEC;
+ * NASA-7 coefficient rows use stride 7. */
+typedef struct ostc_param {
+  int mix_model, turb_method;
+  int inviscid_scheme, reconstruction, limiter, viscous_scheme;
+  int rans_model, turb_implicit, chem_src_method;
+  int implicit_method, dplur_inner_step;
+  int n_spec, n_scalar, n_reac, n_var;
+  double Pr, cfl, Sc, Prt, Sct;
+  double mw[OSTC_MAX_SPEC], t_low[OSTC_MAX_SPEC], t_mid[OSTC_MAX_SPEC], t_high[OSTC_MAX_SPEC];
+  double low_temp_coeff[OSTC_MAX_SPEC * 7], high_temp_coeff[OSTC_MAX_SPEC * 7];
+  double LJ_potent_inv[OSTC_MAX_SPEC], vis_coeff[OSTC_MAX_SPEC];
+  double WjDivWi_to_One4th[OSTC_MAX_SPEC * OSTC_MAX_SPEC];
+  double sqrt_WiDivWjPl1Mul8[OSTC_MAX_SPEC * OSTC_MAX_SPEC];
+  int reac_type[OSTC_MAX_REAC], reac_order[OSTC_MAX_REAC];
+  int stoi_f[OSTC_MAX_REAC * OSTC_MAX_SPEC], stoi_b[OSTC_MAX_REAC * OSTC_MAX_SPEC];
+  double A[OSTC_MAX_REAC], b[OSTC_MAX_REAC], Ea[OSTC_MAX_REAC];
+  double A2[OSTC_MAX_REAC], b2[OSTC_MAX_REAC], Ea2[OSTC_MAX_REAC];
+  double third_body_coeff[OSTC_MAX_REAC * OSTC_MAX_SPEC];
+  double troe_alpha[OSTC_MAX_REAC], troe_t3[OSTC_MAX_REAC], troe_t1[OSTC_MAX_REAC], troe_t2[OSTC_MAX_REAC];
+  double limit_ll[7], limit_ul[7], sv_inf[OSTC_MAX_SPEC + 2];
+} ostc_param;
+
+/* Boundary program of one block: faces in application order (outflow ->
+ * inflow -> symmetry -> wall, src/BoundCond.cuh:321-412) with ranges as
+ * produced by Boundary::register_boundary (src/Mesh.cpp:14-32) and the wall
+ * trimming of src/BoundCond.cu:215-230.  kind is the BC label 2/3/5/6; slot
+ * indexes inflow[] (kind 5) or wall[] (kind 2). */
+#define OSTC_MAX_FACE 32
+#define OSTC_MAX_BCDATA 8
+typedef struct ostc_face {
+  int range_start[3], range_end[3];
+  int face, direction, kind, slot;
+} ostc_face;
+typedef struct ostc_inflow { /* src/BoundCond.h:8-26 */
+  double density, u, v, w, pressure, temperature, mut;
+  double sv[OSTC_MAX_SPEC + 2];
+} ostc_inflow;
+typedef struct ostc_wall { /* src/BoundCond.h:28-38 */
+  int isothermal;
+  double temperature;
+} ostc_wall;
+typedef struct ostc_bc {
+  int n_face;
+  ostc_face face[OSTC_MAX_FACE];
+  ostc_inflow inflow[OSTC_MAX_BCDATA];
+  ostc_wall wall[OSTC_MAX_BCDATA];
+} ostc_bc;
+
+/* Inner (block-to-block) face after InnerFace::register_boundary
+ * (src/Mesh.h:343-370, src/Mesh.cpp:44-178). */
+typedef struct ostc_inner_face {
+  int range_start[3], range_end[3], face, direction;
+  int target_start[3], target_end[3], target_block, target_face, target_direction;
+  int src_tar[3], loop_dir[3], target_loop_dir[3], n_point[3];
+} ostc_inner_face;
+
+/* Storage precision of the bulk fields of a zone.  OSTC_FP64 keeps every
+ * field in double (the reference's real=double).  OSTC_FP32 stores the flow
+ * fields, metrics and residuals in float; the chemistry Jacobian, its LU
+ * factors, Arrhenius/equilibrium evaluation and the T-from-e Newton solve
+ * stay in double. */
+enum { OSTC_FP32 = 4, OSTC_FP64 = 8 };
+
+typedef struct ostc_zone_desc {
+  int mx, my, mz, ngg; /* interior extents and ghost layers (ngg = 2) */
+  int precision;       /* OSTC_FP32 or OSTC_FP64 */
+  int block_id;
+} ostc_zone_desc;
+
+/* Field identifiers for upload/download (layout: the reference's
+ * VectorField3D, src/gxl_lib/Array.hpp:83-130, every field padded by ngg,
+ * idx = ((k+g)*(my+2g) + (j+g))*(mx+2g) + (i+g) + l*plane).  METRIC has 9
+ * planes, plane (r-1)*3+(c-1) = metric(r,c) of src/gxl_lib/Matrix.hpp:47-62. */
+enum ostc_field {
+  OSTC_F_X = 0, OSTC_F_Y, OSTC_F_Z, OSTC_F_JAC, OSTC_F_METRIC, OSTC_F_WALL_DIST,
+  OSTC_F_BV, OSTC_F_SV, OSTC_F_CV, OSTC_F_BV_LAST,
+  OSTC_F_VEL, OSTC_F_SOUND, OSTC_F_MACH, OSTC_F_MUL, OSTC_F_LAMBDA,
+  OSTC_F_RHO_D, OSTC_F_GAMMA, OSTC_F_CP, OSTC_F_CHEM_JAC,
+  OSTC_F_MUT, OSTC_F_LAMBDA_T, OSTC_F_TURB_JAC,
+  OSTC_F_DQ, OSTC_F_DQ0, OSTC_F_DQK, OSTC_F_SPECTR, OSTC_F_DT,
+  OSTC_F_COUNT
+};
+
+typedef struct ostc_ctx ostc_ctx;
+typedef struct ostc_zone ostc_zone;
+
+/* ---- context and zones: stand in for Driver's ctor (src/Driver.cu:14-41),
+ * Field::setup_device_memory (src/Field.cu:69-171) and the DParameter upload
+ * (src/Driver.cu:37-39). */
+int ostc_ctx_create(const ostc_param *param, ostc_ctx **out);
+int ostc_ctx_destroy(ostc_ctx *ctx);
+int ostc_zone_create(ostc_ctx *ctx, const ostc_zone_desc *desc, ostc_zone **out);
+int ostc_zone_destroy(ostc_zone *zone);
+/* n_planes of the field as allocated for this ctx/zone (0 if unused) */
+int ostc_zone_field_planes(const ostc_zone *zone, int field, int *n_planes);
+/* Points per plane (padded) */
+long ostc_zone_plane(const ostc_zone *zone);
+int ostc_zone_upload(ostc_zone *zone, int field, const double *host, int n_planes);
+int ostc_zone_download(const ostc_zone *zone, int field, double *host, int n_planes);
+/* Device pointer and element size of a field (for zero-copy callers). */
+int ostc_zone_field_ptr(const ostc_zone *zone, int field, void **dev_ptr, int *elem_size);
+int ostc_zone_set_bc(ostc_zone *zone, const ostc_bc *bc);
+int ostc_sizeof_param(void);
+int ostc_sizeof_bc(void);
+
+/* ---- host-side setup (CPU, FP64) ---- */
+/* Ghost coordinates (Mesh::init_ghost_grid, src/Mesh.cpp:789-943) and the
+ * 2nd-order Jacobian/metrics (Block::compute_jac_metric, src/Mesh.cpp:
+ * 259-521) of one block with no inner faces.  x/y/z_in: interior points,
+ * i fastest (mx*my*mz).  Outputs are ngg-padded in the zone layout:
+ * x/y/z_out (optional, 1 plane each), jac (1 plane), metric (9 planes).
+ * Bit-exact with the reference's operation order. */
+int ostc_block_geometry(int mx, int my, int mz, int ngg, const double *x_in, const double *y_in, const double *z_in,
+                        double *x_out, double *y_out, double *z_out, double *jac, double *metric, int *n_neg_jac);
+
+/* ---- setup stages ---- */
+/* compute_cv_from_bv<M,T> (src/FieldOperation.cuh:37-73) */
+int ostc_compute_cv_from_bv(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* initialize_mut<M> (src/FieldOperation.cuh:111-137) */
+int ostc_initialize_mut(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* compute_wall_distance (src/Driver.cu:461-482); wall_xyz holds n points */
+int ostc_wall_distance(ostc_ctx *ctx, ostc_zone *zone, const double *wall_xyz, int n_wall_point, void *stream);
+
+/* ---- per-iteration stages (src/Driver.cu:232-272) ---- */
+/* compute_source<M,T> (src/SourceTerm.cuh:11-40); writes dq (no memset needed) */
+int ostc_compute_source(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* compute_inviscid_flux<M,T> (src/SchemeSelector.cuh:36-68) */
+int ostc_inviscid_flux(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* compute_viscous_flux<M,T> (src/SchemeSelector.cuh:172-192) */
+int ostc_viscous_flux(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* local_time_step<M,T> (src/TimeAdvanceFunc.cuh:21-71) */
+int ostc_local_time_step(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* implicit_treatment<M,T> (src/ImplicitTreatmentHPP.cuh:10-59), DPLUR
+ * (src/DPLURHPP.cuh:251-273) with param.dplur_inner_step sweeps */
+int ostc_implicit_treatment(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* update_cv_and_bv<M,T> (src/FieldOperation.cuh:139-183) incl. the
+ * T-from-e Newton solve (src/FieldOperation.cu:3-40) */
+int ostc_update_cv_bv(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* limit_flow (src/TimeAdvanceFunc.cu:34-224) */
+int ostc_limit_flow(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* DBoundCond::apply_boundary_conditions<M,T> (src/BoundCond.cuh:321-412) */
+int ostc_apply_bc(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* inner_communication<M,T> for one face of zones[blk]
+ * (src/DataCommunication.cuh:27-98) */
+int ostc_inner_halo(ostc_ctx *ctx, ostc_zone **zones, int blk, const ostc_inner_face *face, void *stream);
+/* update_physical_properties<M,T> (src/FieldOperation.cuh:75-109) */
+int ostc_update_physical_properties(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* store_last_step (src/TimeAdvanceFunc.cu:5-16) */
+int ostc_store_last_step(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* compute_square_of_dbv + reduction_of_dv_squared (src/TimeAdvanceFunc.cu:
+ * 18-32, src/Driver.cu:309-328): sums[4] of the squared changes of rho, |V|,
+ * p, T over the interior of the given zones (synchronises the stream). */
+int ostc_residual_sums(ostc_ctx *ctx, ostc_zone **zones, int n, double sums[4], void *stream);
+/* Device-side counters since the last reset: [0] Newton iterations that hit
+ * the 1000 cap, [1] cells repaired by limit_flow, [2] surrogate fallbacks
+ * (synchronises the stream). */
+int ostc_zone_counters(ostc_zone *zone, long long out[4], int reset, void *stream);
+
+/* One steady iteration of Driver::steady_simulation over n zones
+ * (src/Driver.cu:218-281) in the reference's stage order.  inner faces of
+ * block b are inner[off_b .. off_b+n_inner[b]).  Returns the residual sums
+ * in res_sums[4] when step%output_screen==0 or step==1 (else leaves them). */
+int ostc_steady_step(ostc_ctx *ctx, ostc_zone **zones, int n, const ostc_inner_face *inner, const int *n_inner,
+                     int step, int output_screen, double res_sums[4], int *res_computed, void *stream);
+
+/* ---- stage timing (no reference counterpart; the reference prints only the
+ * wall time every output_screen steps, src/Driver.cu:382-401) ----
+ * When enabled, every stage launched by the library is bracketed by HIP
+ * events on its stream; ostc_profile_read synchronises and returns, per
+ * stage, the summed milliseconds and the number of bracketed launches. */
+int ostc_profile(ostc_ctx *ctx, int enable);
+int ostc_profile_read(ostc_ctx *ctx, double *ms, long long *count, int n, int reset);
+const char *ostc_profile_stage_name(int stage);
+int ostc_profile_stage_count(void);
+
+/* ---- chemistry points and the learned surrogate ---- */
+/* Plain point-implicit chemistry step on a batch of states (FP64): rho from
+ * EOS, (I - dt*J_chem) d(rhoY) = dt*omega (the reference's explicit EPI,
+ * src/FiniteRateChem.cu:7-63,173-194,211-234, applied to the source alone).
+ * All arrays are device pointers: t[n], p[n], y[n*ns], dt[n] -> dy[n*ns]. */
+int ostc_chem_plain_step(ostc_ctx *ctx, long n, const double *t, const double *p, const double *y,
+                         const double *dt, double *dy, void *stream);
+/* omega[n*ns] (kg/m3/s) and the EPI Jacobian jac[n*ns*ns] at (rho,T,Y);
+ * device pointers (src/FiniteRateChem.cu:7-63,173-194). */
+int ostc_chem_source_batch(ostc_ctx *ctx, long n, const double *rho, const double *t, const double *y,
+                           double *omega, double *jac, void *stream);
+
+int ostc_device_sync(void);
+const char *ostc_last_error(void);
+const char *ostc_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

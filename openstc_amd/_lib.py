@@ -1,0 +1,172 @@
+"""ctypes binding of libostc.so (the C-ABI declared in include/ostc.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C openstc_amd/csrc``).  Loading fails loudly when it is missing: there
+is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+MAX_SPEC = 20
+MAX_REAC = 50
+MAX_FACE = 32
+MAX_BCDATA = 8
+
+OK = 0
+AIR, MIXTURE, FR = 0, 1, 2
+LAMINAR, RANS = 0, 1
+FP32, FP64 = 4, 8
+
+# field identifiers (enum ostc_field)
+FIELDS = [
+    "X", "Y", "Z", "JAC", "METRIC", "WALL_DIST",
+    "BV", "SV", "CV", "BV_LAST",
+    "VEL", "SOUND", "MACH", "MUL", "LAMBDA",
+    "RHO_D", "GAMMA", "CP", "CHEM_JAC",
+    "MUT", "LAMBDA_T", "TURB_JAC",
+    "DQ", "DQ0", "DQK", "SPECTR", "DT",
+]
+F = {name: i for i, name in enumerate(FIELDS)}
+
+
+class Param(C.Structure):
+    """struct ostc_param (mirror of the reference's DParameter)."""
+    _fields_ = [
+        ("mix_model", C.c_int), ("turb_method", C.c_int),
+        ("inviscid_scheme", C.c_int), ("reconstruction", C.c_int), ("limiter", C.c_int),
+        ("viscous_scheme", C.c_int), ("rans_model", C.c_int), ("turb_implicit", C.c_int),
+        ("chem_src_method", C.c_int), ("implicit_method", C.c_int), ("dplur_inner_step", C.c_int),
+        ("n_spec", C.c_int), ("n_scalar", C.c_int), ("n_reac", C.c_int), ("n_var", C.c_int),
+        ("Pr", C.c_double), ("cfl", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double), ("Sct", C.c_double),
+        ("mw", C.c_double * MAX_SPEC), ("t_low", C.c_double * MAX_SPEC), ("t_mid", C.c_double * MAX_SPEC),
+        ("t_high", C.c_double * MAX_SPEC),
+        ("low_temp_coeff", C.c_double * (MAX_SPEC * 7)), ("high_temp_coeff", C.c_double * (MAX_SPEC * 7)),
+        ("LJ_potent_inv", C.c_double * MAX_SPEC), ("vis_coeff", C.c_double * MAX_SPEC),
+        ("WjDivWi_to_One4th", C.c_double * (MAX_SPEC * MAX_SPEC)),
+        ("sqrt_WiDivWjPl1Mul8", C.c_double * (MAX_SPEC * MAX_SPEC)),
+        ("reac_type", C.c_int * MAX_REAC), ("reac_order", C.c_int * MAX_REAC),
+        ("stoi_f", C.c_int * (MAX_REAC * MAX_SPEC)), ("stoi_b", C.c_int * (MAX_REAC * MAX_SPEC)),
+        ("A", C.c_double * MAX_REAC), ("b", C.c_double * MAX_REAC), ("Ea", C.c_double * MAX_REAC),
+        ("A2", C.c_double * MAX_REAC), ("b2", C.c_double * MAX_REAC), ("Ea2", C.c_double * MAX_REAC),
+        ("third_body_coeff", C.c_double * (MAX_REAC * MAX_SPEC)),
+        ("troe_alpha", C.c_double * MAX_REAC), ("troe_t3", C.c_double * MAX_REAC),
+        ("troe_t1", C.c_double * MAX_REAC), ("troe_t2", C.c_double * MAX_REAC),
+        ("limit_ll", C.c_double * 7), ("limit_ul", C.c_double * 7), ("sv_inf", C.c_double * (MAX_SPEC + 2)),
+    ]
+
+
+class Face(C.Structure):
+    _fields_ = [("range_start", C.c_int * 3), ("range_end", C.c_int * 3), ("face", C.c_int),
+                ("direction", C.c_int), ("kind", C.c_int), ("slot", C.c_int)]
+
+
+class Inflow(C.Structure):
+    _fields_ = [("density", C.c_double), ("u", C.c_double), ("v", C.c_double), ("w", C.c_double),
+                ("pressure", C.c_double), ("temperature", C.c_double), ("mut", C.c_double),
+                ("sv", C.c_double * (MAX_SPEC + 2))]
+
+
+class Wall(C.Structure):
+    _fields_ = [("isothermal", C.c_int), ("temperature", C.c_double)]
+
+
+class BC(C.Structure):
+    _fields_ = [("n_face", C.c_int), ("face", Face * MAX_FACE), ("inflow", Inflow * MAX_BCDATA),
+                ("wall", Wall * MAX_BCDATA)]
+
+
+class InnerFace(C.Structure):
+    _fields_ = [("range_start", C.c_int * 3), ("range_end", C.c_int * 3), ("face", C.c_int),
+                ("direction", C.c_int), ("target_start", C.c_int * 3), ("target_end", C.c_int * 3),
+                ("target_block", C.c_int), ("target_face", C.c_int), ("target_direction", C.c_int),
+                ("src_tar", C.c_int * 3), ("loop_dir", C.c_int * 3), ("target_loop_dir", C.c_int * 3),
+                ("n_point", C.c_int * 3)]
+
+
+class ZoneDesc(C.Structure):
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("ngg", C.c_int),
+                ("precision", C.c_int), ("block_id", C.c_int)]
+
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libostc.so")
+
+# every symbol include/ostc.h declares, with (restype, argtypes)
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+SIGNATURES = {
+    "ostc_ctx_create": (C.c_int, [C.POINTER(Param), C.POINTER(_P)]),
+    "ostc_ctx_destroy": (C.c_int, [_P]),
+    "ostc_zone_create": (C.c_int, [_P, C.POINTER(ZoneDesc), C.POINTER(_P)]),
+    "ostc_zone_destroy": (C.c_int, [_P]),
+    "ostc_zone_field_planes": (C.c_int, [_P, C.c_int, _I]),
+    "ostc_zone_plane": (C.c_long, [_P]),
+    "ostc_zone_upload": (C.c_int, [_P, C.c_int, _D, C.c_int]),
+    "ostc_zone_download": (C.c_int, [_P, C.c_int, _D, C.c_int]),
+    "ostc_zone_field_ptr": (C.c_int, [_P, C.c_int, C.POINTER(_P), _I]),
+    "ostc_zone_set_bc": (C.c_int, [_P, C.POINTER(BC)]),
+    "ostc_sizeof_param": (C.c_int, []),
+    "ostc_sizeof_bc": (C.c_int, []),
+    "ostc_block_geometry": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _I]),
+    "ostc_compute_cv_from_bv": (C.c_int, [_P, _P, _P]),
+    "ostc_initialize_mut": (C.c_int, [_P, _P, _P]),
+    "ostc_wall_distance": (C.c_int, [_P, _P, _D, C.c_int, _P]),
+    "ostc_compute_source": (C.c_int, [_P, _P, _P]),
+    "ostc_inviscid_flux": (C.c_int, [_P, _P, _P]),
+    "ostc_viscous_flux": (C.c_int, [_P, _P, _P]),
+    "ostc_local_time_step": (C.c_int, [_P, _P, _P]),
+    "ostc_implicit_treatment": (C.c_int, [_P, _P, _P]),
+    "ostc_update_cv_bv": (C.c_int, [_P, _P, _P]),
+    "ostc_limit_flow": (C.c_int, [_P, _P, _P]),
+    "ostc_apply_bc": (C.c_int, [_P, _P, _P]),
+    "ostc_inner_halo": (C.c_int, [_P, C.POINTER(_P), C.c_int, C.POINTER(InnerFace), _P]),
+    "ostc_update_physical_properties": (C.c_int, [_P, _P, _P]),
+    "ostc_store_last_step": (C.c_int, [_P, _P, _P]),
+    "ostc_residual_sums": (C.c_int, [_P, C.POINTER(_P), C.c_int, _D, _P]),
+    "ostc_zone_counters": (C.c_int, [_P, C.POINTER(C.c_longlong), C.c_int, _P]),
+    "ostc_steady_step": (C.c_int, [_P, C.POINTER(_P), C.c_int, C.POINTER(InnerFace), _I, C.c_int, C.c_int, _D,
+                                   _I, _P]),
+    "ostc_chem_plain_step": (C.c_int, [_P, C.c_long, _P, _P, _P, _P, _P, _P]),
+    "ostc_chem_source_batch": (C.c_int, [_P, C.c_long, _P, _P, _P, _P, _P, _P]),
+    "ostc_profile": (C.c_int, [_P, C.c_int]),
+    "ostc_profile_read": (C.c_int, [_P, _D, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
+    "ostc_profile_stage_name": (C.c_char_p, [C.c_int]),
+    "ostc_profile_stage_count": (C.c_int, []),
+    "ostc_device_sync": (C.c_int, []),
+    "ostc_last_error": (C.c_char_p, []),
+    "ostc_build_info": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load libostc.so and bind every exported entry point."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libostc.so is not built ({p}); run __graft_entry__.build() first -- "
+                           "the HIP path has no fallback")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class OstcError(RuntimeError):
+    pass
+
+
+def check(status: int, what: str = ""):
+    if status != OK:
+        msg = load().ostc_last_error().decode(errors="replace")
+        raise OstcError(f"{what} failed with status {status}: {msg}")

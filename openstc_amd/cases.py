@@ -1,0 +1,346 @@
+"""Synthetic configurations of the benchmark workloads.
+
+Each builder returns a ``Case``: grid, model constants, boundary program and
+initial field, laid out exactly as the reference's setup would produce them
+(inflow states: src/BoundCond.cpp:6-158; boundary ranges:
+src/Mesh.cpp:14-32 and src/BoundCond.cu:73-250; limit_flow bounds:
+src/DParameter.cu:78-100).  Perturbations use a counter hash (splitmix64) of
+(linear index, variable) so that any consumer of the same case sees the same
+bits.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+RU = 8314.462618
+MW_AIR = 28.9647
+GAMMA_AIR = 1.4
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    z = x
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    return z ^ (z >> np.uint64(31))
+
+
+def hash_uniform(seed: int, n: int, var: int) -> np.ndarray:
+    """u in [-1, 1) for cells 0..n-1 and variable `var`."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(n, dtype=np.uint64) * np.uint64(8) + np.uint64(var)
+        h = splitmix64(idx ^ np.uint64(seed))
+    return (h >> np.uint64(11)).astype(np.float64) / float(1 << 53) * 2.0 - 1.0
+
+
+def sutherland(t):
+    return 1.716e-5 * (t / 273) ** 1.5 * (273 + 111) / (t + 111)
+
+
+@dataclass
+class InflowState:
+    density: float
+    u: float
+    v: float
+    w: float
+    pressure: float
+    temperature: float
+    velocity: float
+    mut: float = 0.0
+    sv: list = field(default_factory=list)
+
+
+def air_inflow(mach, pressure, temperature, direction=(1.0, 0.0, 0.0), rans=False, intensity=0.01, mut_ratio=1.0):
+    """Inflow::Inflow for single-species air given (Ma, p, T)
+    (src/BoundCond.cpp:6-158)."""
+    mw = MW_AIR
+    viscosity = sutherland(temperature)
+    c = math.sqrt(GAMMA_AIR * RU / mw * temperature)
+    velocity = mach * c
+    u, v, w = (d * velocity for d in direction)
+    density = pressure * mw / (RU * temperature)
+    st = InflowState(density, u, v, w, pressure, temperature, velocity)
+    if rans:
+        st.mut = mut_ratio * viscosity
+        k = 1.5 * velocity * velocity * intensity * intensity
+        st.sv = [k, density * k / st.mut]
+    return st
+
+
+@dataclass
+class Block:
+    mx: int
+    my: int
+    mz: int
+    x: np.ndarray  # interior (mz, my, mx)
+    y: np.ndarray
+    z: np.ndarray
+    bc: L.BC = None
+    inner: list = field(default_factory=list)
+    xg: np.ndarray = None  # padded geometry / metrics (from libostc)
+    yg: np.ndarray = None
+    zg: np.ndarray = None
+    jac: np.ndarray = None
+    metric: np.ndarray = None
+    bv0: np.ndarray = None
+    sv0: np.ndarray = None
+
+
+@dataclass
+class Case:
+    name: str
+    param: L.Param
+    blocks: list
+    ngg: int = 2
+    precision: int = L.FP64
+    steps: int = 100
+    output_screen: int = 100
+    wall_points: np.ndarray = None
+    seed: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+def register_boundary(row, ngg=2):
+    """Boundary::register_boundary (src/Mesh.cpp:14-32), 3-D."""
+    i1, i2, j1, j2, k1, k2, label = row
+    rs, re = [i1, j1, k1], [i2, j2, k2]
+    face, direction = 0, 1
+    for d in range(3):
+        if rs[d] == re[d]:
+            face = d
+            if rs[d] == 0:
+                direction = -1
+            break
+    for d in range(3):
+        if d == face:
+            continue
+        if re[d] < rs[d]:
+            rs[d], re[d] = re[d], rs[d]
+        rs[d] -= ngg
+        re[d] += ngg
+    return dict(rs=rs, re=re, face=face, direction=direction, label=label)
+
+
+def bc_program(rows, bc_structs, inflows, walls, ngg=2):
+    """Face list in the reference's application order.
+
+    rows: boundary-file rows of one block.  bc_structs: list of (label, type)
+    in the order of the setup's boundary_conditions array.  Ordering follows
+    DBoundCond::initialize_bc_on_GPU / apply_boundary_conditions
+    (src/BoundCond.cu:73-155, src/BoundCond.cuh:321-412)."""
+    faces = [register_boundary(r, ngg) for r in rows]
+    labels = []
+    for f in faces:
+        if f["label"] not in labels:
+            labels.append(f["label"])
+    kind_of = {}
+    order = {"wall": [], "inflow": [], "outflow": [], "symmetry": []}
+    for lab, typ in bc_structs:
+        if lab in labels:
+            labels.remove(lab)
+            order[typ].append(lab)
+            kind_of[lab] = typ
+    for lab in labels:
+        typ = {2: "wall", 3: "symmetry", 5: "inflow", 6: "outflow"}.get(lab)
+        if typ:
+            order[typ].append(lab)
+            kind_of[lab] = typ
+    bc = L.BC()
+    n = 0
+    code = {"wall": 2, "symmetry": 3, "inflow": 5, "outflow": 6}
+    for typ in ("outflow", "inflow", "symmetry", "wall"):
+        for slot, lab in enumerate(order[typ]):
+            for f in faces:
+                if f["label"] != lab:
+                    continue
+                rs, re = list(f["rs"]), list(f["re"])
+                if typ == "wall":  # src/BoundCond.cu:215-230
+                    for q in range(3):
+                        if q != f["face"]:
+                            rs[q] += ngg
+                            re[q] -= ngg
+                fc = bc.face[n]
+                for q in range(3):
+                    fc.range_start[q] = rs[q]
+                    fc.range_end[q] = re[q]
+                fc.face, fc.direction, fc.kind, fc.slot = f["face"], f["direction"], code[typ], slot
+                n += 1
+    bc.n_face = n
+    for s, st in enumerate(inflows):
+        iv = bc.inflow[s]
+        iv.density, iv.u, iv.v, iv.w = st.density, st.u, st.v, st.w
+        iv.pressure, iv.temperature, iv.mut = st.pressure, st.temperature, st.mut
+        for l, val in enumerate(st.sv):
+            iv.sv[l] = val
+    for s, (iso, tw) in enumerate(walls):
+        bc.wall[s].isothermal = 1 if iso else 0
+        bc.wall[s].temperature = tw
+    return bc
+
+
+def air_param(rans: bool, viscous: bool, implicit: bool, dplur_steps: int, cfl: float, reconstruction=2,
+              freestream: InflowState = None) -> L.Param:
+    p = L.Param()
+    p.mix_model = L.AIR
+    p.turb_method = L.RANS if rans else L.LAMINAR
+    p.inviscid_scheme = 3
+    p.reconstruction = reconstruction
+    p.limiter = 0
+    p.viscous_scheme = 2 if viscous else 0
+    p.rans_model = 2 if rans else 0
+    p.turb_implicit = 1
+    p.chem_src_method = 0
+    p.implicit_method = 1 if implicit else 0
+    p.dplur_inner_step = dplur_steps
+    p.n_spec = 0
+    p.n_scalar = 2 if rans else 0
+    p.n_reac = 0
+    p.n_var = 5 + p.n_scalar
+    p.Pr, p.cfl, p.Sc, p.Prt, p.Sct = 0.72, cfl, 0.5, 0.9, 0.9
+    fs = freestream
+    # src/DParameter.cu:78-100
+    p.limit_ll[0] = 1e-6 * fs.density
+    p.limit_ul[0] = 1e+3 * fs.density
+    for l in range(1, 4):
+        p.limit_ll[l] = -1e+3 * fs.velocity
+        p.limit_ul[l] = 1e+3 * fs.velocity
+    p.limit_ll[4] = 1e-6 * fs.pressure
+    p.limit_ul[4] = 1e+3 * fs.pressure
+    if rans:
+        p.limit_ul[5] = np.finfo(np.float64).max
+        p.limit_ul[6] = np.finfo(np.float64).max
+    for l, v in enumerate(fs.sv):
+        p.sv_inf[l] = v
+    return p
+
+
+def _pad(a, ngg):
+    return np.pad(a, ((ngg, ngg), (ngg, ngg), (ngg, ngg)))
+
+
+def finish_block(blk: Block, ngg: int = 2):
+    """Attach geometry computed by libostc on the host."""
+    from .runtime import block_geometry
+    blk.xg, blk.yg, blk.zg, blk.jac, metric, nneg = block_geometry(blk.x, blk.y, blk.z, ngg)
+    blk.metric = metric
+    return blk
+
+
+def uniform_field(blk: Block, st: InflowState, nsc: int, ngg: int):
+    shape = (blk.mz + 2 * ngg, blk.my + 2 * ngg, blk.mx + 2 * ngg)
+    bv = np.empty((6,) + shape)
+    for l, v in enumerate((st.density, st.u, st.v, st.w, st.pressure, st.temperature)):
+        bv[l] = v
+    sv = np.empty((nsc,) + shape)
+    for l in range(nsc):
+        sv[l] = st.sv[l]
+    return bv, sv
+
+
+def perturb_interior(arr, blk: Block, seed: int, var: int, ngg: int, rel: float = None, absval: float = None):
+    """arr[interior] *= 1+rel*u  (or += absval*u), u from the counter hash."""
+    mz, my, mx = blk.mz, blk.my, blk.mx
+    u = hash_uniform(seed, mx * my * mz, var).reshape(mz, my, mx)
+    sl = (slice(ngg, ngg + mz), slice(ngg, ngg + my), slice(ngg, ngg + mx))
+    if rel is not None:
+        arr[sl] *= 1.0 + rel * u
+    else:
+        arr[sl] += absval * u
+
+
+def config1(mx=64, my=64, mz=8, steps=100, precision=L.FP64, seed=1) -> Case:
+    """Plain method, small: single-species air, laminar inviscid, one warped
+    64x64x8 block, Mach 2 freestream (p=101325 Pa, T=300 K) with +-1% seeded
+    perturbation of rho,u,v,w,p; MUSCL+minmod+AUSM+, explicit Euler, 100 steps;
+    inflow at i=0, outflow at i=max, symmetry elsewhere."""
+    ngg = 2
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    xi, eta, zeta = i / (mx - 1), j / (my - 1), k / (mz - 1)
+    tp = 2 * np.pi
+    x = xi + 0.05 * np.sin(tp * eta) * np.sin(tp * zeta)
+    y = eta + 0.05 * np.sin(tp * zeta) * np.sin(tp * xi)
+    z = zeta + 0.05 * np.sin(tp * xi) * np.sin(tp * eta)
+    fs = air_inflow(2.0, 101325.0, 300.0)
+    param = air_param(rans=False, viscous=False, implicit=False, dplur_steps=0, cfl=0.5, reconstruction=2,
+                      freestream=fs)
+    blk = Block(mx, my, mz, x, y, z)
+    rows = [(0, 0, 0, my - 1, 0, mz - 1, 5), (mx - 1, mx - 1, 0, my - 1, 0, mz - 1, 6),
+            (0, mx - 1, 0, 0, 0, mz - 1, 3), (0, mx - 1, my - 1, my - 1, 0, mz - 1, 3),
+            (0, mx - 1, 0, my - 1, 0, 0, 3), (0, mx - 1, 0, my - 1, mz - 1, mz - 1, 3)]
+    blk.bc = bc_program(rows, [(5, "inflow")], [fs], [], ngg)
+    finish_block(blk, ngg)
+    bv, sv = uniform_field(blk, fs, 0, ngg)
+    perturb_interior(bv[0], blk, seed, 0, ngg, rel=0.01)
+    perturb_interior(bv[1], blk, seed, 1, ngg, rel=0.01)
+    perturb_interior(bv[2], blk, seed, 2, ngg, absval=0.01 * fs.u)
+    perturb_interior(bv[3], blk, seed, 3, ngg, absval=0.01 * fs.u)
+    perturb_interior(bv[4], blk, seed, 4, ngg, rel=0.01)
+    # T from the equation of state of the perturbed state
+    bv[5] = bv[4] * MW_AIR / (bv[0] * RU)
+    blk.bv0, blk.sv0 = bv, sv
+    return Case("config1_air_inviscid_explicit", param, [blk], ngg, precision, steps, steps, None, seed,
+                dict(mach=2.0, p=101325.0, T=300.0))
+
+
+def _tanh_stretch(n, height, first):
+    """y_j = H*(1 - tanh(beta*(1-s))/tanh(beta)), s=j/(n-1), beta such that y_1 = first."""
+    s1 = 1.0 / (n - 1)
+
+    def y1(beta):
+        return height * (1 - math.tanh(beta * (1 - s1)) / math.tanh(beta))
+
+    lo, hi = 1e-3, 20.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if y1(mid) > first:
+            lo = mid
+        else:
+            hi = mid
+    beta = 0.5 * (lo + hi)
+    s = np.arange(n) / (n - 1)
+    return height * (1 - np.tanh(beta * (1 - s)) / math.tanh(beta)), beta
+
+
+def config2(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, cfl=5.0, height=0.1,
+            span=0.25, seed=2) -> Case:
+    """Viscous RANS: air, k-omega SST, flat plate with tanh wall clustering
+    (first cell y+~1 from the turbulent flat-plate Cf correlation at x=0.5 m),
+    Mach 2.5, Re=1e7/m, T=220 K, adiabatic no-slip wall at j=0, inflow i=0,
+    outflow i=max and j=max, symmetry in k; MUSCL+AUSM+ + central viscous
+    flux, DPLUR with 4 sweeps, CFL 5."""
+    ngg = 2
+    t_inf, mach, re_m = 220.0, 2.5, 1e7
+    mu = sutherland(t_inf)
+    c = math.sqrt(GAMMA_AIR * RU / MW_AIR * t_inf)
+    u_inf = mach * c
+    rho = re_m * mu / u_inf
+    p_inf = rho * RU * t_inf / MW_AIR
+    fs = air_inflow(mach, p_inf, t_inf, rans=True, intensity=0.01, mut_ratio=1.0)
+    cf = 0.0592 * (re_m * 0.5) ** -0.2
+    u_tau = u_inf * math.sqrt(cf / 2)
+    y_first = (mu / rho) / u_tau
+    yline, beta = _tanh_stretch(my, height, y_first)
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    x = i / (mx - 1) * 1.0
+    y = yline[j]
+    z = k / (mz - 1) * span
+    param = air_param(rans=True, viscous=True, implicit=True, dplur_steps=dplur_steps, cfl=cfl, reconstruction=2,
+                      freestream=fs)
+    blk = Block(mx, my, mz, x, y, z)
+    rows = [(0, 0, 0, my - 1, 0, mz - 1, 5), (mx - 1, mx - 1, 0, my - 1, 0, mz - 1, 6),
+            (0, mx - 1, 0, 0, 0, mz - 1, 2), (0, mx - 1, my - 1, my - 1, 0, mz - 1, 6),
+            (0, mx - 1, 0, my - 1, 0, 0, 3), (0, mx - 1, 0, my - 1, mz - 1, mz - 1, 3)]
+    blk.bc = bc_program(rows, [(5, "inflow"), (2, "wall")], [fs], [(False, 300.0)], ngg)
+    finish_block(blk, ngg)
+    bv, sv = uniform_field(blk, fs, 2, ngg)
+    blk.bv0, blk.sv0 = bv, sv
+    # wall points: the trimmed wall face (src/Driver.cu:128-149)
+    wall = np.stack([x[:, 0, :].ravel(), y[:, 0, :].ravel(), z[:, 0, :].ravel()], axis=1)
+    return Case("config2_air_sst_dplur", param, [blk], ngg, precision, steps, 100, wall, seed,
+                dict(mach=mach, p=p_inf, T=t_inf, rho=rho, u=u_inf, y_first=y_first, beta=beta))

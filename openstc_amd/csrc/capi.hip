@@ -1,0 +1,671 @@
+// C-ABI of the residual-and-update path: contexts, zones, field I/O and the
+// per-stage entry points declared in include/ostc.h.
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ostc_kernels.h"
+
+using namespace ostc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char *where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return OSTC_ERR_HIP;
+}
+#define HIP_TRY(call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+  } while (0)
+
+// Event-based stage profiler (one active context at a time).
+struct Profiler {
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> pool;
+  std::vector<Rec> recs;
+  hipEvent_t open[ostc::P_COUNT] = {};
+  size_t next = 0;
+  hipEvent_t get() {
+    if (next == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[next++];
+  }
+  ~Profiler() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+Profiler *g_prof = nullptr;
+
+}  // namespace
+
+namespace ostc {
+void prof_mark(int stage, hipStream_t st, bool begin) {
+  if (!g_prof || stage < 0 || stage >= P_COUNT) return;
+  hipEvent_t e = g_prof->get();
+  if (!e) return;
+  (void)hipEventRecord(e, st);
+  if (begin) {
+    g_prof->open[stage] = e;
+  } else if (g_prof->open[stage]) {
+    g_prof->recs.push_back({stage, g_prof->open[stage], e});
+    g_prof->open[stage] = nullptr;
+  }
+}
+}  // namespace ostc
+
+struct ostc_ctx {
+  ostc_param host{};
+  ostc_param *dev = nullptr;
+  CellOpts opts{};
+  double *partial = nullptr;  // residual partial sums
+  size_t partial_cap = 0;      // in doubles
+  double *out4 = nullptr;      // device [4]
+  double *host4 = nullptr;     // pinned host [4]
+};
+
+struct ostc_zone {
+  ostc_ctx *ctx = nullptr;
+  ostc_zone_desc desc{};
+  int64_t plane = 0;
+  int esize = 8;
+  void *ptr[OSTC_F_COUNT] = {};
+  int planes[OSTC_F_COUNT] = {};
+  int fsize[OSTC_F_COUNT] = {};  // element size per field
+  int *piv = nullptr;
+  unsigned long long *counters = nullptr;
+  ostc_bc bc{};
+  bool has_bc = false;
+  ostc_inflow *d_inflow = nullptr;
+  ZoneView<float> vf{};
+  ZoneView<double> vd{};
+};
+
+namespace {
+
+template <typename R>
+ZoneView<R> make_view(ostc_zone *z) {
+  ZoneView<R> v{};
+  v.mx = z->desc.mx;
+  v.my = z->desc.my;
+  v.mz = z->desc.mz;
+  v.g = z->desc.ngg;
+  v.nx = v.mx + 2 * v.g;
+  v.ny = v.my + 2 * v.g;
+  v.nz = v.mz + 2 * v.g;
+  v.plane = z->plane;
+  auto P = [&](int f) { return static_cast<R *>(z->ptr[f]); };
+  v.x = static_cast<const double *>(z->ptr[OSTC_F_X]);
+  v.y = static_cast<const double *>(z->ptr[OSTC_F_Y]);
+  v.z = static_cast<const double *>(z->ptr[OSTC_F_Z]);
+  v.jac = P(OSTC_F_JAC);
+  v.metric = P(OSTC_F_METRIC);
+  v.wall_dist = P(OSTC_F_WALL_DIST);
+  v.bv = P(OSTC_F_BV);
+  v.sv = P(OSTC_F_SV);
+  v.cv = P(OSTC_F_CV);
+  v.bv_last = P(OSTC_F_BV_LAST);
+  v.vel = P(OSTC_F_VEL);
+  v.sound = P(OSTC_F_SOUND);
+  v.mach = P(OSTC_F_MACH);
+  v.mul = P(OSTC_F_MUL);
+  v.lambda = P(OSTC_F_LAMBDA);
+  v.rho_d = P(OSTC_F_RHO_D);
+  v.gamma = P(OSTC_F_GAMMA);
+  v.cp = P(OSTC_F_CP);
+  v.mut = P(OSTC_F_MUT);
+  v.lambda_t = P(OSTC_F_LAMBDA_T);
+  v.turb_jac = P(OSTC_F_TURB_JAC);
+  v.dq = P(OSTC_F_DQ);
+  v.dq0 = P(OSTC_F_DQ0);
+  v.dqk = P(OSTC_F_DQK);
+  v.spectr = P(OSTC_F_SPECTR);
+  v.dt = P(OSTC_F_DT);
+  v.chem_jac = static_cast<double *>(z->ptr[OSTC_F_CHEM_JAC]);
+  v.chem_lu = v.chem_jac;  // factors overwrite the Jacobian (it is not needed afterwards)
+  v.chem_piv = z->piv;
+  v.counters = z->counters;
+  return v;
+}
+
+bool fp64(const ostc_zone *z) { return z->desc.precision == OSTC_FP64; }
+hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+
+int check_param(const ostc_param &p) {
+  if (p.mix_model < 0 || p.mix_model > 2) return fail(OSTC_ERR_ARG, "mix_model must be 0..2");
+  if (p.turb_method == OSTC_RANS && p.rans_model != 2) return fail(OSTC_ERR_ARG, "only the k-omega SST RANS model");
+  if (p.mix_model != OSTC_AIR && p.n_spec != 9)
+    return fail(OSTC_ERR_ARG, "mixtures are instantiated for 9 species");
+  if (p.mix_model == OSTC_AIR && p.n_spec != 0) return fail(OSTC_ERR_ARG, "air has no species");
+  const int nt = p.turb_method == OSTC_RANS ? 2 : 0;
+  if (p.n_scalar != p.n_spec + nt || p.n_var != 5 + p.n_spec + nt)
+    return fail(OSTC_ERR_ARG, "n_scalar/n_var inconsistent with n_spec and the turbulence model");
+  if (p.limiter != 0 && p.limiter != 1) return fail(OSTC_ERR_ARG, "only minmod limiters (0, 1)");
+  if (p.n_reac > OSTC_MAX_REAC || p.n_spec > OSTC_MAX_SPEC) return fail(OSTC_ERR_ARG, "mechanism too large");
+  if (p.implicit_method != 0 && p.implicit_method != 1) return fail(OSTC_ERR_ARG, "implicit_method 0 or 1");
+  return OSTC_OK;
+}
+
+}  // namespace
+
+namespace {
+// the increment left by the last implicit treatment, per zone
+template <typename R>
+int implicit_and_update(ostc_ctx *c, const ZoneView<R> &v, hipStream_t st, bool do_update) {
+  R *res = nullptr;
+  if (int r = implicit_treatment(v, c->dev, c->opts, st, &res)) return r;
+  if (!do_update) {
+    if (res != v.dq) {
+      const size_t bytes = (size_t)v.plane * c->host.n_var * sizeof(R);
+      if (hipMemcpyAsync(v.dq, res, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) return OSTC_ERR_HIP;
+    }
+    return OSTC_OK;
+  }
+  return update_cv_bv(v, c->dev, c->opts, res, st);
+}
+}  // namespace
+
+extern "C" {
+
+const char *ostc_last_error(void) { return g_err.c_str(); }
+
+const char *ostc_build_info(void) {
+  return "openstc_amd libostc: HIP gfx950, FP32/FP64 zones, species instantiations {0, 9}";
+}
+
+int ostc_sizeof_param(void) { return (int)sizeof(ostc_param); }
+int ostc_sizeof_bc(void) { return (int)sizeof(ostc_bc); }
+
+int ostc_device_sync(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return OSTC_OK;
+}
+
+int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
+  if (!param || !out) return fail(OSTC_ERR_ARG, "null argument");
+  if (int r = check_param(*param)) return r;
+  auto *c = new (std::nothrow) ostc_ctx;
+  if (!c) return fail(OSTC_ERR_ALLOC, "host allocation");
+  c->host = *param;
+  hipError_t e = hipMalloc(&c->dev, sizeof(ostc_param));
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipMalloc(param)");
+  }
+  e = hipMemcpy(c->dev, param, sizeof(ostc_param), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&c->out4, 4 * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&c->host4, 4 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ostc_ctx_destroy(c);
+    return hip_fail(e, "ctx setup");
+  }
+  CellOpts &o = c->opts;
+  o.ns = param->n_spec;
+  o.nt = param->turb_method == OSTC_RANS ? 2 : 0;
+  o.mix_model = param->mix_model;
+  o.chem = param->chem_src_method;
+  o.rans_model = param->rans_model;
+  o.turb_implicit = param->turb_implicit;
+  o.implicit_method = param->implicit_method;
+  o.dplur_steps = param->dplur_inner_step;
+  *out = c;
+  return OSTC_OK;
+}
+
+int ostc_ctx_destroy(ostc_ctx *ctx) {
+  if (!ctx) return OSTC_OK;
+  if (ctx->dev) (void)hipFree(ctx->dev);
+  if (ctx->partial) (void)hipFree(ctx->partial);
+  if (ctx->out4) (void)hipFree(ctx->out4);
+  if (ctx->host4) (void)hipHostFree(ctx->host4);
+  delete ctx;
+  return OSTC_OK;
+}
+
+int ostc_zone_create(ostc_ctx *ctx, const ostc_zone_desc *d, ostc_zone **out) {
+  if (!ctx || !d || !out) return fail(OSTC_ERR_ARG, "null argument");
+  if (d->mx < 2 || d->my < 2 || d->mz < 2) return fail(OSTC_ERR_ARG, "3-D blocks with mx,my,mz >= 2 only");
+  if (d->ngg != 2) return fail(OSTC_ERR_ARG, "ngg must be 2 (MUSCL/NND-2 stencils)");
+  if (d->precision != OSTC_FP32 && d->precision != OSTC_FP64) return fail(OSTC_ERR_ARG, "precision 4 or 8");
+  auto *z = new (std::nothrow) ostc_zone;
+  if (!z) return fail(OSTC_ERR_ALLOC, "host allocation");
+  z->ctx = ctx;
+  z->desc = *d;
+  const int64_t nx = d->mx + 2 * d->ngg, ny = d->my + 2 * d->ngg, nz = d->mz + 2 * d->ngg;
+  z->plane = nx * ny * nz;
+  z->esize = d->precision;
+  const ostc_param &p = ctx->host;
+  const int ns = p.n_spec, nv = p.n_var, nsc = p.n_scalar;
+  const bool rans = p.turb_method == OSTC_RANS;
+  const bool fr = p.mix_model == OSTC_FR;
+  const bool dplur = p.implicit_method == 1;
+  int pl[OSTC_F_COUNT] = {};
+  pl[OSTC_F_X] = pl[OSTC_F_Y] = pl[OSTC_F_Z] = 1;
+  pl[OSTC_F_JAC] = 1;
+  pl[OSTC_F_METRIC] = 9;
+  pl[OSTC_F_WALL_DIST] = rans ? 1 : 0;
+  pl[OSTC_F_BV] = 6;
+  pl[OSTC_F_SV] = nsc;
+  pl[OSTC_F_CV] = nv;
+  pl[OSTC_F_BV_LAST] = 4;
+  pl[OSTC_F_VEL] = pl[OSTC_F_SOUND] = pl[OSTC_F_MACH] = pl[OSTC_F_MUL] = pl[OSTC_F_LAMBDA] = 1;
+  pl[OSTC_F_RHO_D] = ns;
+  pl[OSTC_F_GAMMA] = pl[OSTC_F_CP] = ns > 0 ? 1 : 0;
+  pl[OSTC_F_CHEM_JAC] = fr ? (p.chem_src_method == 1 ? ns * ns : (p.chem_src_method == 2 ? ns : 0)) : 0;
+  pl[OSTC_F_MUT] = pl[OSTC_F_LAMBDA_T] = rans ? 1 : 0;
+  pl[OSTC_F_TURB_JAC] = rans ? 2 : 0;
+  pl[OSTC_F_DQ] = nv;
+  pl[OSTC_F_DQ0] = pl[OSTC_F_DQK] = dplur ? nv : 0;
+  pl[OSTC_F_SPECTR] = 3;
+  pl[OSTC_F_DT] = 1;
+  for (int f = 0; f < OSTC_F_COUNT; ++f) {
+    z->planes[f] = pl[f];
+    z->fsize[f] = (f == OSTC_F_X || f == OSTC_F_Y || f == OSTC_F_Z || f == OSTC_F_CHEM_JAC) ? 8 : z->esize;
+    if (pl[f] == 0) continue;
+    const size_t bytes = (size_t)z->plane * pl[f] * z->fsize[f];
+    hipError_t e = hipMalloc(&z->ptr[f], bytes);
+    if (e == hipSuccess) e = hipMemset(z->ptr[f], 0, bytes);
+    if (e != hipSuccess) {
+      ostc_zone_destroy(z);
+      return hip_fail(e, "hipMalloc(zone field)");
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (fr && p.chem_src_method == 1 && dplur) {
+    e = hipMalloc(&z->piv, (size_t)z->plane * ns * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(z->piv, 0, (size_t)z->plane * ns * sizeof(int));
+  }
+  if (e == hipSuccess) e = hipMalloc(&z->counters, 4 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(z->counters, 0, 4 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&z->d_inflow, OSTC_MAX_BCDATA * sizeof(ostc_inflow));
+  if (e != hipSuccess) {
+    ostc_zone_destroy(z);
+    return hip_fail(e, "zone setup");
+  }
+  z->vf = make_view<float>(z);
+  z->vd = make_view<double>(z);
+  *out = z;
+  return OSTC_OK;
+}
+
+int ostc_zone_destroy(ostc_zone *z) {
+  if (!z) return OSTC_OK;
+  for (int f = 0; f < OSTC_F_COUNT; ++f)
+    if (z->ptr[f]) (void)hipFree(z->ptr[f]);
+  if (z->piv) (void)hipFree(z->piv);
+  if (z->counters) (void)hipFree(z->counters);
+  if (z->d_inflow) (void)hipFree(z->d_inflow);
+  delete z;
+  return OSTC_OK;
+}
+
+long ostc_zone_plane(const ostc_zone *z) { return z ? (long)z->plane : 0; }
+
+int ostc_zone_field_planes(const ostc_zone *z, int f, int *n) {
+  if (!z || f < 0 || f >= OSTC_F_COUNT || !n) return fail(OSTC_ERR_ARG, "bad field");
+  *n = z->planes[f];
+  return OSTC_OK;
+}
+
+int ostc_zone_field_ptr(const ostc_zone *z, int f, void **dev, int *es) {
+  if (!z || f < 0 || f >= OSTC_F_COUNT) return fail(OSTC_ERR_ARG, "bad field");
+  if (dev) *dev = z->ptr[f];
+  if (es) *es = z->fsize[f];
+  return OSTC_OK;
+}
+
+int ostc_zone_upload(ostc_zone *z, int f, const double *host, int n_planes) {
+  if (!z || f < 0 || f >= OSTC_F_COUNT || !host) return fail(OSTC_ERR_ARG, "bad field");
+  if (n_planes > z->planes[f] || n_planes < 0) return fail(OSTC_ERR_ARG, "too many planes for this field");
+  const size_t n = (size_t)z->plane * n_planes;
+  if (n == 0) return OSTC_OK;
+  if (z->fsize[f] == 8) {
+    HIP_TRY(hipMemcpy(z->ptr[f], host, n * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(n);
+    for (size_t q = 0; q < n; ++q) tmp[q] = (float)host[q];
+    HIP_TRY(hipMemcpy(z->ptr[f], tmp.data(), n * 4, hipMemcpyHostToDevice));
+  }
+  return OSTC_OK;
+}
+
+int ostc_zone_download(const ostc_zone *z, int f, double *host, int n_planes) {
+  if (!z || f < 0 || f >= OSTC_F_COUNT || !host) return fail(OSTC_ERR_ARG, "bad field");
+  if (n_planes > z->planes[f] || n_planes < 0) return fail(OSTC_ERR_ARG, "too many planes for this field");
+  const size_t n = (size_t)z->plane * n_planes;
+  if (n == 0) return OSTC_OK;
+  HIP_TRY(hipDeviceSynchronize());
+  if (z->fsize[f] == 8) {
+    HIP_TRY(hipMemcpy(host, z->ptr[f], n * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(n);
+    HIP_TRY(hipMemcpy(tmp.data(), z->ptr[f], n * 4, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; ++q) host[q] = tmp[q];
+  }
+  return OSTC_OK;
+}
+
+int ostc_zone_set_bc(ostc_zone *z, const ostc_bc *bc) {
+  if (!z || !bc) return fail(OSTC_ERR_ARG, "null argument");
+  if (bc->n_face < 0 || bc->n_face > OSTC_MAX_FACE) return fail(OSTC_ERR_ARG, "n_face out of range");
+  const int g = z->desc.ngg;
+  const int lo[3] = {-g, -g, -g}, hi[3] = {z->desc.mx - 1 + g, z->desc.my - 1 + g, z->desc.mz - 1 + g};
+  for (int f = 0; f < bc->n_face; ++f) {
+    const ostc_face &b = bc->face[f];
+    if (b.face < 0 || b.face > 2 || (b.direction != 1 && b.direction != -1))
+      return fail(OSTC_ERR_ARG, "bad face orientation");
+    for (int q = 0; q < 3; ++q)
+      if (b.range_start[q] < lo[q] || b.range_end[q] > hi[q] || b.range_start[q] > b.range_end[q])
+        return fail(OSTC_ERR_ARG, "face range outside the padded block");
+    const int ext[3] = {z->desc.mx, z->desc.my, z->desc.mz};
+    const int at = b.range_start[b.face];
+    if (b.range_end[b.face] != at || (b.direction < 0 ? at != 0 : at != ext[b.face] - 1))
+      return fail(OSTC_ERR_ARG, "face must lie on the block boundary");
+    if ((b.kind == 5 || b.kind == 2) && (b.slot < 0 || b.slot >= OSTC_MAX_BCDATA))
+      return fail(OSTC_ERR_ARG, "bad BC slot");
+    if (b.kind != 2 && b.kind != 3 && b.kind != 5 && b.kind != 6) return fail(OSTC_ERR_ARG, "unknown BC kind");
+  }
+  z->bc = *bc;
+  z->has_bc = true;
+  HIP_TRY(hipMemcpy(z->d_inflow, bc->inflow, sizeof(bc->inflow), hipMemcpyHostToDevice));
+  return OSTC_OK;
+}
+
+#define ZONE_CALL(z, EXPR_F, EXPR_D) (fp64(z) ? (EXPR_D) : (EXPR_F))
+
+int ostc_compute_cv_from_bv(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  return ZONE_CALL(z, cv_from_bv(z->vf, c->dev, c->opts, S(s)), cv_from_bv(z->vd, c->dev, c->opts, S(s)));
+}
+
+int ostc_initialize_mut(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  return ZONE_CALL(z, initialize_mut(z->vf, c->dev, c->opts, S(s)), initialize_mut(z->vd, c->dev, c->opts, S(s)));
+}
+
+int ostc_wall_distance(ostc_ctx *c, ostc_zone *z, const double *wall_xyz, int n, void *s) {
+  if (!c || !z || (n > 0 && !wall_xyz) || n < 0) return fail(OSTC_ERR_ARG, "bad wall points");
+  if (!z->ptr[OSTC_F_WALL_DIST]) return fail(OSTC_ERR_STATE, "zone has no wall distance (laminar)");
+  double *d = nullptr;
+  if (n > 0) {
+    HIP_TRY(hipMalloc(&d, (size_t)n * 3 * sizeof(double)));
+    HIP_TRY(hipMemcpy(d, wall_xyz, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
+  }
+  int r = ZONE_CALL(z, wall_distance(z->vf, d, n, S(s)), wall_distance(z->vd, d, n, S(s)));
+  HIP_TRY(hipStreamSynchronize(S(s)));
+  if (d) (void)hipFree(d);
+  return r;
+}
+
+int ostc_compute_source(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  bool wrote = false;
+  return ZONE_CALL(z, source_terms(z->vf, c->dev, c->opts, S(s), true, &wrote),
+                   source_terms(z->vd, c->dev, c->opts, S(s), true, &wrote));
+}
+
+int ostc_inviscid_flux(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  const int ns = c->opts.ns, nt = c->opts.nt;
+  return ZONE_CALL(z, flux_sweeps(z->vf, c->dev, ns, nt, true, false, false, S(s)),
+                   flux_sweeps(z->vd, c->dev, ns, nt, true, false, false, S(s)));
+}
+
+int ostc_viscous_flux(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  if (c->host.viscous_scheme == 0) return OSTC_OK;
+  const int ns = c->opts.ns, nt = c->opts.nt;
+  return ZONE_CALL(z, flux_sweeps(z->vf, c->dev, ns, nt, false, true, false, S(s)),
+                   flux_sweeps(z->vd, c->dev, ns, nt, false, true, false, S(s)));
+}
+
+int ostc_local_time_step(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  return ZONE_CALL(z, local_time_step(z->vf, c->dev, c->opts, S(s)), local_time_step(z->vd, c->dev, c->opts, S(s)));
+}
+
+
+int ostc_implicit_treatment(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  // leaves the final increment in dq, as the reference does after its copies
+  return ZONE_CALL(z, implicit_and_update(c, z->vf, S(s), false), implicit_and_update(c, z->vd, S(s), false));
+}
+
+int ostc_update_cv_bv(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  return ZONE_CALL(z, update_cv_bv(z->vf, c->dev, c->opts, (const float *)z->vf.dq, S(s)),
+                   update_cv_bv(z->vd, c->dev, c->opts, (const double *)z->vd.dq, S(s)));
+}
+
+int ostc_limit_flow(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  return ZONE_CALL(z, limit_flow(z->vf, c->dev, c->opts, S(s)), limit_flow(z->vd, c->dev, c->opts, S(s)));
+}
+
+int ostc_apply_bc(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  if (!z->has_bc) return OSTC_OK;
+  return ZONE_CALL(z, apply_bc(z->vf, c->dev, c->opts, z->bc, z->d_inflow, S(s)),
+                   apply_bc(z->vd, c->dev, c->opts, z->bc, z->d_inflow, S(s)));
+}
+
+int ostc_inner_halo(ostc_ctx *c, ostc_zone **zones, int blk, const ostc_inner_face *f, void *s) {
+  if (!c || !zones || !f || blk < 0) return fail(OSTC_ERR_ARG, "null argument");
+  ostc_zone *z = zones[blk], *t = zones[f->target_block];
+  if (!z || !t || z->desc.precision != t->desc.precision) return fail(OSTC_ERR_ARG, "bad inner face blocks");
+  return ZONE_CALL(z, inner_halo(z->vf, t->vf, c->dev, c->opts, *f, S(s)),
+                   inner_halo(z->vd, t->vd, c->dev, c->opts, *f, S(s)));
+}
+
+int ostc_update_physical_properties(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  return ZONE_CALL(z, physical_properties(z->vf, c->dev, c->opts, S(s)),
+                   physical_properties(z->vd, c->dev, c->opts, S(s)));
+}
+
+int ostc_store_last_step(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  return ZONE_CALL(z, store_last_step(z->vf, S(s)), store_last_step(z->vd, S(s)));
+}
+
+int ostc_residual_sums(ostc_ctx *c, ostc_zone **zones, int n, double sums[4], void *s) {
+  if (!c || !zones || n <= 0 || !sums) return fail(OSTC_ERR_ARG, "null argument");
+  size_t need = 0;
+  for (int b = 0; b < n; ++b) {
+    const auto g = cell_grid(zones[b]->desc.mx, zones[b]->desc.my, zones[b]->desc.mz);
+    need += (size_t)g.grid.x * g.grid.y * g.grid.z * 4;
+  }
+  if (need > c->partial_cap) {
+    HIP_TRY(hipStreamSynchronize(S(s)));
+    if (c->partial) (void)hipFree(c->partial);
+    c->partial = nullptr;
+    HIP_TRY(hipMalloc(&c->partial, need * sizeof(double)));
+    c->partial_cap = need;
+  }
+  int off = 0;
+  for (int b = 0; b < n; ++b) {
+    int nb = 0;
+    ostc_zone *z = zones[b];
+    int r = ZONE_CALL(z, residual_partials(z->vf, c->partial + (size_t)off * 4, S(s), &nb),
+                      residual_partials(z->vd, c->partial + (size_t)off * 4, S(s), &nb));
+    if (r) return r;
+    off += nb;
+  }
+  if (int r = residual_finalize(c->partial, off, c->out4, S(s))) return r;
+  HIP_TRY(hipMemcpyAsync(c->host4, c->out4, 4 * sizeof(double), hipMemcpyDeviceToHost, S(s)));
+  HIP_TRY(hipStreamSynchronize(S(s)));
+  for (int l = 0; l < 4; ++l) sums[l] = c->host4[l];
+  return OSTC_OK;
+}
+
+int ostc_zone_counters(ostc_zone *z, long long out[4], int reset, void *s) {
+  if (!z || !out) return fail(OSTC_ERR_ARG, "null argument");
+  unsigned long long h[4];
+  HIP_TRY(hipStreamSynchronize(S(s)));
+  HIP_TRY(hipMemcpy(h, z->counters, sizeof(h), hipMemcpyDeviceToHost));
+  for (int l = 0; l < 4; ++l) out[l] = (long long)h[l];
+  if (reset) HIP_TRY(hipMemset(z->counters, 0, sizeof(h)));
+  return OSTC_OK;
+}
+
+int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_face *inner, const int *n_inner,
+                     int step, int output_screen, double res_sums[4], int *res_computed, void *s) {
+  if (!c || !zones || n <= 0 || output_screen <= 0) return fail(OSTC_ERR_ARG, "bad arguments");
+  const hipStream_t st = S(s);
+  const CellOpts &o = c->opts;
+  const bool visc = c->host.viscous_scheme != 0;
+  for (int b = 0; b < n; ++b) {
+    ostc_zone *z = zones[b];
+    int r = OSTC_OK;
+    if (step % output_screen == 0) {
+      prof_mark(P_STORE, st, true);
+      r = ostc_store_last_step(c, z, s);
+      prof_mark(P_STORE, st, false);
+    }
+    if (r) return r;
+    bool wrote = false;
+    prof_mark(P_SOURCE, st, true);
+    r = ZONE_CALL(z, source_terms(z->vf, c->dev, o, st, false, &wrote),
+                  source_terms(z->vd, c->dev, o, st, false, &wrote));
+    prof_mark(P_SOURCE, st, false);
+    if (r) return r;
+    r = ZONE_CALL(z, flux_sweeps(z->vf, c->dev, o.ns, o.nt, true, visc, !wrote, st),
+                  flux_sweeps(z->vd, c->dev, o.ns, o.nt, true, visc, !wrote, st));
+    if (r) return r;
+    prof_mark(P_DT, st, true);
+    r = ZONE_CALL(z, local_time_step(z->vf, c->dev, o, st), local_time_step(z->vd, c->dev, o, st));
+    prof_mark(P_DT, st, false);
+    if (r) return r;
+    if (o.implicit_method == 0) prof_mark(P_EXPL_IMPL, st, true);
+    {
+      // implicit treatment, then the update with the T-from-e solve
+      const bool dplur = o.implicit_method == 1;
+      if (fp64(z)) {
+        double *res = nullptr;
+        r = implicit_treatment(z->vd, c->dev, o, st, &res);
+        if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
+        if (r) return r;
+        prof_mark(P_UPDATE, st, true);
+        r = update_cv_bv(z->vd, c->dev, o, res, st);
+      } else {
+        float *res = nullptr;
+        r = implicit_treatment(z->vf, c->dev, o, st, &res);
+        if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
+        if (r) return r;
+        prof_mark(P_UPDATE, st, true);
+        r = update_cv_bv(z->vf, c->dev, o, res, st);
+      }
+      prof_mark(P_UPDATE, st, false);
+    }
+    if (r) return r;
+    prof_mark(P_LIMIT, st, true);
+    r = ZONE_CALL(z, limit_flow(z->vf, c->dev, o, st), limit_flow(z->vd, c->dev, o, st));
+    prof_mark(P_LIMIT, st, false);
+    if (r) return r;
+    prof_mark(P_BC, st, true);
+    r = ostc_apply_bc(c, z, s);
+    prof_mark(P_BC, st, false);
+    if (r) return r;
+  }
+  if (inner && n_inner) {
+    prof_mark(P_HALO, st, true);
+    int off = 0;
+    for (int b = 0; b < n; ++b) {
+      for (int f = 0; f < n_inner[b]; ++f)
+        if (int r = ostc_inner_halo(c, zones, b, &inner[off + f], s)) return r;
+      off += n_inner[b];
+    }
+    prof_mark(P_HALO, st, false);
+  }
+  prof_mark(P_PROPS, st, true);
+  for (int b = 0; b < n; ++b)
+    if (int r = ostc_update_physical_properties(c, zones[b], s)) return r;
+  prof_mark(P_PROPS, st, false);
+  const bool do_res = (step % output_screen == 0) || step == 1;
+  if (res_computed) *res_computed = do_res ? 1 : 0;
+  if (do_res && res_sums) {
+    prof_mark(P_RESID, st, true);
+    int r = ostc_residual_sums(c, zones, n, res_sums, s);
+    prof_mark(P_RESID, st, false);
+    return r;
+  }
+  return OSTC_OK;
+}
+
+static const char *kStageNames[P_COUNT] = {"source", "flux_i", "flux_j", "flux_k", "local_dt", "dplur_dq0",
+                                           "dplur_sweep", "explicit_implicit", "update", "limit_flow", "bc",
+                                           "inner_halo", "properties", "store_last", "residual"};
+
+int ostc_profile(ostc_ctx *c, int enable) {
+  (void)c;
+  if (enable) {
+    if (!g_prof) g_prof = new Profiler;
+    g_prof->recs.clear();
+    g_prof->next = 0;
+  } else if (g_prof) {
+    delete g_prof;
+    g_prof = nullptr;
+  }
+  return OSTC_OK;
+}
+
+int ostc_profile_read(ostc_ctx *c, double *ms, long long *count, int n, int reset) {
+  (void)c;
+  if (!ms || !count || n <= 0) return fail(OSTC_ERR_ARG, "bad arguments");
+  for (int q = 0; q < n; ++q) {
+    ms[q] = 0;
+    count[q] = 0;
+  }
+  if (!g_prof) return OSTC_OK;
+  for (auto &r : g_prof->recs) {
+    HIP_TRY(hipEventSynchronize(r.b));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+    if (r.stage < n) {
+      ms[r.stage] += t;
+      count[r.stage] += 1;
+    }
+  }
+  if (reset) {
+    g_prof->recs.clear();
+    g_prof->next = 0;
+  }
+  return OSTC_OK;
+}
+
+const char *ostc_profile_stage_name(int stage) {
+  return (stage >= 0 && stage < P_COUNT) ? kStageNames[stage] : "";
+}
+
+int ostc_profile_stage_count(void) { return P_COUNT; }
+
+int ostc_chem_plain_step(ostc_ctx *c, long n, const double *t, const double *p, const double *y, const double *dt,
+                         double *dy, void *s) {
+  if (!c || n < 0 || (n > 0 && (!t || !p || !y || !dt || !dy))) return fail(OSTC_ERR_ARG, "bad arguments");
+  if (c->host.n_spec != 9 || c->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
+  return chem_plain_step(c->dev, c->host.n_spec, c->host.n_reac, n, t, p, y, dt, dy, S(s));
+}
+
+int ostc_chem_source_batch(ostc_ctx *c, long n, const double *rho, const double *t, const double *y, double *omega,
+                           double *jac, void *s) {
+  if (!c || n < 0 || (n > 0 && (!rho || !t || !y || !omega))) return fail(OSTC_ERR_ARG, "bad arguments");
+  if (c->host.n_spec != 9) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
+  return chem_source_batch(c->dev, c->host.n_spec, c->host.n_reac, n, rho, t, y, omega, jac, S(s));
+}
+
+}  // extern "C"

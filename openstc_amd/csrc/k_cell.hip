@@ -1,0 +1,991 @@
+// Cell-parallel stages of the per-iteration path: properties, sources,
+// local time step, point-implicit / DPLUR treatment, update with the
+// T-from-e Newton solve, limit_flow, residual reduction, wall distance.
+#include "ostc_kernels.h"
+#include "ostc_thermo.h"
+
+namespace ostc {
+
+namespace {
+
+template <typename R>
+struct CellIdx {
+  int i, j, k;
+  bool ok;
+};
+
+// interior + `halo` layers, workgroups of 64 x 4 over (i, j)
+template <typename R>
+__device__ __forceinline__ CellIdx<R> cell_index(const ZoneView<R> &z, int halo) {
+  CellIdx<R> c;
+  c.i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - halo;
+  c.j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - halo;
+  c.k = (int)blockIdx.z - halo;
+  c.ok = c.i < z.mx + halo && c.j < z.my + halo && c.k < z.mz + halo;
+  return c;
+}
+
+// ------------------------------------------------------------------ setup
+
+// compute_cv_from_bv (src/FieldOperation.cuh:37-73) incl. compute_total_energy
+// (src/FieldOperation.cuh:15-35); all points incl. ghosts
+template <typename R, int NS, int NT>
+__global__ __launch_bounds__(256) void k_cv_from_bv(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  const auto c = cell_index(z, z.g);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const R rho = z.bv[id], u = z.bv[P + id], v = z.bv[2 * P + id], w = z.bv[3 * P + id];
+  z.cv[id] = rho;
+  z.cv[P + id] = rho * u;
+  z.cv[2 * P + id] = rho * v;
+  z.cv[3 * P + id] = rho * w;
+#pragma unroll
+  for (int l = 0; l < NS + NT; ++l) z.cv[(5 + l) * P + id] = rho * z.sv[l * P + id];
+  const R vel2 = u * u + v * v + w * w;
+  R e = (R)0.5 * rho * vel2;
+  if constexpr (NS > 0) {
+    R h[NS];
+    enthalpy<R>(z.bv[5 * P + id], h, p, NS);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) e += h[l] * z.cv[(5 + l) * P + id];
+    e -= z.bv[4 * P + id];
+  } else {
+    e += z.bv[4 * P + id] / (R)(kGammaAir - 1);
+  }
+  z.cv[4 * P + id] = e;
+  z.vel[id] = sqrt(vel2);
+}
+
+// update_physical_properties (src/FieldOperation.cuh:75-109); interior + 1
+template <typename R, int NS>
+__global__ __launch_bounds__(256) void k_props(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  const auto c = cell_index(z, 1);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const R t = z.bv[5 * P + id];
+  if constexpr (NS > 0) {
+    R cp[NS], y[NS];
+    heat_capacity<R>(t, cp, p, NS);
+    R mw = 0, cp_tot = 0, cvv = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      y[l] = z.sv[l * P + id];
+      mw += y[l] * (R)(1.0 / p->mw[l]);
+      cp_tot += y[l] * cp[l];
+      cvv += y[l] * (cp[l] - (R)(kRu / p->mw[l]));
+    }
+    mw = (R)1 / mw;
+    const R gam = cp_tot / cvv;
+    z.cp[id] = cp_tot;
+    z.gamma[id] = gam;
+    const R snd = sqrt(gam * (R)kRu * t / mw);
+    z.sound[id] = snd;
+    R mu, lam, rhod[NS];
+    mixture_transport<R>(p, NS, t, mw, y, cp, &mu, &lam, rhod);
+    z.mul[id] = mu;
+    z.lambda[id] = lam;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.rho_d[l * P + id] = rhod[l];
+    z.mach[id] = z.vel[id] / snd;
+  } else {
+    const R c_temp = (R)(kGammaAir * kRu / kMwAir);
+    const R cp = (R)(kGammaAir * kRu / kMwAir / (kGammaAir - 1));
+    const R snd = sqrt(c_temp * t);
+    z.sound[id] = snd;
+    const R mu = sutherland<R>(t);
+    z.mul[id] = mu;
+    z.lambda[id] = mu * cp / (R)p->Pr;
+    z.mach[id] = z.vel[id] / snd;
+  }
+}
+
+// velocity-gradient helper: 0.5*(m_1c*d_i + m_2c*d_j + m_3c*d_k) of plane f
+template <typename R>
+__device__ __forceinline__ R cgrad(const R *__restrict__ f, int64_t id, const int64_t s[3], R m1, R m2, R m3) {
+  return (R)0.5 * (m1 * (f[id + s[0]] - f[id - s[0]]) + m2 * (f[id + s[1]] - f[id - s[1]]) +
+                   m3 * (f[id + s[2]] - f[id - s[2]]));
+}
+
+// SST::compute_mut (src/SST.cu:6-58): rho*k from cv; interior + 1
+template <typename R, int NS>
+__global__ __launch_bounds__(256) void k_init_mut(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  const auto c = cell_index(z, 1);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const int64_t s[3] = {1, (int64_t)z.nx, (int64_t)z.nx * z.ny};
+  const R t = z.bv[5 * P + id];
+  R mul = sutherland<R>(t);
+  if constexpr (NS > 0) {
+    R y[NS], mw = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      y[l] = z.sv[l * P + id];
+      mw += y[l] * (R)(1.0 / p->mw[l]);
+    }
+    mw = (R)1 / mw;
+    mixture_transport<R>(p, NS, t, mw, y, (const R *)nullptr, &mul, (R *)nullptr, (R *)nullptr);
+  }
+  R m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = z.metric[q * P + id];
+  const R *u = z.bv + P, *v = z.bv + 2 * P, *w = z.bv + 3 * P;
+  const R u_y = cgrad(u, id, s, m[1], m[4], m[7]), u_z = cgrad(u, id, s, m[2], m[5], m[8]);
+  const R v_x = cgrad(v, id, s, m[0], m[3], m[6]), v_z = cgrad(v, id, s, m[2], m[5], m[8]);
+  const R w_x = cgrad(w, id, s, m[0], m[3], m[6]), w_y = cgrad(w, id, s, m[1], m[4], m[7]);
+  const R rhoK = z.cv[(NS + 5) * P + id];
+  const R tke = z.sv[NS * P + id], omega = z.sv[(NS + 1) * P + id];
+  const R vort = sqrt((v_x - u_y) * (v_x - u_y) + (w_x - u_z) * (w_x - u_z) + (w_y - v_z) * (w_y - v_z));
+  const R rho = z.bv[id];
+  R f2 = 1;
+  const R dy = z.wall_dist[id];
+  if (dy > (R)1e-25) {
+    const R param1 = (R)2 * sqrt(tke) / ((R)0.09 * omega * dy);
+    const R param2 = (R)500 * mul / (rho * dy * dy * omega);
+    const R arg2 = dmax(param1, param2);
+    f2 = tanh(arg2 * arg2);
+  }
+  R mut = 0;
+  const R den = dmax((R)sst::a_1 * omega, vort * f2);
+  if (den > (R)1e-25) mut = (R)sst::a_1 * rhoK / den;
+  z.mut[id] = mut;
+}
+
+// ------------------------------------------------------------------ source
+
+// compute_source (src/SourceTerm.cuh:11-40): SST source and mut
+// (src/SST.cu:60-178), turbulent conductivity, finite-rate chemistry
+// (src/FiniteRateChem.cu:7-63, FP64).  First writer of dq.
+template <typename R, int NS, int NT, bool CHEM>
+__global__ __launch_bounds__(256) void k_source(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  constexpr int NV = 5 + NS + NT;
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  R dq[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) dq[l] = 0;
+  const R J = z.jac[id];
+  if constexpr (NT > 0) {
+    const int64_t s[3] = {1, (int64_t)z.nx, (int64_t)z.nx * z.ny};
+    R m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = z.metric[q * P + id];
+    const R *u = z.bv + P, *v = z.bv + 2 * P, *w = z.bv + 3 * P;
+    const R u_x = cgrad(u, id, s, m[0], m[3], m[6]), u_y = cgrad(u, id, s, m[1], m[4], m[7]),
+            u_z = cgrad(u, id, s, m[2], m[5], m[8]);
+    const R v_x = cgrad(v, id, s, m[0], m[3], m[6]), v_y = cgrad(v, id, s, m[1], m[4], m[7]),
+            v_z = cgrad(v, id, s, m[2], m[5], m[8]);
+    const R w_x = cgrad(w, id, s, m[0], m[3], m[6]), w_y = cgrad(w, id, s, m[1], m[4], m[7]),
+            w_z = cgrad(w, id, s, m[2], m[5], m[8]);
+    const R rho = z.bv[id];
+    const R *kf = z.sv + NS * P, *of = z.sv + (NS + 1) * P;
+    const R omega = of[id], tke = kf[id];
+    const R k_x = cgrad(kf, id, s, m[0], m[3], m[6]), k_y = cgrad(kf, id, s, m[1], m[4], m[7]),
+            k_z = cgrad(kf, id, s, m[2], m[5], m[8]);
+    const R o_x = cgrad(of, id, s, m[0], m[3], m[6]), o_y = cgrad(of, id, s, m[1], m[4], m[7]),
+            o_z = cgrad(of, id, s, m[2], m[5], m[8]);
+    const R inter = (R)2 * rho * (R)sst::sigma_omega2 / omega * (k_x * o_x + k_y * o_y + k_z * o_z);
+    const R rhoK = rho * tke;
+    const R vort = sqrt((v_x - u_y) * (v_x - u_y) + (w_x - u_z) * (w_x - u_z) + (w_y - v_z) * (w_y - v_z));
+    R f1 = 1, f2 = 1;
+    const R dy = z.wall_dist[id];
+    if (dy > (R)1e-25) {
+      const R param1 = sqrt(tke) / ((R)0.09 * omega * dy);
+      const R d2 = dy * dy;
+      const R param2 = (R)500 * z.mul[id] / (rho * d2 * omega);
+      const R arg2 = dmax((R)2 * param1, param2);
+      f2 = tanh(arg2 * arg2);
+      const R cd = dmax((R)1e-20, inter);
+      const R param3 = (R)4 * rho * (R)sst::sigma_omega2 * tke / (cd * d2);
+      const R arg1 = dmin(dmax(param1, param2), param3);
+      f1 = tanh(arg1 * arg1 * arg1 * arg1);
+    }
+    R mut = 0;
+    const R den = dmax((R)sst::a_1 * omega, vort * f2);
+    if (den > (R)1e-25) mut = (R)sst::a_1 * rhoK / den;
+    z.mut[id] = mut;
+    const R beta = (R)sst::beta_2 + (R)sst::delta_beta * f1;
+    if (mut > (R)1e-25) {
+      const R divU = u_x + v_y + w_z;
+      const R prod_k = mut * ((R)2 * (u_x * u_x + v_y * v_y + w_z * w_z) - (R)(2.0 / 3) * divU * divU +
+                              (u_y + v_x) * (u_y + v_x) + (u_z + w_x) * (u_z + w_x) + (v_z + w_y) * (v_z + w_y)) -
+                       (R)(2.0 / 3) * rhoK * divU;
+      const R diss_k = (R)sst::beta_star * rhoK * omega;
+      dq[NS + 5] = J * (prod_k - diss_k);
+      const R gam = (R)sst::gamma2 + (R)sst::delta_gamma * f1;
+      const R prod_o = gam * rho / mut * prod_k + ((R)1 - f1) * inter;
+      const R diss_o = beta * rho * omega * omega;
+      dq[NS + 6] = J * (prod_o - diss_o);
+    }
+    if (p->turb_implicit == 1) {
+      z.turb_jac[id] = (R)(-2 * sst::beta_star) * omega;
+      z.turb_jac[P + id] = (R)-2 * beta * omega;
+    }
+    if constexpr (NS > 0)
+      z.lambda_t[id] = mut * z.cp[id] / (R)p->Prt;
+    else
+      z.lambda_t[id] = mut * (R)(kGammaAir * kRu / kMwAir / (kGammaAir - 1)) / (R)p->Prt;
+  }
+  if constexpr (CHEM && NS > 0) {
+    double y[NS], omega[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) y[l] = (double)z.sv[l * P + id];
+    const double rho = (double)z.bv[id], t = (double)z.bv[5 * P + id];
+    const int meth = p->chem_src_method;
+    if (meth == 1) {
+      double jac[NS * NS];
+      chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, t, y, omega, jac, nullptr);
+#pragma unroll
+      for (int q = 0; q < NS * NS; ++q) z.chem_jac[q * P + id] = jac[q];
+    } else if (meth == 2) {
+      double jd[NS];
+      chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, t, y, omega, nullptr, jd);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) z.chem_jac[q * P + id] = jd[q];
+    } else {
+      chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, t, y, omega, nullptr, nullptr);
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) dq[5 + l] = (R)((double)J * omega[l]);
+  }
+#pragma unroll
+  for (int l = 0; l < NV; ++l) z.dq[l * P + id] = dq[l];
+}
+
+// ------------------------------------------------------------------ dt
+
+// local_time_step (src/TimeAdvanceFunc.cuh:21-71); the viscous radius is not
+// stored (the reference never reads visc_spectr_rad)
+template <typename R, int NS, int NT>
+__global__ __launch_bounds__(256) void k_local_dt(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  R m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = z.metric[q * P + id];
+  const R gx = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const R ge = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+  const R gz = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+  const R u = z.bv[P + id], v = z.bv[2 * P + id], w = z.bv[3 * P + id];
+  const R U = u * m[0] + v * m[1] + w * m[2];
+  const R V = u * m[3] + v * m[4] + w * m[5];
+  const R W = u * m[6] + v * m[7] + w * m[8];
+  const R a = z.sound[id];
+  const bool three = z.mz > 1;
+  const R r0 = fabs(U) + a * gx, r1 = fabs(V) + a * ge, r2 = three ? fabs(W) + a * gz : (R)0;
+  z.spectr[id] = r0;
+  z.spectr[P + id] = r1;
+  z.spectr[2 * P + id] = r2;
+  const R sri = r0 + r1 + r2;
+  R gam = (R)kGammaAir;
+  if constexpr (NS > 0) gam = z.gamma[id];
+  const R c1 = dmax(gam, (R)(4.0 / 3.0)) / z.bv[id];
+  R c2 = z.mul[id] / (R)p->Pr;
+  if constexpr (NT > 0) c2 += z.mut[id] / (R)p->Prt;
+  R srv = gx * gx + ge * ge;
+  if (three) srv += gz * gz;
+  srv *= c1 * c2;
+  z.dt[id] = (R)p->cfl / (sri + srv);
+}
+
+// ------------------------------------------------------------------ implicit
+
+// explicit-mode point-implicit treatment (src/ImplicitTreatmentHPP.cuh:10-59,
+// EPI src/FiniteRateChem.cu:211-234, DA :400-412, SST src/SST.cu:180-194)
+template <typename R, int NS, int NT, bool CHEM>
+__global__ __launch_bounds__(256) void k_explicit_implicit(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const R dt = z.dt[id];
+  if constexpr (CHEM && NS > 0) {
+    const double dtd = (double)dt;
+    if (p->chem_src_method == 1) {
+      double a[NS * NS], b[NS];
+      int piv[NS];
+#pragma unroll
+      for (int q = 0; q < NS * NS; ++q) a[q] = -dtd * z.chem_jac[q * P + id];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        a[q * NS + q] = 1.0 - dtd * z.chem_jac[(q * NS + q) * P + id];
+        b[q] = (double)z.dq[(5 + q) * P + id];
+      }
+      lu_factor<NS>(a, piv, NS);
+      lu_solve<NS>(a, piv, b, NS);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) z.dq[(5 + q) * P + id] = (R)b[q];
+    } else if (p->chem_src_method == 2) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+        z.dq[(5 + q) * P + id] = (R)((double)z.dq[(5 + q) * P + id] / (1.0 - dtd * z.chem_jac[q * P + id]));
+    }
+  }
+  if constexpr (NT > 0) {
+    if (p->turb_implicit == 1) {
+      z.dq[(NS + 5) * P + id] /= (R)1 - dt * z.turb_jac[id];
+      z.dq[(NS + 6) * P + id] /= (R)1 - dt * z.turb_jac[P + id];
+    }
+  }
+}
+
+// compute_DQ_0 (src/DPLURHPP.cuh:13-76): dq0 = dq / diag with the chemistry
+// (EPI_for_dq0, src/FiniteRateChem.cu:236-253, LU factors kept for the sweeps)
+// and SST (src/SST.cu:196-204) blocks treated point-implicitly.
+template <typename R, int NS, int NT, bool CHEM>
+__global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const R dt = z.dt[id];
+  const R diag = (R)1 + dt * (z.spectr[id] + z.spectr[P + id] + z.spectr[2 * P + id]);
+  const int nflow = (CHEM && NS > 0) ? 5 : 5 + NS;
+  for (int l = 0; l < nflow; ++l) z.dq0[l * P + id] = z.dq[l * P + id] / diag;
+  if constexpr (CHEM && NS > 0) {
+    const double dtd = (double)dt, dg = (double)diag;
+    const int meth = p->chem_src_method;
+    if (meth == 1) {
+      double a[NS * NS], b[NS];
+      int piv[NS];
+#pragma unroll
+      for (int q = 0; q < NS * NS; ++q) a[q] = -dtd * z.chem_jac[q * P + id];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        a[q * NS + q] = dg - dtd * z.chem_jac[(q * NS + q) * P + id];
+        b[q] = (double)z.dq[(5 + q) * P + id];
+      }
+      lu_factor<NS>(a, piv, NS);
+#pragma unroll
+      for (int q = 0; q < NS * NS; ++q) z.chem_lu[q * P + id] = a[q];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) z.chem_piv[q * P + id] = piv[q];
+      lu_solve<NS>(a, piv, b, NS);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) z.dq0[(5 + q) * P + id] = (R)b[q];
+    } else if (meth == 2) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+        z.dq0[(5 + q) * P + id] = (R)((double)z.dq[(5 + q) * P + id] / (dg - dtd * z.chem_jac[q * P + id]));
+    } else {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) z.dq0[(5 + q) * P + id] = z.dq[(5 + q) * P + id] / diag;
+    }
+  }
+  if constexpr (NT > 0) {
+    if (p->turb_implicit == 1) {
+      z.dq0[(NS + 5) * P + id] = z.dq[(NS + 5) * P + id] / (diag - dt * z.turb_jac[id]);
+      z.dq0[(NS + 6) * P + id] = z.dq[(NS + 6) * P + id] / (diag - dt * z.turb_jac[P + id]);
+    } else {
+      z.dq0[(NS + 5) * P + id] = z.dq[(NS + 5) * P + id] / diag;
+      z.dq0[(NS + 6) * P + id] = z.dq[(NS + 6) * P + id] / diag;
+    }
+  }
+}
+
+// compute_jacobian_times_dq (src/DPLURHPP.cuh:78-125) at neighbour nb,
+// accumulated into acc with weight sgn*0.5
+template <typename R, int NS, int NT>
+__device__ __forceinline__ void jac_times_dq(const ZoneView<R> &z, const ostc_param *__restrict__ p,
+                                             const R *__restrict__ din, int64_t nb, int dir, R pm_sr, R w,
+                                             R *acc) {
+  constexpr int NV = 5 + NS + NT;
+  const int64_t P = z.plane;
+  const R xi_x = z.metric[(dir * 3) * P + nb], xi_y = z.metric[(dir * 3 + 1) * P + nb],
+          xi_z = z.metric[(dir * 3 + 2) * P + nb];
+  const R u = z.bv[P + nb], v = z.bv[2 * P + nb], wv = z.bv[3 * P + nb];
+  const R U = xi_x * u + xi_y * v + xi_z * wv;
+  const R lmd1 = U + pm_sr;
+  const R vel = z.vel[nb];
+  const R e = (R)0.5 * vel * vel;
+  R d[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) d[l] = din[l * P + nb];
+  R gamma = (R)kGammaAir, b3 = 0, b4 = 0, h = 0;
+  if constexpr (NS > 0) {
+    R hk[NS];
+    const R t = z.bv[5 * P + nb];
+    enthalpy<R>(t, hk, p, NS);
+    gamma = z.gamma[nb];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      b3 += (R)kRu * t / (R)p->mw[l] * d[5 + l];
+      b4 += hk[l] * d[5 + l];
+      h += z.sv[l * P + nb] * hk[l];
+    }
+    b3 *= gamma;
+    b4 *= gamma - (R)1;
+    h += e;
+  } else {
+    h = gamma / (gamma - (R)1) * z.bv[4 * P + nb] / z.bv[nb] + e;
+  }
+  const R b1 = xi_x * d[1] + xi_y * d[2] + xi_z * d[3] - U * d[0];
+  const R b2 = (gamma - (R)1) * (e * d[0] - u * d[1] - v * d[2] - wv * d[3] + d[4]);
+  acc[0] += w * (b1 + lmd1 * d[0]);
+  acc[1] += w * (u * b1 + xi_x * b2 + lmd1 * d[1] + xi_x * (b3 - b4));
+  acc[2] += w * (v * b1 + xi_y * b2 + lmd1 * d[2] + xi_y * (b3 - b4));
+  acc[3] += w * (wv * b1 + xi_z * b2 + lmd1 * d[3] + xi_z * (b3 - b4));
+  acc[4] += w * (h * b1 + U * b2 + lmd1 * d[4] + U * (b3 - b4));
+#pragma unroll
+  for (int l = 0; l < NS + NT; ++l) acc[5 + l] += w * (lmd1 * d[5 + l] + z.sv[l * P + nb] * b1);
+}
+
+// DPLUR_inner_iteration (src/DPLURHPP.cuh:127-249): one Jacobi sweep,
+// din -> dout (dq0 read for the cell itself); EPI_for_dqk
+// (src/FiniteRateChem.cu:255-284) with the LU factors from k_dq0.
+template <typename R, int NS, int NT, bool CHEM>
+__global__ __launch_bounds__(256) void k_dplur(ZoneView<R> z, const ostc_param *__restrict__ p,
+                                               const R *__restrict__ din, R *__restrict__ dout) {
+  constexpr int NV = 5 + NS + NT;
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const int64_t s[3] = {1, (int64_t)z.nx, (int64_t)z.nx * z.ny};
+  R acc[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) acc[l] = 0;
+  const R *sr = z.spectr;
+  if (c.i > 0) jac_times_dq<R, NS, NT>(z, p, din, id - s[0], 0, sr[id - s[0]], (R)0.5, acc);
+  if (c.j > 0) jac_times_dq<R, NS, NT>(z, p, din, id - s[1], 1, sr[P + id - s[1]], (R)0.5, acc);
+  if (c.k > 0) jac_times_dq<R, NS, NT>(z, p, din, id - s[2], 2, sr[2 * P + id - s[2]], (R)0.5, acc);
+  if (c.i != z.mx - 1) jac_times_dq<R, NS, NT>(z, p, din, id + s[0], 0, -sr[id + s[0]], (R)-0.5, acc);
+  if (c.j != z.my - 1) jac_times_dq<R, NS, NT>(z, p, din, id + s[1], 1, -sr[P + id + s[1]], (R)-0.5, acc);
+  if (c.k != z.mz - 1) jac_times_dq<R, NS, NT>(z, p, din, id + s[2], 2, -sr[2 * P + id + s[2]], (R)-0.5, acc);
+  const R dt = z.dt[id];
+  const R diag = (R)1 + dt * (sr[id] + sr[P + id] + sr[2 * P + id]);
+  const int nflow = (CHEM && NS > 0) ? 5 : 5 + NS;
+  for (int l = 0; l < nflow; ++l) dout[l * P + id] = z.dq0[l * P + id] + dt * acc[l] / diag;
+  if constexpr (CHEM && NS > 0) {
+    const int meth = p->chem_src_method;
+    const double dtd = (double)dt, dg = (double)diag;
+    if (meth == 1) {
+      double a[NS * NS], b[NS];
+      int piv[NS];
+#pragma unroll
+      for (int q = 0; q < NS * NS; ++q) a[q] = z.chem_lu[q * P + id];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        piv[q] = z.chem_piv[q * P + id];
+        b[q] = (double)acc[5 + q] * dtd;
+      }
+      lu_solve<NS>(a, piv, b, NS);
+#pragma unroll
+      for (int q = 0; q < NS; ++q) dout[(5 + q) * P + id] = (R)((double)z.dq0[(5 + q) * P + id] + b[q]);
+    } else if (meth == 2) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+        dout[(5 + q) * P + id] =
+            (R)((double)z.dq0[(5 + q) * P + id] + dtd * (double)acc[5 + q] / (dg - dtd * z.chem_jac[q * P + id]));
+    } else {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) dout[(5 + q) * P + id] = z.dq0[(5 + q) * P + id] + dt * acc[5 + q] / diag;
+    }
+  }
+  if constexpr (NT > 0) {
+    if (p->turb_implicit == 1) {
+      dout[(NS + 5) * P + id] = z.dq0[(NS + 5) * P + id] + dt * acc[5 + NS] / (diag - dt * z.turb_jac[id]);
+      dout[(NS + 6) * P + id] = z.dq0[(NS + 6) * P + id] + dt * acc[6 + NS] / (diag - dt * z.turb_jac[P + id]);
+    } else {
+      dout[(NS + 5) * P + id] = z.dq0[(NS + 5) * P + id] + dt * acc[5 + NS] / diag;
+      dout[(NS + 6) * P + id] = z.dq0[(NS + 6) * P + id] + dt * acc[6 + NS] / diag;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ update
+
+// T from internal energy by the reference's Newton iteration
+// (src/FieldOperation.cu:3-40), FP64; returns iterations used
+template <int NS>
+__device__ __forceinline__ int newton_t(const ostc_param *__restrict__ p, const double *y, double gas_const,
+                                        double e, double &t) {
+  double err = 1;
+  int iter = 0;
+  while (err > 1e-3 && iter++ < 1000) {
+    double h[NS], cp[NS];
+    enthalpy_cp<double>(t, h, cp, p, NS);
+    double cp_tot = 0, hm = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cp_tot += cp[l] * y[l];
+      hm += h[l] * y[l];
+    }
+    const double e_t = hm - gas_const * t;
+    const double cvv = cp_tot - gas_const;
+    const double t1 = t - (e_t - e) / cvv;
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  return iter;
+}
+
+// update_cv_and_bv (src/FieldOperation.cuh:139-183)
+template <typename R, int NS, int NT>
+__global__ __launch_bounds__(256) void k_update(ZoneView<R> z, const ostc_param *__restrict__ p,
+                                                const R *__restrict__ dqf) {
+  constexpr int NV = 5 + NS + NT;
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const R dtj = z.dt[id] / z.jac[id];
+  R cv[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    cv[l] = z.cv[l * P + id] + dqf[l * P + id] * dtj;
+    z.cv[l * P + id] = cv[l];
+  }
+  const R rho = cv[0];
+  const R rinv = (R)1.0 / rho;
+  const R u = cv[1] * rinv, v = cv[2] * rinv, w = cv[3] * rinv;
+  z.bv[id] = rho;
+  z.bv[P + id] = u;
+  z.bv[2 * P + id] = v;
+  z.bv[3 * P + id] = w;
+  const R vel2 = u * u + v * v + w * w;
+#pragma unroll
+  for (int l = 0; l < NS + NT; ++l) z.sv[l * P + id] = cv[5 + l] * rinv;
+  if constexpr (NS > 0) {
+    double y[NS], mw = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      y[l] = (double)(cv[5 + l] * rinv);
+      mw += y[l] / p->mw[l];
+    }
+    mw = 1 / mw;
+    const double gas_const = kRu / mw;
+    const double ud = (double)u, vd = (double)v, wd = (double)w;
+    const double e = (double)cv[4] / (double)cv[0] - 0.5 * (ud * ud + vd * vd + wd * wd);
+    double t = (double)z.bv[5 * P + id];
+    const int it = newton_t<NS>(p, y, gas_const, e, t);
+    if (it > 1000) atomicAdd(&z.counters[0], 1ull);
+    z.bv[5 * P + id] = (R)t;
+    z.bv[4 * P + id] = (R)((double)rho * t * gas_const);
+  } else {
+    const R pr = (R)(kGammaAir - 1) * (cv[4] - (R)0.5 * rho * vel2);
+    z.bv[4 * P + id] = pr;
+    z.bv[5 * P + id] = pr * (R)kMwAir * rinv / (R)kRu;
+  }
+  z.vel[id] = sqrt(vel2);
+}
+
+// ------------------------------------------------------------------ limit_flow
+
+// limit_flow (src/TimeAdvanceFunc.cu:34-224) with the reference's i-neighbour
+// bound by mz (quirk kept for parity)
+template <typename R, int NS, int NT>
+__global__ __launch_bounds__(256) void k_limit_flow(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const double *ll = p->limit_ll, *ul = p->limit_ul;
+  R var[5];
+  bool bad = false;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) {
+    var[l] = z.bv[l * P + id];
+    if (isnan(var[l]) || var[l] < (R)ll[l] || var[l] > (R)ul[l]) bad = true;
+  }
+  if (bad) {
+    atomicAdd(&z.counters[1], 1ull);
+    R uv[5 + NS + NT];
+#pragma unroll
+    for (int l = 0; l < 5 + NS + NT; ++l) uv[l] = 0;
+    int kn = 0;
+    for (int ka = -1; ka < 2; ++ka) {
+      const int k1 = c.k + ka;
+      if (k1 < 0 || k1 >= z.mz) continue;
+      for (int ja = -1; ja < 2; ++ja) {
+        const int j1 = c.j + ja;
+        if (j1 < 0 || j1 >= z.my) continue;
+        for (int ia = -1; ia < 2; ++ia) {
+          const int i1 = c.i + ia;
+          if (i1 < 0 || i1 >= z.mz) continue;
+          const int64_t n1 = z.idx(i1, j1, k1);
+          bool nbad = false;
+          R nb[5];
+#pragma unroll
+          for (int l = 0; l < 5; ++l) {
+            nb[l] = z.bv[l * P + n1];
+            if (isnan(nb[l]) || nb[l] < (R)ll[l] || nb[l] > (R)ul[l]) nbad = true;
+          }
+          if (nbad) continue;
+#pragma unroll
+          for (int l = 0; l < 5; ++l) uv[l] += nb[l];
+#pragma unroll
+          for (int l = 0; l < NS + NT; ++l) uv[5 + l] += z.sv[l * P + n1];
+          ++kn;
+        }
+      }
+    }
+    if (kn > 0) {
+      const R kinv = (R)1.0 / (R)kn;
+#pragma unroll
+      for (int l = 0; l < 5 + NS + NT; ++l) uv[l] *= kinv;
+    } else {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) uv[l] = dmin(dmax(var[l], (R)ll[l]), (R)ul[l]);
+#pragma unroll
+      for (int l = 0; l < NS + NT; ++l) uv[5 + l] = (R)p->sv_inf[l];
+    }
+#pragma unroll
+    for (int l = 0; l < 5; ++l) z.bv[l * P + id] = uv[l];
+    z.cv[id] = uv[0];
+    z.cv[P + id] = uv[0] * uv[1];
+    z.cv[2 * P + id] = uv[0] * uv[2];
+    z.cv[3 * P + id] = uv[0] * uv[3];
+    R e = (R)0.5 * uv[0] * (uv[1] * uv[1] + uv[2] * uv[2] + uv[3] * uv[3]);
+#pragma unroll
+    for (int l = 0; l < NS + NT; ++l) {
+      z.sv[l * P + id] = uv[5 + l];
+      z.cv[(5 + l) * P + id] = uv[0] * uv[5 + l];
+    }
+    if constexpr (NS > 0) {
+      R mw = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) mw += uv[5 + l] * (R)(1.0 / p->mw[l]);
+      mw = (R)1 / mw;
+      const R t = uv[4] * mw / (uv[0] * (R)kRu);
+      z.bv[5 * P + id] = t;
+      R h[NS];
+      enthalpy<R>(t, h, p, NS);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) e += h[l] * uv[0] * uv[5 + l];
+      e -= uv[4];
+    } else {
+      z.bv[5 * P + id] = uv[4] * (R)kMwAir / (uv[0] * (R)kRu);
+      e += uv[4] / (R)(kGammaAir - 1);
+    }
+    z.cv[4 * P + id] = e;
+  }
+  if constexpr (NT > 0) {
+    if (p->rans_model == 2) {
+      const R t0 = z.sv[NS * P + id], t1 = z.sv[(NS + 1) * P + id];
+      if (isnan(t0) || isnan(t1) || t0 < (R)0 || t1 < (R)0) {
+        R a0 = 0, a1 = 0;
+        int kn = 0;
+        for (int ka = -1; ka < 2; ++ka) {
+          const int k1 = c.k + ka;
+          if (k1 < 0 || k1 >= z.mz) continue;
+          for (int ja = -1; ja < 2; ++ja) {
+            const int j1 = c.j + ja;
+            if (j1 < 0 || j1 >= z.my) continue;
+            for (int ia = -1; ia < 2; ++ia) {
+              const int i1 = c.i + ia;
+              if (i1 < 0 || i1 >= z.mz) continue;
+              const int64_t n1 = z.idx(i1, j1, k1);
+              const R x0 = z.sv[NS * P + n1], x1 = z.sv[(NS + 1) * P + n1];
+              if (isnan(x0) || isnan(x1) || x0 < (R)0 || x1 < (R)0) continue;
+              a0 += x0;
+              a1 += x1;
+              ++kn;
+            }
+          }
+        }
+        if (kn > 0) {
+          const R kinv = (R)1.0 / (R)kn;
+          a0 *= kinv;
+          a1 *= kinv;
+        } else {
+          a0 = t0 < (R)0 ? (R)p->sv_inf[NS] : t0;
+          a1 = t1 < (R)0 ? (R)p->sv_inf[NS + 1] : t1;
+        }
+        z.sv[NS * P + id] = a0;
+        z.sv[(NS + 1) * P + id] = a1;
+        const R rho = z.cv[id];
+        z.cv[(NS + 5) * P + id] = rho * a0;
+        z.cv[(NS + 6) * P + id] = rho * a1;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ residual
+
+// store_last_step (src/TimeAdvanceFunc.cu:5-16)
+template <typename R>
+__global__ __launch_bounds__(256) void k_store_last(ZoneView<R> z) {
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  z.bv_last[id] = z.bv[id];
+  z.bv_last[P + id] = z.vel[id];
+  z.bv_last[2 * P + id] = z.bv[4 * P + id];
+  z.bv_last[3 * P + id] = z.bv[5 * P + id];
+}
+
+// compute_square_of_dbv + first reduction pass (src/TimeAdvanceFunc.cu:18-32,
+// src/Driver.cu:417-459): per-workgroup FP64 sums, fixed order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_residual_partial(ZoneView<R> z, double *__restrict__ partial) {
+  const auto c = cell_index(z, 0);
+  double s[4] = {0, 0, 0, 0};
+  if (c.ok) {
+    const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+    double d;
+    d = (double)z.bv[id] - (double)z.bv_last[id];
+    s[0] = d * d;
+    d = (double)z.vel[id] - (double)z.bv_last[P + id];
+    s[1] = d * d;
+    d = (double)z.bv[4 * P + id] - (double)z.bv_last[2 * P + id];
+    s[2] = d * d;
+    d = (double)z.bv[5 * P + id] - (double)z.bv_last[3 * P + id];
+    s[3] = d * d;
+  }
+  __shared__ double red[4][256];
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) red[l][t] = s[l];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int l = 0; l < 4; ++l) red[l][t] += red[l][t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const int64_t b = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) partial[b * 4 + l] = red[l][0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_residual_final(const double *__restrict__ partial, int n,
+                                                        double *__restrict__ out) {
+  __shared__ double red[4][256];
+  const int t = threadIdx.x;
+  double s[4] = {0, 0, 0, 0};
+  for (int b = t; b < n; b += 256)
+#pragma unroll
+    for (int l = 0; l < 4; ++l) s[l] += partial[b * 4 + l];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) red[l][t] = s[l];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int l = 0; l < 4; ++l) red[l][t] += red[l][t + w];
+    __syncthreads();
+  }
+  if (t == 0)
+#pragma unroll
+    for (int l = 0; l < 4; ++l) out[l] = red[l][0];
+}
+
+// compute_wall_distance (src/Driver.cu:461-482): all points incl. ghosts.
+// Wall points are staged through LDS in chunks.
+template <typename R>
+__global__ __launch_bounds__(256) void k_wall_dist(ZoneView<R> z, const double *__restrict__ wp, int n) {
+  const auto c = cell_index(z, z.g);
+  __shared__ double sw[3 * 256];
+  double x = 0, y = 0, zz = 0, best = 1e+6;
+  int64_t id = 0;
+  if (c.ok) {
+    id = z.idx(c.i, c.j, c.k);
+    x = z.x[id];
+    y = z.y[id];
+    zz = z.z[id];
+  }
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int base = 0; base < n; base += 256) {
+    const int cnt = n - base < 256 ? n - base : 256;
+    __syncthreads();
+    if (t < cnt) {
+      sw[3 * t] = wp[3 * (base + t)];
+      sw[3 * t + 1] = wp[3 * (base + t) + 1];
+      sw[3 * t + 2] = wp[3 * (base + t) + 2];
+    }
+    __syncthreads();
+    for (int l = 0; l < cnt; ++l) {
+      const double dx = x - sw[3 * l], dy = y - sw[3 * l + 1], dz = zz - sw[3 * l + 2];
+      const double d = dx * dx + dy * dy + dz * dz;
+      if (best > d) best = d;
+    }
+  }
+  if (c.ok) z.wall_dist[id] = (R)sqrt(best);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+#define OSTC_DISPATCH_NSNT(o, CALL)              \
+  do {                                           \
+    if ((o).ns == 0 && (o).nt == 0) {            \
+      constexpr int NS_ = 0, NT_ = 0;            \
+      CALL;                                      \
+    } else if ((o).ns == 0 && (o).nt == 2) {     \
+      constexpr int NS_ = 0, NT_ = 2;            \
+      CALL;                                      \
+    } else if ((o).ns == 9 && (o).nt == 0) {     \
+      constexpr int NS_ = 9, NT_ = 0;            \
+      CALL;                                      \
+    } else if ((o).ns == 9 && (o).nt == 2) {     \
+      constexpr int NS_ = 9, NT_ = 2;            \
+      CALL;                                      \
+    } else {                                     \
+      return OSTC_ERR_ARG;                       \
+    }                                            \
+  } while (0)
+
+static inline int last_err() { return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP; }
+
+template <typename R>
+int cv_from_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st) {
+  const auto g = cell_grid(z.mx + 2 * z.g, z.my + 2 * z.g, z.mz + 2 * z.g);
+  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_cv_from_bv<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp));
+  return last_err();
+}
+
+template <typename R>
+int physical_properties(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st) {
+  const auto g = cell_grid(z.mx + 2, z.my + 2, z.mz + 2);
+  if (o.ns == 0)
+    hipLaunchKernelGGL((k_props<R, 0>), g.grid, g.block, 0, st, z, dp);
+  else if (o.ns == 9)
+    hipLaunchKernelGGL((k_props<R, 9>), g.grid, g.block, 0, st, z, dp);
+  else
+    return OSTC_ERR_ARG;
+  return last_err();
+}
+
+template <typename R>
+int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st) {
+  if (o.nt == 0) return OSTC_OK;
+  const auto g = cell_grid(z.mx + 2, z.my + 2, z.mz + 2);
+  if (o.ns == 0)
+    hipLaunchKernelGGL((k_init_mut<R, 0>), g.grid, g.block, 0, st, z, dp);
+  else if (o.ns == 9)
+    hipLaunchKernelGGL((k_init_mut<R, 9>), g.grid, g.block, 0, st, z, dp);
+  else
+    return OSTC_ERR_ARG;
+  return last_err();
+}
+
+template <typename R>
+int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, bool force,
+                 bool *wrote) {
+  const bool chem = o.mix_model == OSTC_FR;
+  *wrote = false;
+  if (o.nt == 0 && !chem && !force) return OSTC_OK;
+  const auto g = cell_grid(z.mx, z.my, z.mz);
+  if (chem) {
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp));
+  } else {
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp));
+  }
+  *wrote = true;
+  return last_err();
+}
+
+template <typename R>
+int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st) {
+  const auto g = cell_grid(z.mx, z.my, z.mz);
+  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_local_dt<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp));
+  return last_err();
+}
+
+template <typename R>
+int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result) {
+  const auto g = cell_grid(z.mx, z.my, z.mz);
+  const bool chem = o.mix_model == OSTC_FR;
+  *result = z.dq;
+  if (o.implicit_method == 0) {
+    const bool need = (chem && o.chem != 0) || (o.nt > 0 && o.turb_implicit == 1);
+    if (!need) return OSTC_OK;
+    if (chem) {
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_explicit_implicit<R, NS_, NT_, true>), g.grid, g.block, 0, st, z,
+                                               dp));
+    } else {
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_explicit_implicit<R, NS_, NT_, false>), g.grid, g.block, 0, st,
+                                               z, dp));
+    }
+    return last_err();
+  }
+  // DPLUR: dq -> dq0; sweeps ping-pong dq0 -> dqk -> dq -> dqk ...
+  prof_mark(P_DQ0, st, true);
+  if (chem) {
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp));
+  } else {
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp));
+  }
+  prof_mark(P_DQ0, st, false);
+  if (last_err()) return OSTC_ERR_HIP;
+  R *in = z.dq0, *out = z.dqk;
+  for (int s = 0; s < o.dplur_steps; ++s) {
+    prof_mark(P_DPLUR, st, true);
+    if (chem) {
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp,
+                                               (const R *)in, out));
+    } else {
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp,
+                                               (const R *)in, out));
+    }
+    prof_mark(P_DPLUR, st, false);
+    if (last_err()) return OSTC_ERR_HIP;
+    in = out;
+    out = (out == z.dqk) ? z.dq : z.dqk;
+  }
+  *result = in;
+  return OSTC_OK;
+}
+
+template <typename R>
+int update_cv_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, const R *dq_final, hipStream_t st) {
+  const auto g = cell_grid(z.mx, z.my, z.mz);
+  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_update<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp, dq_final));
+  return last_err();
+}
+
+template <typename R>
+int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st) {
+  const auto g = cell_grid(z.mx, z.my, z.mz);
+  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_flow<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp));
+  return last_err();
+}
+
+template <typename R>
+int store_last_step(const ZoneView<R> &z, hipStream_t st) {
+  const auto g = cell_grid(z.mx, z.my, z.mz);
+  hipLaunchKernelGGL((k_store_last<R>), g.grid, g.block, 0, st, z);
+  return last_err();
+}
+
+template <typename R>
+int residual_partials(const ZoneView<R> &z, double *partial, hipStream_t st, int *nblocks) {
+  const auto g = cell_grid(z.mx, z.my, z.mz);
+  hipLaunchKernelGGL((k_residual_partial<R>), g.grid, g.block, 0, st, z, partial);
+  *nblocks = (int)(g.grid.x * g.grid.y * g.grid.z);
+  return last_err();
+}
+
+int residual_finalize(const double *partial, int nparts, double *out4, hipStream_t st) {
+  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, partial, nparts, out4);
+  return last_err();
+}
+
+template <typename R>
+int wall_distance(const ZoneView<R> &z, const double *wall_xyz, int n_wall, hipStream_t st) {
+  const auto g = cell_grid(z.mx + 2 * z.g, z.my + 2 * z.g, z.mz + 2 * z.g);
+  hipLaunchKernelGGL((k_wall_dist<R>), g.grid, g.block, 0, st, z, wall_xyz, n_wall);
+  return last_err();
+}
+
+#define OSTC_INST(R)                                                                                            \
+  template int cv_from_bv<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);           \
+  template int physical_properties<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);  \
+  template int initialize_mut<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);       \
+  template int source_terms<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t, bool,   \
+                                bool *);                                                                        \
+  template int local_time_step<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);      \
+  template int implicit_treatment<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t,    \
+                                     R **);                                                                     \
+  template int update_cv_bv<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, const R *,            \
+                               hipStream_t);                                                                    \
+  template int limit_flow<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);           \
+  template int store_last_step<R>(const ZoneView<R> &, hipStream_t);                                            \
+  template int residual_partials<R>(const ZoneView<R> &, double *, hipStream_t, int *);                         \
+  template int wall_distance<R>(const ZoneView<R> &, const double *, int, hipStream_t);
+OSTC_INST(float)
+OSTC_INST(double)
+#undef OSTC_INST
+
+}  // namespace ostc

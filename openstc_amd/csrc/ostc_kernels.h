@@ -1,0 +1,81 @@
+// Host-side launchers of the device kernels (one translation unit each).
+#pragma once
+
+#include "ostc_dev.h"
+
+namespace ostc {
+
+// Stage timing with HIP events on the launch stream (enabled per context by
+// ostc_profile); prof_mark is a no-op when profiling is off.
+enum ProfStage {
+  P_SOURCE = 0, P_FLUX_I, P_FLUX_J, P_FLUX_K, P_DT, P_DQ0, P_DPLUR, P_EXPL_IMPL, P_UPDATE, P_LIMIT, P_BC,
+  P_HALO, P_PROPS, P_STORE, P_RESID, P_COUNT
+};
+void prof_mark(int stage, hipStream_t st, bool begin);
+
+// Cell-parallel launch shape: 256 threads = 64 (i) x 4 (j) per workgroup.
+struct CellGrid {
+  dim3 grid, block;
+};
+inline CellGrid cell_grid(int ni, int nj, int nk) {
+  CellGrid g;
+  g.block = dim3(64, 4, 1);
+  g.grid = dim3((ni + 63) / 64, (nj + 3) / 4, nk);
+  return g;
+}
+
+// k_flux.hip
+template <typename R>
+int flux_sweeps(const ZoneView<R> &z, const ostc_param *dp, int ns, int nt, bool inv, bool visc, bool first_writer,
+                hipStream_t st);
+
+// k_cell.hip
+struct CellOpts {
+  int ns, nt;      // species count, turbulence equations (0 or 2)
+  int mix_model;   // OSTC_AIR / OSTC_MIXTURE / OSTC_FR
+  int chem;        // chem_src_method
+  int rans_model, turb_implicit, implicit_method, dplur_steps;
+};
+template <typename R>
+int cv_from_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
+template <typename R>
+int physical_properties(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
+template <typename R>
+int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
+// writes dq (first writer; zeros when the model has no source and force is set)
+template <typename R>
+int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, bool force,
+                 bool *wrote);
+template <typename R>
+int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
+// leaves the final increment in *result (one of z.dq/z.dq0/z.dqk)
+template <typename R>
+int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result);
+template <typename R>
+int update_cv_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, const R *dq_final, hipStream_t st);
+template <typename R>
+int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
+template <typename R>
+int store_last_step(const ZoneView<R> &z, hipStream_t st);
+// per-zone partial sums -> partial[nblocks*4]; returns nblocks
+template <typename R>
+int residual_partials(const ZoneView<R> &z, double *partial, hipStream_t st, int *nblocks);
+int residual_finalize(const double *partial, int nparts, double *out4, hipStream_t st);
+template <typename R>
+int wall_distance(const ZoneView<R> &z, const double *wall_xyz, int n_wall, hipStream_t st);
+
+// k_bc.hip
+template <typename R>
+int apply_bc(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, const ostc_bc &bc,
+             const ostc_inflow *d_inflow, hipStream_t st);
+template <typename R>
+int inner_halo(const ZoneView<R> &z, const ZoneView<R> &t, const ostc_param *dp, const CellOpts &o,
+               const ostc_inner_face &f, hipStream_t st);
+
+// k_chem.hip
+int chem_plain_step(const ostc_param *dp, int ns, int nr, long n, const double *t, const double *p, const double *y,
+                    const double *dt, double *dy, hipStream_t st);
+int chem_source_batch(const ostc_param *dp, int ns, int nr, long n, const double *rho, const double *t,
+                      const double *y, double *omega, double *jac, hipStream_t st);
+
+}  // namespace ostc

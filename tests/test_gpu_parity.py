@@ -1,0 +1,163 @@
+"""GPU parity of the HIP path against the FP64 oracle, stage by stage and over
+whole iterations, on small instances of the benchmark configurations.
+
+Tolerances (scaled max error = max|gpu-oracle| / max|oracle| per field):
+  * FP64 zones: 1e-11 per stage, 1e-9 after several iterations (ulp-level
+    differences of division/sqrt/exp implementations and FMA contraction);
+  * FP32 zones: 2e-5 per stage with the GPU state copied into the oracle
+    before the stage, so the bound measures the kernel's own FP32 arithmetic.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from harness import Pair, interior, scaled_err
+from openstc_amd import _lib as L
+from openstc_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-11
+TOL32 = 2e-5
+
+
+def say(*a):
+    print("[gpu-parity]", *a, flush=True)
+
+
+def stage_check(pr: Pair, tol, sync32: bool):
+    """Run every stage of one iteration on both sides and compare."""
+    lib, c, p = pr.lib, pr.c, pr.p
+    z, zo = pr.zg(), pr.zo()
+    O = pr.orc.lib
+    errs = {}
+    state = ["BV", "SV", "CV", "VEL", "SOUND", "MUL", "LAMBDA", "MACH", "MUT", "LAMBDA_T", "GAMMA", "CP", "RHO_D"]
+    if sync32:
+        pr.sync_to_oracle(state + ["WALL_DIST", "JAC", "METRIC"])
+    # source (first writer of dq)
+    pr.orc.blocks[0].arr["dq"][:] = 0
+    L.check(lib.ostc_compute_source(c, z.h, None), "source")
+    O.orc_compute_source(zo, p)
+    errs["source:dq"] = pr.err("DQ")
+    if pr.case.param.turb_method == L.RANS:
+        errs["source:mut"] = pr.err("MUT")
+        errs["source:turb_jac"] = pr.err("TURB_JAC")
+    if sync32:
+        pr.sync_to_oracle(["DQ", "MUT", "LAMBDA_T", "TURB_JAC"])
+    L.check(lib.ostc_inviscid_flux(c, z.h, None), "inviscid")
+    O.orc_inviscid_flux(zo, p)
+    errs["inviscid:dq"] = pr.err("DQ")
+    if sync32:
+        pr.sync_to_oracle(["DQ"])
+    L.check(lib.ostc_viscous_flux(c, z.h, None), "viscous")
+    O.orc_viscous_flux(zo, p)
+    errs["viscous:dq"] = pr.err("DQ")
+    L.check(lib.ostc_local_time_step(c, z.h, None), "dt")
+    O.orc_local_time_step(zo, p)
+    errs["dt"] = pr.err("DT")
+    errs["spectr"] = pr.err("SPECTR")
+    if sync32:
+        pr.sync_to_oracle(["DQ", "DT", "SPECTR"])
+    L.check(lib.ostc_implicit_treatment(c, z.h, None), "implicit")
+    O.orc_implicit_treatment(zo, p)
+    errs["implicit:dq"] = pr.err("DQ")
+    if sync32:
+        pr.sync_to_oracle(["DQ"])
+    L.check(lib.ostc_update_cv_bv(c, z.h, None), "update")
+    O.orc_update_cv_and_bv(zo, p)
+    errs["update:bv"] = pr.err("BV")
+    errs["update:cv"] = pr.err("CV")
+    L.check(lib.ostc_limit_flow(c, z.h, None), "limit")
+    O.orc_limit_flow(zo, p)
+    if sync32:
+        pr.sync_to_oracle(["BV", "CV", "SV", "VEL"])
+    L.check(lib.ostc_apply_bc(c, z.h, None), "bc")
+    O.orc_apply_bc(zo, p, C.byref(pr.orc.bcs[0]))
+    errs["bc:bv(padded)"] = pr.err("BV", halo=2)
+    errs["bc:cv(padded)"] = pr.err("CV", halo=2)
+    L.check(lib.ostc_update_physical_properties(c, z.h, None), "props")
+    O.orc_update_physical_properties(zo, p)
+    errs["props:mul"] = pr.err("MUL", halo=1)
+    errs["props:sound"] = pr.err("SOUND", halo=1)
+    say({k: f"{v:.2e}" for k, v in errs.items()})
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, bad
+
+
+def test_config1_fp64_init_and_stages():
+    case = cases.config1(mx=40, my=24, mz=8)
+    say("config1 40x24x8 FP64 stages")
+    pr = Pair(case)
+    pr.init()
+    for f in ("BV", "CV", "SOUND", "MUL"):
+        assert pr.err(f, halo=1) < TOL64, f
+    stage_check(pr, TOL64, sync32=False)
+    pr.close()
+
+
+def test_config1_fp64_iterations():
+    case = cases.config1(mx=40, my=24, mz=8)
+    say("config1 40x24x8 FP64 10 steps")
+    pr = Pair(case)
+    pr.init()
+    for s in range(10):
+        rg = pr.gpu.step(output_screen=5)
+        ro = pr.orc.step(output_screen=5)
+        if rg is not None:
+            assert abs(rg - ro) <= 1e-8 * max(1.0, abs(ro)), (s, rg, ro)
+    e = pr.err("BV", halo=2)
+    say("bv err after 10 steps", e)
+    assert e < 1e-9
+    pr.close()
+
+
+def test_config2_fp64_stages():
+    case = cases.config2(mx=48, my=32, mz=12)
+    say("config2 48x32x12 FP64 stages")
+    pr = Pair(case, precision=L.FP64)
+    pr.init()
+    for f in ("BV", "SV", "CV", "MUT", "WALL_DIST"):
+        e = pr.err(f, halo=1)
+        assert e < TOL64, (f, e)
+    stage_check(pr, TOL64, sync32=False)
+    pr.close()
+
+
+def test_config2_fp64_iterations():
+    case = cases.config2(mx=48, my=32, mz=12)
+    say("config2 48x32x12 FP64 5 steps")
+    pr = Pair(case, precision=L.FP64)
+    pr.init()
+    for s in range(5):
+        pr.gpu.step()
+        pr.orc.step()
+    for f in ("BV", "SV"):
+        e = pr.err(f, halo=2)
+        say(f, "err after 5 steps", e)
+        assert e < 1e-9, (f, e)
+    pr.close()
+
+
+def test_config2_fp32_stages():
+    case = cases.config2(mx=48, my=32, mz=12)
+    say("config2 48x32x12 FP32 stages")
+    pr = Pair(case, precision=L.FP32)
+    pr.init()
+    stage_check(pr, TOL32, sync32=True)
+    pr.close()
+
+
+def test_config1_fp32_vs_fp64_trajectory():
+    """FP32 bulk vs FP64 over 20 explicit steps: field error stays small."""
+    case = cases.config1(mx=40, my=24, mz=8)
+    say("config1 FP32 vs oracle 20 steps")
+    pr = Pair(case, precision=L.FP32)
+    pr.init()
+    for s in range(20):
+        pr.gpu.step()
+        pr.orc.step()
+    e = pr.err("BV")
+    say("bv scaled err FP32 after 20 steps", e)
+    assert e < 1e-4
+    pr.close()
