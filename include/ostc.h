@@ -154,6 +154,21 @@ int ostc_sizeof_bc(void);
 int ostc_block_geometry(int mx, int my, int mz, int ngg, const double *x_in, const double *y_in, const double *z_in,
                         double *x_out, double *y_out, double *z_out, double *jac, double *metric, int *n_neg_jac);
 
+/* CHEMKIN inputs (Species/Reaction constructors, src/ChemData.cpp:11-614):
+ * fills mw, NASA-7 tables, T ranges, Lennard-Jones/viscosity constants, the
+ * Wilke tables and (with_reactions != 0) the reaction arrays of *param, and
+ * n_spec / n_reac.  The caller sets the remaining model switches. */
+#define OSTC_MAX_ELEM 8
+typedef struct ostc_mechanism_info {
+  int n_spec, n_elem;
+  char spec_names[OSTC_MAX_SPEC][24];
+  char elem_names[OSTC_MAX_ELEM][4];
+  int elem_comp[OSTC_MAX_SPEC * OSTC_MAX_ELEM]; /* [species][element], elements in file order */
+} ostc_mechanism_info;
+int ostc_read_chemkin(const char *mechanism, const char *thermo, const char *transport, int with_reactions,
+                      ostc_param *param, ostc_mechanism_info *info);
+const char *ostc_chemkin_error(void);
+
 /* ---- setup stages ---- */
 /* compute_cv_from_bv<M,T> (src/FieldOperation.cuh:37-73) */
 int ostc_compute_cv_from_bv(ostc_ctx *ctx, ostc_zone *zone, void *stream);

@@ -308,7 +308,7 @@ def _tanh_stretch(n, height, first):
 
 
 def config2(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, cfl=5.0, height=0.1,
-            span=0.25, seed=2) -> Case:
+            span=0.25, seed=2, perturb=False) -> Case:
     """Viscous RANS: air, k-omega SST, flat plate with tanh wall clustering
     (first cell y+~1 from the turbulent flat-plate Cf correlation at x=0.5 m),
     Mach 2.5, Re=1e7/m, T=220 K, adiabatic no-slip wall at j=0, inflow i=0,
@@ -339,6 +339,14 @@ def config2(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, 
     blk.bc = bc_program(rows, [(5, "inflow"), (2, "wall")], [fs], [(False, 300.0)], ngg)
     finish_block(blk, ngg)
     bv, sv = uniform_field(blk, fs, 2, ngg)
+    if perturb:  # +-1% seeded perturbation (parity tests: no cancellation-only residuals)
+        for l in (0, 1, 4):
+            perturb_interior(bv[l], blk, seed, l, ngg, rel=0.01)
+        perturb_interior(bv[2], blk, seed, 2, ngg, absval=0.01 * fs.u)
+        perturb_interior(bv[3], blk, seed, 3, ngg, absval=0.01 * fs.u)
+        perturb_interior(sv[0], blk, seed, 5, ngg, rel=0.01)
+        perturb_interior(sv[1], blk, seed, 6, ngg, rel=0.01)
+        bv[5] = bv[4] * MW_AIR / (bv[0] * RU)
     blk.bv0, blk.sv0 = bv, sv
     # wall points: the trimmed wall face (src/Driver.cu:128-149)
     wall = np.stack([x[:, 0, :].ravel(), y[:, 0, :].ravel(), z[:, 0, :].ravel()], axis=1)

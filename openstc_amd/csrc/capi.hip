@@ -89,6 +89,9 @@ struct ostc_zone {
   int fsize[OSTC_F_COUNT] = {};  // element size per field
   int *piv = nullptr;
   unsigned long long *counters = nullptr;
+  int *lf_count = nullptr;
+  int64_t *lf_idx = nullptr;
+  void *lf_vals = nullptr;
   ostc_bc bc{};
   bool has_bc = false;
   ostc_inflow *d_inflow = nullptr;
@@ -140,6 +143,9 @@ ZoneView<R> make_view(ostc_zone *z) {
   v.chem_lu = v.chem_jac;  // factors overwrite the Jacobian (it is not needed afterwards)
   v.chem_piv = z->piv;
   v.counters = z->counters;
+  v.lf_count = z->lf_count;
+  v.lf_idx = z->lf_idx;
+  v.lf_vals = static_cast<R *>(z->lf_vals);
   return v;
 }
 
@@ -293,6 +299,15 @@ int ostc_zone_create(ostc_ctx *ctx, const ostc_zone_desc *d, ostc_zone **out) {
   if (e == hipSuccess) e = hipMalloc(&z->counters, 4 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMemset(z->counters, 0, 4 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMalloc(&z->d_inflow, OSTC_MAX_BCDATA * sizeof(ostc_inflow));
+  {
+    // limit_flow repair list, sized for every interior cell
+    const size_t ncell = (size_t)d->mx * d->my * d->mz;
+    const size_t nval = 6 + nsc + nv;
+    if (e == hipSuccess) e = hipMalloc(&z->lf_count, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(z->lf_count, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&z->lf_idx, ncell * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&z->lf_vals, ncell * nval * z->esize);
+  }
   if (e != hipSuccess) {
     ostc_zone_destroy(z);
     return hip_fail(e, "zone setup");
@@ -310,6 +325,9 @@ int ostc_zone_destroy(ostc_zone *z) {
   if (z->piv) (void)hipFree(z->piv);
   if (z->counters) (void)hipFree(z->counters);
   if (z->d_inflow) (void)hipFree(z->d_inflow);
+  if (z->lf_count) (void)hipFree(z->lf_count);
+  if (z->lf_idx) (void)hipFree(z->lf_idx);
+  if (z->lf_vals) (void)hipFree(z->lf_vals);
   delete z;
   return OSTC_OK;
 }

@@ -569,10 +569,17 @@ __global__ __launch_bounds__(256) void k_update(ZoneView<R> z, const ostc_param 
 
 // ------------------------------------------------------------------ limit_flow
 
-// limit_flow (src/TimeAdvanceFunc.cu:34-224) with the reference's i-neighbour
-// bound by mz (quirk kept for parity)
+// limit_flow (src/TimeAdvanceFunc.cu:34-224), in two passes so that every
+// repair reads the state as it was before the pass (the reference's single
+// kernel lets a repaired cell be read by a neighbour in the same launch, an
+// order-dependent race; here the result is deterministic and equal to the
+// reference wherever bad cells are not adjacent).  The i-neighbour bound by
+// mz is the reference's and is kept.
+//   pass 1: classify; for bad cells compute the final bv/sv/cv into a list
+//   pass 2: write the list back
 template <typename R, int NS, int NT>
-__global__ __launch_bounds__(256) void k_limit_flow(ZoneView<R> z, const ostc_param *__restrict__ p) {
+__global__ __launch_bounds__(256) void k_limit_check(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  constexpr int NSC = NS + NT, NV = 5 + NSC, NVAL = 6 + NSC + NV;
   const auto c = cell_index(z, 0);
   if (!c.ok) return;
   const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
@@ -584,11 +591,24 @@ __global__ __launch_bounds__(256) void k_limit_flow(ZoneView<R> z, const ostc_pa
     var[l] = z.bv[l * P + id];
     if (isnan(var[l]) || var[l] < (R)ll[l] || var[l] > (R)ul[l]) bad = true;
   }
+  bool tbad = false;
+  if constexpr (NT > 0) {
+    const R t0 = z.sv[NS * P + id], t1 = z.sv[(NS + 1) * P + id];
+    tbad = p->rans_model == 2 && (isnan(t0) || isnan(t1) || t0 < (R)0 || t1 < (R)0);
+  }
+  if (!bad && !tbad) return;
+  R bv[6], sv[NSC > 0 ? NSC : 1], cv[NV];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) bv[l] = z.bv[l * P + id];
+#pragma unroll
+  for (int l = 0; l < NSC; ++l) sv[l] = z.sv[l * P + id];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) cv[l] = z.cv[l * P + id];
   if (bad) {
     atomicAdd(&z.counters[1], 1ull);
-    R uv[5 + NS + NT];
+    R uv[5 + NSC];
 #pragma unroll
-    for (int l = 0; l < 5 + NS + NT; ++l) uv[l] = 0;
+    for (int l = 0; l < 5 + NSC; ++l) uv[l] = 0;
     int kn = 0;
     for (int ka = -1; ka < 2; ++ka) {
       const int k1 = c.k + ka;
@@ -611,7 +631,7 @@ __global__ __launch_bounds__(256) void k_limit_flow(ZoneView<R> z, const ostc_pa
 #pragma unroll
           for (int l = 0; l < 5; ++l) uv[l] += nb[l];
 #pragma unroll
-          for (int l = 0; l < NS + NT; ++l) uv[5 + l] += z.sv[l * P + n1];
+          for (int l = 0; l < NSC; ++l) uv[5 + l] += z.sv[l * P + n1];
           ++kn;
         }
       }
@@ -619,24 +639,24 @@ __global__ __launch_bounds__(256) void k_limit_flow(ZoneView<R> z, const ostc_pa
     if (kn > 0) {
       const R kinv = (R)1.0 / (R)kn;
 #pragma unroll
-      for (int l = 0; l < 5 + NS + NT; ++l) uv[l] *= kinv;
+      for (int l = 0; l < 5 + NSC; ++l) uv[l] *= kinv;
     } else {
 #pragma unroll
       for (int l = 0; l < 5; ++l) uv[l] = dmin(dmax(var[l], (R)ll[l]), (R)ul[l]);
 #pragma unroll
-      for (int l = 0; l < NS + NT; ++l) uv[5 + l] = (R)p->sv_inf[l];
+      for (int l = 0; l < NSC; ++l) uv[5 + l] = (R)p->sv_inf[l];
     }
 #pragma unroll
-    for (int l = 0; l < 5; ++l) z.bv[l * P + id] = uv[l];
-    z.cv[id] = uv[0];
-    z.cv[P + id] = uv[0] * uv[1];
-    z.cv[2 * P + id] = uv[0] * uv[2];
-    z.cv[3 * P + id] = uv[0] * uv[3];
+    for (int l = 0; l < 5; ++l) bv[l] = uv[l];
+    cv[0] = uv[0];
+    cv[1] = uv[0] * uv[1];
+    cv[2] = uv[0] * uv[2];
+    cv[3] = uv[0] * uv[3];
     R e = (R)0.5 * uv[0] * (uv[1] * uv[1] + uv[2] * uv[2] + uv[3] * uv[3]);
 #pragma unroll
-    for (int l = 0; l < NS + NT; ++l) {
-      z.sv[l * P + id] = uv[5 + l];
-      z.cv[(5 + l) * P + id] = uv[0] * uv[5 + l];
+    for (int l = 0; l < NSC; ++l) {
+      sv[l] = uv[5 + l];
+      cv[5 + l] = uv[0] * uv[5 + l];
     }
     if constexpr (NS > 0) {
       R mw = 0;
@@ -644,59 +664,84 @@ __global__ __launch_bounds__(256) void k_limit_flow(ZoneView<R> z, const ostc_pa
       for (int l = 0; l < NS; ++l) mw += uv[5 + l] * (R)(1.0 / p->mw[l]);
       mw = (R)1 / mw;
       const R t = uv[4] * mw / (uv[0] * (R)kRu);
-      z.bv[5 * P + id] = t;
+      bv[5] = t;
       R h[NS];
       enthalpy<R>(t, h, p, NS);
 #pragma unroll
-      for (int l = 0; l < NS; ++l) e += h[l] * uv[0] * uv[5 + l];
+      for (int l = 0; l < NS; ++l) e += h[l] * cv[5 + l];
       e -= uv[4];
     } else {
-      z.bv[5 * P + id] = uv[4] * (R)kMwAir / (uv[0] * (R)kRu);
+      bv[5] = uv[4] * (R)kMwAir / (uv[0] * (R)kRu);
       e += uv[4] / (R)(kGammaAir - 1);
     }
-    z.cv[4 * P + id] = e;
+    cv[4] = e;
   }
   if constexpr (NT > 0) {
-    if (p->rans_model == 2) {
-      const R t0 = z.sv[NS * P + id], t1 = z.sv[(NS + 1) * P + id];
-      if (isnan(t0) || isnan(t1) || t0 < (R)0 || t1 < (R)0) {
-        R a0 = 0, a1 = 0;
-        int kn = 0;
-        for (int ka = -1; ka < 2; ++ka) {
-          const int k1 = c.k + ka;
-          if (k1 < 0 || k1 >= z.mz) continue;
-          for (int ja = -1; ja < 2; ++ja) {
-            const int j1 = c.j + ja;
-            if (j1 < 0 || j1 >= z.my) continue;
-            for (int ia = -1; ia < 2; ++ia) {
-              const int i1 = c.i + ia;
-              if (i1 < 0 || i1 >= z.mz) continue;
-              const int64_t n1 = z.idx(i1, j1, k1);
-              const R x0 = z.sv[NS * P + n1], x1 = z.sv[(NS + 1) * P + n1];
-              if (isnan(x0) || isnan(x1) || x0 < (R)0 || x1 < (R)0) continue;
-              a0 += x0;
-              a1 += x1;
-              ++kn;
-            }
+    const R t0 = sv[NS], t1 = sv[NS + 1];
+    if (p->rans_model == 2 && (isnan(t0) || isnan(t1) || t0 < (R)0 || t1 < (R)0)) {
+      R a0 = 0, a1 = 0;
+      int kn = 0;
+      for (int ka = -1; ka < 2; ++ka) {
+        const int k1 = c.k + ka;
+        if (k1 < 0 || k1 >= z.mz) continue;
+        for (int ja = -1; ja < 2; ++ja) {
+          const int j1 = c.j + ja;
+          if (j1 < 0 || j1 >= z.my) continue;
+          for (int ia = -1; ia < 2; ++ia) {
+            const int i1 = c.i + ia;
+            if (i1 < 0 || i1 >= z.mz) continue;
+            const int64_t n1 = z.idx(i1, j1, k1);
+            const R x0 = z.sv[NS * P + n1], x1 = z.sv[(NS + 1) * P + n1];
+            if (isnan(x0) || isnan(x1) || x0 < (R)0 || x1 < (R)0) continue;
+            a0 += x0;
+            a1 += x1;
+            ++kn;
           }
         }
-        if (kn > 0) {
-          const R kinv = (R)1.0 / (R)kn;
-          a0 *= kinv;
-          a1 *= kinv;
-        } else {
-          a0 = t0 < (R)0 ? (R)p->sv_inf[NS] : t0;
-          a1 = t1 < (R)0 ? (R)p->sv_inf[NS + 1] : t1;
-        }
-        z.sv[NS * P + id] = a0;
-        z.sv[(NS + 1) * P + id] = a1;
-        const R rho = z.cv[id];
-        z.cv[(NS + 5) * P + id] = rho * a0;
-        z.cv[(NS + 6) * P + id] = rho * a1;
       }
+      if (kn > 0) {
+        const R kinv = (R)1.0 / (R)kn;
+        a0 *= kinv;
+        a1 *= kinv;
+      } else {
+        a0 = t0 < (R)0 ? (R)p->sv_inf[NS] : t0;
+        a1 = t1 < (R)0 ? (R)p->sv_inf[NS + 1] : t1;
+      }
+      sv[NS] = a0;
+      sv[NS + 1] = a1;
+      cv[NS + 5] = cv[0] * a0;
+      cv[NS + 6] = cv[0] * a1;
     }
   }
+  const int slot = atomicAdd(z.lf_count, 1);
+  z.lf_idx[slot] = id;
+  R *out = z.lf_vals + (int64_t)slot * NVAL;
+#pragma unroll
+  for (int l = 0; l < 6; ++l) out[l] = bv[l];
+#pragma unroll
+  for (int l = 0; l < NSC; ++l) out[6 + l] = sv[l];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) out[6 + NSC + l] = cv[l];
 }
+
+template <typename R, int NS, int NT>
+__global__ __launch_bounds__(256) void k_limit_apply(ZoneView<R> z) {
+  constexpr int NSC = NS + NT, NV = 5 + NSC, NVAL = 6 + NSC + NV;
+  const int n = *z.lf_count;
+  const int64_t P = z.plane;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int64_t id = z.lf_idx[s];
+    const R *v = z.lf_vals + (int64_t)s * NVAL;
+#pragma unroll
+    for (int l = 0; l < 6; ++l) z.bv[l * P + id] = v[l];
+#pragma unroll
+    for (int l = 0; l < NSC; ++l) z.sv[l * P + id] = v[6 + l];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) z.cv[l * P + id] = v[6 + NSC + l];
+  }
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
 
 // ------------------------------------------------------------------ residual
 
@@ -938,7 +983,9 @@ int update_cv_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, 
 template <typename R>
 int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st) {
   const auto g = cell_grid(z.mx, z.my, z.mz);
-  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_flow<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp));
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, z.lf_count);
+  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_check<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp));
+  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_apply<R, NS_, NT_>), dim3(256), dim3(256), 0, st, z));
   return last_err();
 }
 

@@ -912,32 +912,41 @@ int orc_update_cv_and_bv(orc_zone *z, const orc_param *p) {
 
 /* ---------------------------------------------------------------- limit_flow */
 
-/* limit_flow: src/TimeAdvanceFunc.cu:34-224.  Cells are visited in one pass
- * with the state as left by the previous cells, which matches the reference
- * whenever repaired cells are not neighbours of each other (the GPU kernel
- * reads a snapshot; the two agree away from clusters of bad cells).  The
- * i-neighbour bound uses mz, as the reference does (quirk A-6).  Returns the
- * number of repaired cells. */
+/* limit_flow: src/TimeAdvanceFunc.cu:34-224.  Every repair reads the field
+ * as it was before the pass (snapshot semantics; the reference's parallel
+ * kernel is order-dependent only where bad cells are adjacent, and equals this
+ * everywhere else).  A cell's flow repair precedes its turbulence repair, as in
+ * the reference.  The i-neighbour bound uses mz, as the reference does.
+ * Returns the number of cells with a flow repair. */
 int orc_limit_flow(orc_zone *z, const orc_param *p) {
-  const int mx = z->mx, my = z->my, mz = z->mz, ns = p->n_spec;
+  const int mx = z->mx, my = z->my, mz = z->mz, ns = p->n_spec, nsc = p->n_scalar, nv = p->n_var;
   const double *ll = p->limit_ll, *ul = p->limit_ul;
+  const int nval = 6 + nsc + nv;
+  const long ncell = (long)mx * my * mz;
+  long *lidx = (long *)malloc(sizeof(long) * (size_t)ncell);
+  double *lval = (double *)malloc(sizeof(double) * (size_t)ncell * nval);
+  long n = 0;
   int repaired = 0;
+  /* pass 1: the final state of every bad cell from the un-repaired field */
   for (int k = 0; k < mz; ++k)
     for (int j = 0; j < my; ++j)
       for (int i = 0; i < mx; ++i) {
         double var[5];
-        for (int l = 0; l < 5; ++l) var[l] = F(bv, i, j, k, l);
         int unphysical = 0;
         for (int l = 0; l < 5; ++l) {
-          if (isnan(var[l])) {
-            unphysical = 1;
-            break;
-          }
-          if (var[l] < ll[l] || var[l] > ul[l]) {
-            unphysical = 1;
-            break;
-          }
+          var[l] = F(bv, i, j, k, l);
+          if (isnan(var[l]) || var[l] < ll[l] || var[l] > ul[l]) unphysical = 1;
         }
+        int tbad = 0;
+        if (p->rans_model == 2) {
+          const double t0 = F(sv, i, j, k, ns), t1 = F(sv, i, j, k, ns + 1);
+          tbad = isnan(t0) || isnan(t1) || t0 < 0 || t1 < 0;
+        }
+        if (!unphysical && !tbad) continue;
+        double bvv[6], svv[NSMAX + 2], cvv[NVMAX];
+        for (int l = 0; l < 6; ++l) bvv[l] = F(bv, i, j, k, l);
+        for (int l = 0; l < nsc; ++l) svv[l] = F(sv, i, j, k, l);
+        for (int l = 0; l < nv; ++l) cvv[l] = F(cv, i, j, k, l);
         if (unphysical) {
           ++repaired;
           double uv[5 + NSMAX + 2];
@@ -959,49 +968,46 @@ int orc_limit_flow(orc_zone *z, const orc_param *p) {
                 }
                 if (bad) continue;
                 for (int l = 0; l < 5; ++l) uv[l] += F(bv, i1, j1, k1, l);
-                for (int l = 0; l < p->n_scalar; ++l) uv[l + 5] += F(sv, i1, j1, k1, l);
+                for (int l = 0; l < nsc; ++l) uv[l + 5] += F(sv, i1, j1, k1, l);
                 ++kn;
               }
             }
           }
           if (kn > 0) {
             const double kn_inv = 1.0 / kn;
-            for (int l = 0; l < 5 + p->n_scalar; ++l) uv[l] *= kn_inv;
+            for (int l = 0; l < 5 + nsc; ++l) uv[l] *= kn_inv;
           } else {
-            for (int l = 0; l < 5; ++l) {
-              uv[l] = dmax(var[l], ll[l]);
-              uv[l] = dmin(uv[l], ul[l]);
-            }
-            for (int l = 0; l < p->n_scalar; ++l) uv[l + 5] = p->sv_inf[l];
+            for (int l = 0; l < 5; ++l) uv[l] = dmin(dmax(var[l], ll[l]), ul[l]);
+            for (int l = 0; l < nsc; ++l) uv[l + 5] = p->sv_inf[l];
           }
-          for (int l = 0; l < 5; ++l) F(bv, i, j, k, l) = uv[l];
-          F(cv, i, j, k, 0) = uv[0];
-          F(cv, i, j, k, 1) = uv[0] * uv[1];
-          F(cv, i, j, k, 2) = uv[0] * uv[2];
-          F(cv, i, j, k, 3) = uv[0] * uv[3];
-          F(cv, i, j, k, 4) = 0.5 * uv[0] * (uv[1] * uv[1] + uv[2] * uv[2] + uv[3] * uv[3]);
-          for (int l = 0; l < p->n_scalar; ++l) {
-            F(sv, i, j, k, l) = uv[5 + l];
-            F(cv, i, j, k, 5 + l) = uv[0] * uv[5 + l];
+          for (int l = 0; l < 5; ++l) bvv[l] = uv[l];
+          cvv[0] = uv[0];
+          cvv[1] = uv[0] * uv[1];
+          cvv[2] = uv[0] * uv[2];
+          cvv[3] = uv[0] * uv[3];
+          cvv[4] = 0.5 * uv[0] * (uv[1] * uv[1] + uv[2] * uv[2] + uv[3] * uv[3]);
+          for (int l = 0; l < nsc; ++l) {
+            svv[l] = uv[5 + l];
+            cvv[5 + l] = uv[0] * uv[5 + l];
           }
           if (ns > 0) {
             double mw = 0;
-            for (int l = 0; l < ns; ++l) mw += F(sv, i, j, k, l) / p->mw[l];
+            for (int l = 0; l < ns; ++l) mw += svv[l] / p->mw[l];
             mw = 1 / mw;
-            F(bv, i, j, k, 5) = uv[4] * mw / (uv[0] * ORC_RU);
+            bvv[5] = uv[4] * mw / (uv[0] * ORC_RU);
             double h[NSMAX];
-            orc_enthalpy(F(bv, i, j, k, 5), h, p);
-            for (int l = 0; l < ns; ++l) F(cv, i, j, k, 4) += h[l] * F(cv, i, j, k, 5 + l);
-            F(cv, i, j, k, 4) -= F(bv, i, j, k, 4);
+            orc_enthalpy(bvv[5], h, p);
+            for (int l = 0; l < ns; ++l) cvv[4] += h[l] * cvv[5 + l];
+            cvv[4] -= bvv[4];
           } else {
-            F(bv, i, j, k, 5) = uv[4] * ORC_MW_AIR / (uv[0] * ORC_RU);
-            F(cv, i, j, k, 4) += uv[4] / (ORC_GAMMA_AIR - 1);
+            bvv[5] = uv[4] * ORC_MW_AIR / (uv[0] * ORC_RU);
+            cvv[4] += uv[4] / (ORC_GAMMA_AIR - 1);
           }
         }
         if (p->rans_model == 2) {
-          const double t0 = F(sv, i, j, k, ns), t1 = F(sv, i, j, k, ns + 1);
+          const double t0 = svv[ns], t1 = svv[ns + 1];
           if (isnan(t0) || isnan(t1) || t0 < 0 || t1 < 0) {
-            double uv[2] = {0, 0};
+            double a0 = 0, a1 = 0;
             int kn = 0;
             for (int ka = -1; ka < 2; ++ka) {
               const int k1 = k + ka;
@@ -1014,27 +1020,43 @@ int orc_limit_flow(orc_zone *z, const orc_param *p) {
                   if (i1 < 0 || i1 >= mz) continue;
                   const double a = F(sv, i1, j1, k1, ns), b = F(sv, i1, j1, k1, ns + 1);
                   if (isnan(a) || isnan(b) || a < 0 || b < 0) continue;
-                  uv[0] += a;
-                  uv[1] += b;
+                  a0 += a;
+                  a1 += b;
                   ++kn;
                 }
               }
             }
             if (kn > 0) {
               const double kn_inv = 1.0 / kn;
-              uv[0] *= kn_inv;
-              uv[1] *= kn_inv;
+              a0 *= kn_inv;
+              a1 *= kn_inv;
             } else {
-              uv[0] = t0 < 0 ? p->sv_inf[ns] : t0;
-              uv[1] = t1 < 0 ? p->sv_inf[ns + 1] : t1;
+              a0 = t0 < 0 ? p->sv_inf[ns] : t0;
+              a1 = t1 < 0 ? p->sv_inf[ns + 1] : t1;
             }
-            F(sv, i, j, k, ns) = uv[0];
-            F(sv, i, j, k, ns + 1) = uv[1];
-            F(cv, i, j, k, ns + 5) = F(cv, i, j, k, 0) * uv[0];
-            F(cv, i, j, k, ns + 6) = F(cv, i, j, k, 0) * uv[1];
+            svv[ns] = a0;
+            svv[ns + 1] = a1;
+            cvv[ns + 5] = cvv[0] * a0;
+            cvv[ns + 6] = cvv[0] * a1;
           }
         }
+        lidx[n] = IX(z, i, j, k);
+        double *o = &lval[n * nval];
+        for (int l = 0; l < 6; ++l) o[l] = bvv[l];
+        for (int l = 0; l < nsc; ++l) o[6 + l] = svv[l];
+        for (int l = 0; l < nv; ++l) o[6 + nsc + l] = cvv[l];
+        ++n;
       }
+  /* pass 2: write back */
+  for (long s = 0; s < n; ++s) {
+    const long id = lidx[s];
+    const double *o = &lval[s * nval];
+    for (int l = 0; l < 6; ++l) z->bv[id + (long)l * z->plane] = o[l];
+    for (int l = 0; l < nsc; ++l) z->sv[id + (long)l * z->plane] = o[6 + l];
+    for (int l = 0; l < nv; ++l) z->cv[id + (long)l * z->plane] = o[6 + nsc + l];
+  }
+  free(lidx);
+  free(lval);
   return repaired;
 }
 

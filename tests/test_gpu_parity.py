@@ -113,7 +113,7 @@ def test_config1_fp64_iterations():
 
 
 def test_config2_fp64_stages():
-    case = cases.config2(mx=48, my=32, mz=12)
+    case = cases.config2(mx=48, my=32, mz=12, perturb=True)
     say("config2 48x32x12 FP64 stages")
     pr = Pair(case, precision=L.FP64)
     pr.init()
@@ -125,7 +125,7 @@ def test_config2_fp64_stages():
 
 
 def test_config2_fp64_iterations():
-    case = cases.config2(mx=48, my=32, mz=12)
+    case = cases.config2(mx=48, my=32, mz=12, perturb=True)
     say("config2 48x32x12 FP64 5 steps")
     pr = Pair(case, precision=L.FP64)
     pr.init()
@@ -140,7 +140,7 @@ def test_config2_fp64_iterations():
 
 
 def test_config2_fp32_stages():
-    case = cases.config2(mx=48, my=32, mz=12)
+    case = cases.config2(mx=48, my=32, mz=12, perturb=True)
     say("config2 48x32x12 FP32 stages")
     pr = Pair(case, precision=L.FP32)
     pr.init()
