@@ -38,7 +38,31 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--profile-json", default=None, help="also write per-stage timings here")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="instead: run the streaming-copy calibration kernel on N MiB and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib: int, reps: int = 5):
+    """Known-size copy kernel: the HBM-counter calibration and achievable-bandwidth reference."""
+    from openstc_amd import _lib as L
+
+    lib = L.load()
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    L.check(lib.ostc_device_alloc(C.byref(a), nbytes), "alloc")
+    L.check(lib.ostc_device_alloc(C.byref(b), nbytes), "alloc")
+    L.check(lib.ostc_stream_copy(b, a, nbytes, None), "copy")
+    L.check(lib.ostc_device_sync(), "sync")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.check(lib.ostc_stream_copy(b, a, nbytes, None), "copy")
+    L.check(lib.ostc_device_sync(), "sync")
+    dt = (time.perf_counter() - t0) / reps
+    lib.ostc_device_free(a)
+    lib.ostc_device_free(b)
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                      "avg_s": dt, "GBs_read_plus_write": 2 * nbytes / dt / 1e9}), flush=True)
 
 
 # Algorithmic scalars per cell (distinct values read + written, stencil
@@ -108,6 +132,9 @@ def cpu_baseline(case_builder, steps: int):
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only (one block per GPU is the scale-out "
                                    "design; multi-GPU halos are not built)", "n_gpus": a.gpus}))

@@ -241,6 +241,16 @@ int ostc_chem_plain_step(ostc_ctx *ctx, long n, const double *t, const double *p
 int ostc_chem_source_batch(ostc_ctx *ctx, long n, const double *rho, const double *t, const double *y,
                            double *omega, double *jac, void *stream);
 
+/* ---- device memory utilities (no reference counterpart; used by callers
+ * that hand batches of chemistry states to the library) ---- */
+int ostc_device_alloc(void **ptr, long bytes);
+int ostc_device_free(void *ptr);
+int ostc_memcpy_h2d(void *dst, const void *src, long bytes);
+int ostc_memcpy_d2h(void *dst, const void *src, long bytes);
+/* Streaming copy of `bytes` (multiple of 16): the calibration kernel for HBM
+ * traffic counters and the achievable-bandwidth reference. */
+int ostc_stream_copy(void *dst, const void *src, long bytes, void *stream);
+
 int ostc_device_sync(void);
 const char *ostc_last_error(void);
 const char *ostc_build_info(void);

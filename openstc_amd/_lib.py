@@ -85,6 +85,14 @@ class InnerFace(C.Structure):
                 ("n_point", C.c_int * 3)]
 
 
+MAX_ELEM = 8
+
+
+class MechanismInfo(C.Structure):
+    _fields_ = [("n_spec", C.c_int), ("n_elem", C.c_int), ("spec_names", (C.c_char * 24) * MAX_SPEC),
+                ("elem_names", (C.c_char * 4) * MAX_ELEM), ("elem_comp", C.c_int * (MAX_SPEC * MAX_ELEM))]
+
+
 class ZoneDesc(C.Structure):
     _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("ngg", C.c_int),
                 ("precision", C.c_int), ("block_id", C.c_int)]
@@ -135,6 +143,14 @@ SIGNATURES = {
     "ostc_profile_read": (C.c_int, [_P, _D, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
     "ostc_profile_stage_name": (C.c_char_p, [C.c_int]),
     "ostc_profile_stage_count": (C.c_int, []),
+    "ostc_device_alloc": (C.c_int, [C.POINTER(_P), C.c_long]),
+    "ostc_device_free": (C.c_int, [_P]),
+    "ostc_memcpy_h2d": (C.c_int, [_P, _P, C.c_long]),
+    "ostc_memcpy_d2h": (C.c_int, [_P, _P, C.c_long]),
+    "ostc_stream_copy": (C.c_int, [_P, _P, C.c_long, _P]),
+    "ostc_read_chemkin": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(Param),
+                                    C.POINTER(MechanismInfo)]),
+    "ostc_chemkin_error": (C.c_char_p, []),
     "ostc_device_sync": (C.c_int, []),
     "ostc_last_error": (C.c_char_p, []),
     "ostc_build_info": (C.c_char_p, []),

@@ -352,3 +352,186 @@ def config2(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, 
     wall = np.stack([x[:, 0, :].ravel(), y[:, 0, :].ravel(), z[:, 0, :].ravel()], axis=1)
     return Case("config2_air_sst_dplur", param, [blk], ngg, precision, steps, 100, wall, seed,
                 dict(mach=mach, p=p_inf, T=t_inf, rho=rho, u=u_inf, y_first=y_first, beta=beta))
+
+
+# ---------------------------------------------------------------- mixtures
+
+import os as _os
+
+DATA = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "data", "chemkin")
+
+
+def load_mechanism(with_reactions=True, directory: str = DATA):
+    """CHEMKIN inputs -> (Param with the chemistry tables filled, MechanismInfo)."""
+    lib = L.load()
+    p = L.Param()
+    info = L.MechanismInfo()
+    st = lib.ostc_read_chemkin(_os.path.join(directory, "combustion_mechanism.inp").encode(),
+                               _os.path.join(directory, "therm.dat").encode(),
+                               _os.path.join(directory, "tran.dat").encode(), 1 if with_reactions else 0,
+                               C.byref(p), C.byref(info))
+    if st != 0:
+        raise L.OstcError("ostc_read_chemkin: " + lib.ostc_chemkin_error().decode())
+    return p, info
+
+
+def species_names(info):
+    return [info.spec_names[i].value.decode() for i in range(info.n_spec)]
+
+
+def nasa_cp(p: L.Param, t: float) -> np.ndarray:
+    """cp_k(T) [J/kg/K] (Species::compute_cp, src/ChemData.cpp:100-117)."""
+    ns = p.n_spec
+    out = np.empty(ns)
+    for i in range(ns):
+        tt = max(t, p.t_low[i])
+        c = p.low_temp_coeff if tt < p.t_mid[i] else p.high_temp_coeff
+        c = [c[i * 7 + q] for q in range(5)]
+        out[i] = (c[0] + c[1] * tt + c[2] * tt ** 2 + c[3] * tt ** 3 + c[4] * tt ** 4) * RU / p.mw[i]
+    return out
+
+
+def mixture_viscosity(p: L.Param, t: float, y: np.ndarray) -> float:
+    """Wilke mixture viscosity (host compute_viscosity, src/Transport.cu:11-38)."""
+    ns = p.n_spec
+    mw = np.array(p.mw[:ns])
+    mwt = 1.0 / np.sum(y / mw)
+    x = y * mwt / mw
+    vis = np.empty(ns)
+    for i in range(ns):
+        tdl = t * p.LJ_potent_inv[i]
+        ci = 1.147 * tdl ** -0.145 + (tdl + 0.5) ** -2
+        vis[i] = p.vis_coeff[i] * math.sqrt(t) / ci
+    mu = 0.0
+    for i in range(ns):
+        s = 0.0
+        for j in range(ns):
+            if i == j:
+                pf = 1.0
+            else:
+                num = 1 + math.sqrt(vis[i] / vis[j]) * p.WjDivWi_to_One4th[i * L.MAX_SPEC + j]
+                pf = num * num * p.sqrt_WiDivWjPl1Mul8[i * L.MAX_SPEC + j]
+            s += pf * x[j]
+        mu += vis[i] * x[i] / s
+    return mu
+
+
+def mixture_inflow(p: L.Param, y: np.ndarray, mach, pressure, temperature, direction=(1.0, 0.0, 0.0), rans=False,
+                   intensity=0.01, mut_ratio=1.0):
+    """Inflow::Inflow for a mixture given (Y, Ma, p, T) (src/BoundCond.cpp:6-158)."""
+    ns = p.n_spec
+    mwk = np.array(p.mw[:ns])
+    mw = 1.0 / np.sum(y / mwk)
+    viscosity = mixture_viscosity(p, temperature, y)
+    cpi = nasa_cp(p, temperature)
+    cp = np.sum(y * cpi)
+    cv = np.sum(y * (cpi - RU / mwk))
+    gamma = cp / cv
+    c = math.sqrt(gamma * RU / mw * temperature)
+    velocity = mach * c
+    u, v, w = (d * velocity for d in direction)
+    density = pressure * mw / (RU * temperature)
+    st = InflowState(density, u, v, w, pressure, temperature, velocity)
+    st.sv = list(y)
+    if rans:
+        st.mut = mut_ratio * viscosity
+        k = 1.5 * velocity * velocity * intensity * intensity
+        st.sv += [k, density * k / st.mut]
+    return st
+
+
+def mixture_param(mech: L.Param, rans: bool, implicit: bool, dplur_steps: int, cfl: float, chem_method: int,
+                  freestream: InflowState, reconstruction=2) -> L.Param:
+    p = L.Param()
+    C.memmove(C.addressof(p), C.addressof(mech), C.sizeof(L.Param))
+    p.mix_model = L.FR if mech.n_reac > 0 else L.MIXTURE
+    p.turb_method = L.RANS if rans else L.LAMINAR
+    p.inviscid_scheme = 3
+    p.reconstruction = reconstruction
+    p.limiter = 0
+    p.viscous_scheme = 2
+    p.rans_model = 2 if rans else 0
+    p.turb_implicit = 1
+    p.chem_src_method = chem_method
+    p.implicit_method = 1 if implicit else 0
+    p.dplur_inner_step = dplur_steps
+    p.n_scalar = p.n_spec + (2 if rans else 0)
+    p.n_var = 5 + p.n_scalar
+    # setup/9_transport_property.txt of the reference's sample case
+    p.Pr, p.cfl, p.Sc, p.Prt, p.Sct = 0.72, cfl, 0.5, 0.9, 0.9
+    fs = freestream
+    p.limit_ll[0] = 1e-6 * fs.density
+    p.limit_ul[0] = 1e+3 * fs.density
+    for l in range(1, 4):
+        p.limit_ll[l] = -1e+3 * fs.velocity
+        p.limit_ul[l] = 1e+3 * fs.velocity
+    p.limit_ll[4] = 1e-6 * fs.pressure
+    p.limit_ul[4] = 1e+3 * fs.pressure
+    if rans:
+        p.limit_ul[5] = np.finfo(np.float64).max
+        p.limit_ul[6] = np.finfo(np.float64).max
+    for l, v in enumerate(fs.sv):
+        p.sv_inf[l] = v
+    return p
+
+
+def config3(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=4, cfl=1.0, chem_method=1,
+            lx=0.2, ly=0.1, lz=0.1, seed=3) -> Case:
+    """Finite-rate chemistry: 9-species/19-reaction H2-air, Cartesian box with
+    a seeded mixing layer: phi = (1+tanh((y-y0)/(2*dy)))/2 between H2 (Y=1,
+    1000 K, Ma 1.2, below y0) and air (O2 0.233 / N2 0.767, 1500 K, Ma 2.0,
+    above y0), p = 1 atm, +-2% perturbation of rho,u,v,w,p; inflow as two
+    labelled patches split at y0 (air label 5, fuel label 9), outflow at
+    i=max and j=0/j=max, symmetry in k; laminar, EPI chemistry, DPLUR."""
+    ngg = 2
+    mech, info = load_mechanism(True)
+    names = species_names(info)
+    ns = mech.n_spec
+    y_air = np.zeros(ns)
+    y_air[names.index("O2")] = 0.233
+    y_air[names.index("N2")] = 0.767
+    y_fuel = np.zeros(ns)
+    y_fuel[names.index("H2")] = 1.0
+    air = mixture_inflow(mech, y_air, 2.0, 101325.0, 1500.0)
+    fuel = mixture_inflow(mech, y_fuel, 1.2, 101325.0, 1000.0)
+    param = mixture_param(mech, rans=False, implicit=True, dplur_steps=dplur_steps, cfl=cfl, chem_method=chem_method,
+                          freestream=air)
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    x = i * (lx / (mx - 1))
+    y = j * (ly / (my - 1))
+    z = k * (lz / (mz - 1))
+    blk = Block(mx, my, mz, x, y, z)
+    jsplit = my // 2
+    rows = [(0, 0, jsplit, my - 1, 0, mz - 1, 5), (0, 0, 0, jsplit, 0, mz - 1, 9),
+            (mx - 1, mx - 1, 0, my - 1, 0, mz - 1, 6),
+            (0, mx - 1, 0, 0, 0, mz - 1, 6), (0, mx - 1, my - 1, my - 1, 0, mz - 1, 6),
+            (0, mx - 1, 0, my - 1, 0, 0, 3), (0, mx - 1, 0, my - 1, mz - 1, mz - 1, 3)]
+    blk.bc = bc_program(rows, [(5, "inflow"), (9, "inflow")], [air, fuel], [], ngg)
+    finish_block(blk, ngg)
+    # mixing-layer initial field on the padded block (ghosts get the profile too)
+    shape = (mz + 2 * ngg, my + 2 * ngg, mx + 2 * ngg)
+    yc = blk.yg
+    y0 = 0.5 * (y[0, jsplit, 0] + y[0, jsplit - 1, 0])
+    dy = ly / (my - 1)
+    phi = 0.5 * (1.0 + np.tanh((yc - y0) / (2 * dy)))
+    sv = np.empty((ns,) + shape)
+    for l in range(ns):
+        sv[l] = phi * y_air[l] + (1 - phi) * y_fuel[l]
+    bv = np.empty((6,) + shape)
+    bv[1] = phi * air.u + (1 - phi) * fuel.u
+    bv[2] = 0.0
+    bv[3] = 0.0
+    bv[4] = 101325.0
+    bv[5] = phi * air.temperature + (1 - phi) * fuel.temperature
+    perturb_interior(bv[1], blk, seed, 1, ngg, rel=0.02)
+    perturb_interior(bv[2], blk, seed, 2, ngg, absval=0.02 * air.u)
+    perturb_interior(bv[3], blk, seed, 3, ngg, absval=0.02 * air.u)
+    perturb_interior(bv[4], blk, seed, 4, ngg, rel=0.02)
+    mwk = np.array(mech.mw[:ns])
+    mw = 1.0 / np.einsum("l...,l->...", sv, 1.0 / mwk)
+    bv[0] = bv[4] * mw / (RU * bv[5])
+    perturb_interior(bv[0], blk, seed, 0, ngg, rel=0.02)
+    bv[5] = bv[4] * mw / (RU * bv[0])
+    blk.bv0, blk.sv0 = bv, sv
+    return Case("config3_h2air_fr_epi_dplur", param, [blk], ngg, precision, steps, 100, None, seed,
+                dict(species=names, air=air.__dict__, fuel=fuel.__dict__, mech_info=info))
