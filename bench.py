@@ -28,6 +28,19 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of a stage's kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on this workload (FETCH_SIZE scaled x2.0 and
+# WRITE_SIZE x1.0, the factors a 512 MiB copy kernel gives).  Keyed by
+# (workload, stage); the summary with the commands is
+# profiles/config2_sst_dplur_256x128x64_kernels.md.  Re-measured whenever the
+# kernel changes; a stage without an entry reports null.
+MEASURED_TRAFFIC = {
+    ("config2_air_sst_dplur", "dplur_sweep"): 1108.0e6,
+    ("config2_air_sst_dplur", "flux_i"): 492.0e6,
+    ("config2_air_sst_dplur", "flux_j"): 547.0e6,
+    ("config2_air_sst_dplur", "flux_k"): 722.0e6,
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -217,7 +230,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(d["frac_hbm_peak"], 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((case.name, dom)),
+            "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, calibrated)",
             "alg_bytes_per_cell": scal[dom] * esize,
             "avg_launch_ms": round(d["avg_ms"], 5),
         },
