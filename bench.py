@@ -49,6 +49,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-chem", action="store_true", help="skip the finite-rate / surrogate leg")
+    ap.add_argument("--chem-steps", type=int, default=10)
+    ap.add_argument("--chem-eval-iter", type=int, default=200,
+                    help="solver iteration at which surrogate and plain chemistry are compared")
+    ap.add_argument("--chem-reps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--profile-json", default=None, help="also write per-stage timings here")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -141,6 +146,89 @@ def cpu_baseline(case_builder, steps: int):
                   f"over (k,j); setup excluded",
         "seconds": dt,
     }
+
+
+SUR_FLOP_PER_CELL = 2 * (12 * 128 + 3 * 128 * 128 + 128 * 10)  # algorithmic MLP forward FLOP per cell
+
+
+def chemistry_leg(steps: int, warmup: int, eval_iter: int, reps: int):
+    """Finite-rate workload (192x96x96 mixing layer, FP32 bulk + FP64
+    chemistry, DPLUR x4, CFL 0.2): whole-iteration throughput, then, on the
+    solver's own state at iteration `eval_iter`, the chemistry step of every
+    cell by the plain FP64 point-implicit integrator vs the surrogate with
+    fallback (same zone, same states)."""
+    import numpy as np
+
+    from openstc_amd import cases, surrogate as S
+    from openstc_amd import _lib as L
+    from openstc_amd.runtime import Simulation
+
+    case = cases.config3(cfl=0.2, seed=5)
+    cells = sum(b.mx * b.my * b.mz for b in case.blocks)
+    print(f"[bench] {case.name}: {cells} cells, setup", file=sys.stderr, flush=True)
+    sim = Simulation(case)
+    sim.initialize_computation()
+    lib = sim.ctx.lib
+    for _ in range(warmup):
+        sim.step(output_screen=case.output_screen)
+    sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(output_screen=case.output_screen)
+    sim.sync()
+    el = time.perf_counter() - t0
+    out = {"workload": case.name, "cells": cells, "steps": steps,
+           "value": round(cells * steps / el / 1e6, 3), "unit": "Mcell-iter/s",
+           "ms_per_step": round(el / steps * 1e3, 4)}
+    done = warmup + steps
+    while done < eval_iter:
+        sim.step(output_screen=case.output_screen)
+        done += 1
+    sim.sync()
+    if not os.path.exists(S.DEFAULT_WEIGHTS):
+        out["surrogate"] = None
+        sim.close()
+        return out
+    sur = S.Surrogate.load(sim.ctx)
+    z = sim.zones[0]
+    dy_p = S.DeviceArray(nbytes=cells * 9 * 8)
+    dy_s = S.DeviceArray(nbytes=cells * 9 * 8)
+    stats = (C.c_longlong * 4)()
+
+    def run(mode, buf, st):
+        L.check(lib.ostc_chem_zone_step(sim.ctx.h, z.h, sur.h, mode, buf.p, st, None), "zone chemistry")
+
+    run(0, dy_p, None)
+    run(1, dy_s, stats)
+    times = {}
+    for mode, buf in ((0, dy_p), (1, dy_s)):
+        L.check(lib.ostc_device_sync(), "sync")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run(mode, buf, None)
+        L.check(lib.ostc_device_sync(), "sync")
+        times[mode] = (time.perf_counter() - t0) / reps
+    run(1, dy_s, stats)
+    ref = dy_p.get(np.float64, (cells, 9))
+    got = dy_s.get(np.float64, (cells, 9))
+    rep = S.error_report(got, ref, list(stats))
+    out["surrogate"] = {
+        "eval_iteration": eval_iter,
+        "plain_ms": round(times[0] * 1e3, 4), "surrogate_ms": round(times[1] * 1e3, 4),
+        "speedup": round(times[0] / times[1], 3),
+        "plain_Mcell_s": round(cells / times[0] / 1e6, 2), "surrogate_Mcell_s": round(cells / times[1] / 1e6, 2),
+        "surrogate_TFLOPs_incl_fallback": round(SUR_FLOP_PER_CELL * cells / times[1] / 1e12, 2),
+        "mfma_peak_TFLOPs_bf16": 2500.0,
+        **{k: (round(v, 6) if isinstance(v, float) else v) for k, v in rep.items()},
+        "error_definition": "rel_l2 = ||dY_sur - dY_plain||_2 / ||dY_plain||_2 over all cells and species "
+                            "(dY/dt is the effective production rate); cell_* = max_k|err_k| / max_k|dY_plain_k| "
+                            "over cells with max|dY_plain| > 1e-8",
+    }
+    for b in (dy_p, dy_s):
+        b.free()
+    sur.close()
+    sim.close()
+    return out
 
 
 def main():
@@ -239,6 +327,8 @@ def main():
                    for k, v in stages.items()},
     }
     sim.close()
+    if not a.no_chem:
+        out["chemistry"] = chemistry_leg(a.chem_steps, 2, a.chem_eval_iter, a.chem_reps)
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(build, a.cpu_steps)

@@ -124,6 +124,7 @@ enum ostc_field {
 
 typedef struct ostc_ctx ostc_ctx;
 typedef struct ostc_zone ostc_zone;
+typedef struct ostc_surrogate ostc_surrogate;
 
 /* ---- context and zones: stand in for Driver's ctor (src/Driver.cu:14-41),
  * Field::setup_device_memory (src/Field.cu:69-171) and the DParameter upload
@@ -240,6 +241,72 @@ int ostc_chem_plain_step(ostc_ctx *ctx, long n, const double *t, const double *p
  * device pointers (src/FiniteRateChem.cu:7-63,173-194). */
 int ostc_chem_source_batch(ostc_ctx *ctx, long n, const double *rho, const double *t, const double *y,
                            double *omega, double *jac, void *stream);
+
+/* Chemistry step of every interior cell of a zone from its T, p, Y and local
+ * dt (the zone's current state): mode 0 = the plain point-implicit step
+ * (FP64), mode 1 = the learned surrogate with per-cell fallback to that step.
+ * dy[n_interior*9] (device), cells in i-fastest order.  stats[4] (host, may be
+ * NULL; mode 1 only): cells, fallbacks for input range, output range, element
+ * balance.  No counterpart in the reference: it is the speed/accuracy harness
+ * of the surrogate on the solver's own states. */
+int ostc_chem_zone_step(ostc_ctx *ctx, ostc_zone *zone, ostc_surrogate *sur, int mode, double *dy,
+                        long long stats[4], void *stream);
+
+/* ---- learned chemistry surrogate (no counterpart in the reference) ----
+ * MLP 12-128-128-128-128-n_out, GELU (tanh form), on the features
+ * (1000/T, ln(p/1 atm), BoxCox_0.1(Y_k) for the 9 species, ln dt),
+ * standardised with in_mean/in_std; its standardised outputs (out_mean,
+ * out_std) encode the plain step's dY: kind 0 (n_out 10): log10 of
+ * max(max_k|dY_k|, floor), then dY_k divided by that magnitude; kind 1
+ * (n_out 9): asinh(dY_k/floor).  Inference: BF16 weights and activations,
+ * FP32 accumulation (MFMA).  A cell falls back to the FP64 plain step when
+ * its input lies outside [t_min,t_max] x [p_min,p_max] x [dt_min,dt_max]
+ * (or Y_k < -y_tol), when Y_k+dY_k leaves [-y_tol, 1+y_tol], or when an
+ * element balance |sum_k elem_w[j][k] dY_k| exceeds
+ * elem_tol * sum_jk |elem_w[j][k] dY_k| + elem_abs (elem_w = atoms/W_k).
+ * Master weights are FP32, per layer W[out][in] row-major then b[out]. */
+#define OSTC_SUR_IN 12
+#define OSTC_SUR_MAX_OUT 16
+#define OSTC_SUR_MAX_ELEM 4
+typedef struct ostc_surrogate_desc {
+  int n_out, kind, n_elem, reserved;
+  float in_mean[OSTC_SUR_IN], in_std[OSTC_SUR_IN];
+  float out_mean[OSTC_SUR_MAX_OUT], out_std[OSTC_SUR_MAX_OUT];
+  float loss_w[OSTC_SUR_MAX_OUT];             /* weights of the training loss */
+  float elem_w[OSTC_SUR_MAX_ELEM * 9];        /* [element][species] */
+  float floor;
+  float t_min, t_max, p_min, p_max, dt_min, dt_max;
+  float y_tol, elem_tol, elem_abs;
+} ostc_surrogate_desc;
+
+int ostc_surrogate_n_params(int n_out);
+/* weights: host FP32 [ostc_surrogate_n_params(n_out)]; max_batch: largest
+ * training batch (multiple of 16; 0 = inference only). */
+int ostc_surrogate_create(ostc_ctx *ctx, const ostc_surrogate_desc *desc, const float *weights, long max_batch,
+                          ostc_surrogate **out);
+int ostc_surrogate_destroy(ostc_surrogate *sur);
+int ostc_surrogate_get_weights(ostc_surrogate *sur, float *weights);
+int ostc_surrogate_set_weights(ostc_surrogate *sur, const float *weights);
+/* dY of a batch (device arrays as ostc_chem_plain_step) with fallback;
+ * stats as ostc_chem_zone_step. */
+int ostc_surrogate_step(ostc_surrogate *sur, long n, const double *t, const double *p, const double *y,
+                        const double *dt, double *dy, long long stats[4], void *stream);
+/* Standardised network outputs raw[n*16] (device) of a batch, beside the
+ * decoded dy (tests). */
+int ostc_surrogate_forward(ostc_surrogate *sur, long n, const double *t, const double *p, const double *y,
+                           const double *dt, float *raw, double *dy, void *stream);
+/* Gradient of L = sum_b sum_o loss_w[o] (net_o(x_b) - target[r_b][o])^2 / nb
+ * over the batch r_b = idx[b] (idx may be NULL: r_b = b) of the data set
+ * (t, p, y, dt, target[*][16] standardised; device arrays) into grad
+ * (device, n_params); *loss (host, may be NULL) = L.  nb: multiple of 16,
+ * <= max_batch. */
+int ostc_surrogate_gradients(ostc_surrogate *sur, long nb, const double *t, const double *p, const double *y,
+                             const double *dt, const int *idx, const float *target, float *grad, double *loss,
+                             void *stream);
+/* One Adam step (beta 0.9/0.999, eps 1e-8, rate lr) on that gradient. */
+int ostc_surrogate_train_step(ostc_surrogate *sur, long nb, const double *t, const double *p, const double *y,
+                              const double *dt, const int *idx, const float *target, float lr, double *loss,
+                              void *stream);
 
 /* ---- device memory utilities (no reference counterpart; used by callers
  * that hand batches of chemistry states to the library) ---- */

@@ -93,6 +93,20 @@ class MechanismInfo(C.Structure):
                 ("elem_names", (C.c_char * 4) * MAX_ELEM), ("elem_comp", C.c_int * (MAX_SPEC * MAX_ELEM))]
 
 
+SUR_IN, SUR_MAX_OUT, SUR_MAX_ELEM = 12, 16, 4
+
+
+class SurDesc(C.Structure):
+    """struct ostc_surrogate_desc."""
+    _fields_ = [("n_out", C.c_int), ("kind", C.c_int), ("n_elem", C.c_int), ("reserved", C.c_int),
+                ("in_mean", C.c_float * SUR_IN), ("in_std", C.c_float * SUR_IN),
+                ("out_mean", C.c_float * SUR_MAX_OUT), ("out_std", C.c_float * SUR_MAX_OUT),
+                ("loss_w", C.c_float * SUR_MAX_OUT), ("elem_w", C.c_float * (SUR_MAX_ELEM * 9)),
+                ("floor", C.c_float), ("t_min", C.c_float), ("t_max", C.c_float), ("p_min", C.c_float),
+                ("p_max", C.c_float), ("dt_min", C.c_float), ("dt_max", C.c_float),
+                ("y_tol", C.c_float), ("elem_tol", C.c_float), ("elem_abs", C.c_float)]
+
+
 class ZoneDesc(C.Structure):
     _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("ngg", C.c_int),
                 ("precision", C.c_int), ("block_id", C.c_int)]
@@ -143,6 +157,16 @@ SIGNATURES = {
     "ostc_profile_read": (C.c_int, [_P, _D, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
     "ostc_profile_stage_name": (C.c_char_p, [C.c_int]),
     "ostc_profile_stage_count": (C.c_int, []),
+    "ostc_chem_zone_step": (C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_longlong), _P]),
+    "ostc_surrogate_n_params": (C.c_int, [C.c_int]),
+    "ostc_surrogate_create": (C.c_int, [_P, C.POINTER(SurDesc), C.POINTER(C.c_float), C.c_long, C.POINTER(_P)]),
+    "ostc_surrogate_destroy": (C.c_int, [_P]),
+    "ostc_surrogate_get_weights": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "ostc_surrogate_set_weights": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "ostc_surrogate_step": (C.c_int, [_P, C.c_long, _P, _P, _P, _P, _P, C.POINTER(C.c_longlong), _P]),
+    "ostc_surrogate_forward": (C.c_int, [_P, C.c_long, _P, _P, _P, _P, _P, _P, _P]),
+    "ostc_surrogate_gradients": (C.c_int, [_P, C.c_long, _P, _P, _P, _P, _P, _P, _P, _D, _P]),
+    "ostc_surrogate_train_step": (C.c_int, [_P, C.c_long, _P, _P, _P, _P, _P, _P, C.c_float, _D, _P]),
     "ostc_device_alloc": (C.c_int, [C.POINTER(_P), C.c_long]),
     "ostc_device_free": (C.c_int, [_P]),
     "ostc_memcpy_h2d": (C.c_int, [_P, _P, C.c_long]),

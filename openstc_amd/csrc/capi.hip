@@ -1,5 +1,6 @@
 // C-ABI of the residual-and-update path: contexts, zones, field I/O and the
 // per-stage entry points declared in include/ostc.h.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -7,6 +8,7 @@
 #include <vector>
 
 #include "ostc_kernels.h"
+#include "ostc_surrogate.h"
 
 using namespace ostc;
 
@@ -684,6 +686,245 @@ int ostc_chem_source_batch(ostc_ctx *c, long n, const double *rho, const double 
   if (!c || n < 0 || (n > 0 && (!rho || !t || !y || !omega))) return fail(OSTC_ERR_ARG, "bad arguments");
   if (c->host.n_spec != 9) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
   return chem_source_batch(c->dev, c->host.n_spec, c->host.n_reac, n, rho, t, y, omega, jac, S(s));
+}
+
+// ---- learned chemistry surrogate ------------------------------------------
+}  // extern "C"
+
+struct ostc_surrogate {
+  ostc_ctx *ctx = nullptr;
+  ostc::SurState st;
+  std::vector<void *> owned;
+};
+
+namespace {
+
+template <typename T>
+hipError_t sur_alloc(ostc_surrogate *s, T **p, size_t count) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e == hipSuccess) {
+    s->owned.push_back(q);
+    *p = static_cast<T *>(q);
+  }
+  return e;
+}
+
+int sur_check_desc(const ostc_surrogate_desc &d) {
+  if (d.kind == 0 && d.n_out != 10) return fail(OSTC_ERR_ARG, "surrogate kind 0 has 10 outputs");
+  if (d.kind == 1 && d.n_out != 9) return fail(OSTC_ERR_ARG, "surrogate kind 1 has 9 outputs");
+  if (d.kind < 0 || d.kind > 1) return fail(OSTC_ERR_ARG, "surrogate kind must be 0 or 1");
+  if (d.n_elem < 0 || d.n_elem > OSTC_SUR_MAX_ELEM) return fail(OSTC_ERR_ARG, "n_elem out of range");
+  for (int f = 0; f < OSTC_SUR_IN; ++f)
+    if (!(d.in_std[f] > 0)) return fail(OSTC_ERR_ARG, "in_std must be positive");
+  return OSTC_OK;
+}
+
+ostc::SurParams sur_params(const ostc_surrogate_desc &d) {
+  ostc::SurParams p{};
+  p.n_out = d.n_out;
+  p.kind = d.kind;
+  p.n_elem = d.n_elem;
+  for (int f = 0; f < OSTC_SUR_IN; ++f) {
+    p.in_mean[f] = d.in_mean[f];
+    p.in_rstd[f] = (float)(1.0 / (double)d.in_std[f]);
+  }
+  for (int o = 0; o < 16; ++o) {
+    p.out_mean[o] = d.out_mean[o];
+    p.out_std[o] = d.out_std[o];
+  }
+  for (int q = 0; q < OSTC_SUR_MAX_ELEM * 9; ++q) p.elem_w[q] = d.elem_w[q];
+  p.floor = d.floor;
+  p.t_min = d.t_min, p.t_max = d.t_max, p.p_min = d.p_min, p.p_max = d.p_max;
+  p.dt_min = d.dt_min, p.dt_max = d.dt_max;
+  p.y_tol = d.y_tol, p.elem_tol = d.elem_tol, p.elem_abs = d.elem_abs;
+  return p;
+}
+
+int sur_ensure_fallback(ostc_surrogate *s, long n) {
+  if (n <= s->st.fb_cap) return OSTC_OK;
+  HIP_TRY(hipDeviceSynchronize());
+  if (s->st.fb_list) {
+    for (auto &q : s->owned)
+      if (q == s->st.fb_list) q = nullptr;
+    HIP_TRY(hipFree(s->st.fb_list));
+    s->st.fb_list = nullptr;
+  }
+  HIP_TRY(sur_alloc(s, &s->st.fb_list, (size_t)n));
+  s->st.fb_cap = n;
+  return OSTC_OK;
+}
+
+int sur_read_stats(ostc_surrogate *s, long long stats[4], hipStream_t st) {
+  if (!stats) return OSTC_OK;
+  unsigned long long h[4];
+  HIP_TRY(hipMemcpyAsync(h, s->st.stats, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int q = 0; q < 4; ++q) stats[q] = (long long)h[q];
+  return OSTC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ostc_surrogate_n_params(int n_out) { return (n_out > 0 && n_out <= 16) ? ostc::sur_n_params(n_out) : 0; }
+
+int ostc_surrogate_create(ostc_ctx *ctx, const ostc_surrogate_desc *desc, const float *weights, long max_batch,
+                          ostc_surrogate **out) {
+  if (!ctx || !desc || !weights || !out || max_batch < 0 || max_batch % 16)
+    return fail(OSTC_ERR_ARG, "bad arguments");
+  if (ctx->host.n_spec != 9 || ctx->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
+  int st = sur_check_desc(*desc);
+  if (st) return st;
+  auto *s = new (std::nothrow) ostc_surrogate;
+  if (!s) return fail(OSTC_ERR_ALLOC, "host allocation");
+  s->ctx = ctx;
+  ostc::SurState &S = s->st;
+  S.params = sur_params(*desc);
+  S.n_params = ostc::sur_n_params(desc->n_out);
+  S.max_batch = max_batch;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&S.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || S.cus <= 0)
+    S.cus = 256;
+  const size_t np = (size_t)S.n_params;
+  hipError_t e = hipSuccess;
+  auto A = [&](auto **p, size_t n) {
+    if (e == hipSuccess) e = sur_alloc(s, p, n);
+  };
+  A(&S.master, np);
+  A(&S.grad, np);
+  A(&S.m1, np);
+  A(&S.m2, np);
+  A(&S.wq, np);
+  char *pk = nullptr;
+  A(&pk, (size_t)ostc::kSurTiles * 64 * 16);
+  S.packed = pk;
+  A(&S.bias, (size_t)ostc::kSurBias);
+  A(&S.loss_w, 16);
+  A(&S.fb_count, 1);
+  A(&S.stats, 4);
+  if (max_batch > 0) {
+    const size_t B = (size_t)max_batch;
+    A(&S.a0, B * 16);
+    A(&S.z, 4 * B * 128);
+    A(&S.a, 4 * B * 144);
+    A(&S.yout, B * 16);
+    A(&S.g5, B * 16);
+    A(&S.gbuf[0], B * 128);
+    A(&S.gbuf[1], B * 128);
+    A(&S.part, (size_t)ostc::kSurSplits * 128 * 144);
+    A(&S.loss_part, (B * 16 + 255) / 256);
+    A(&S.loss_dev, 1);
+  }
+  if (e == hipSuccess) e = hipMemcpy(S.master, weights, np * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(S.m1, 0, np * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(S.m2, 0, np * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(S.loss_w, desc->loss_w, 16 * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    ostc_surrogate_destroy(s);
+    return hip_fail(e, "surrogate setup");
+  }
+  st = ostc::sur_pack(S, nullptr);
+  if (st == OSTC_OK && hipDeviceSynchronize() != hipSuccess) st = OSTC_ERR_HIP;
+  if (st) {
+    ostc_surrogate_destroy(s);
+    return fail(st, "surrogate weight packing failed");
+  }
+  *out = s;
+  return OSTC_OK;
+}
+
+int ostc_surrogate_destroy(ostc_surrogate *s) {
+  if (!s) return OSTC_OK;
+  (void)hipDeviceSynchronize();
+  for (void *q : s->owned)
+    if (q) (void)hipFree(q);
+  delete s;
+  return OSTC_OK;
+}
+
+int ostc_surrogate_get_weights(ostc_surrogate *s, float *w) {
+  if (!s || !w) return fail(OSTC_ERR_ARG, "bad arguments");
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(w, s->st.master, (size_t)s->st.n_params * sizeof(float), hipMemcpyDeviceToHost));
+  return OSTC_OK;
+}
+
+int ostc_surrogate_set_weights(ostc_surrogate *s, const float *w) {
+  if (!s || !w) return fail(OSTC_ERR_ARG, "bad arguments");
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(s->st.master, w, (size_t)s->st.n_params * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(s->st.m1, 0, (size_t)s->st.n_params * sizeof(float)));
+  HIP_TRY(hipMemset(s->st.m2, 0, (size_t)s->st.n_params * sizeof(float)));
+  s->st.adam_step = 0;
+  int st = ostc::sur_pack(s->st, nullptr);
+  if (st) return fail(st, "surrogate weight packing failed");
+  HIP_TRY(hipDeviceSynchronize());
+  return OSTC_OK;
+}
+
+int ostc_surrogate_step(ostc_surrogate *s, long n, const double *t, const double *p, const double *y,
+                        const double *dt, double *dy, long long stats[4], void *strm) {
+  if (!s || n < 0 || (n > 0 && (!t || !p || !y || !dt || !dy))) return fail(OSTC_ERR_ARG, "bad arguments");
+  int st = sur_ensure_fallback(s, n);
+  if (st) return st;
+  st = ostc::sur_infer_batch(s->st, s->ctx->dev, n, t, p, y, dt, dy, nullptr, S(strm));
+  if (st) return fail(st, "surrogate inference launch failed");
+  return sur_read_stats(s, stats, S(strm));
+}
+
+int ostc_surrogate_forward(ostc_surrogate *s, long n, const double *t, const double *p, const double *y,
+                           const double *dt, float *raw, double *dy, void *strm) {
+  if (!s || n < 0 || (n > 0 && (!t || !p || !y || !dt || !raw || !dy))) return fail(OSTC_ERR_ARG, "bad arguments");
+  int st = sur_ensure_fallback(s, n);
+  if (st) return st;
+  st = ostc::sur_infer_batch(s->st, s->ctx->dev, n, t, p, y, dt, dy, raw, S(strm));
+  return st ? fail(st, "surrogate inference launch failed") : OSTC_OK;
+}
+
+int ostc_surrogate_gradients(ostc_surrogate *s, long nb, const double *t, const double *p, const double *y,
+                             const double *dt, const int *idx, const float *target, float *grad, double *loss,
+                             void *strm) {
+  if (!s || !t || !p || !y || !dt || !target || !grad) return fail(OSTC_ERR_ARG, "bad arguments");
+  if (nb <= 0 || nb % 16 || nb > s->st.max_batch)
+    return fail(OSTC_ERR_ARG, "batch must be a positive multiple of 16 within max_batch");
+  int st = ostc::sur_gradients(s->st, nb, t, p, y, dt, idx, target, grad, s->st.loss_dev, S(strm));
+  if (st) return fail(st, "surrogate gradient launch failed");
+  if (loss) {
+    HIP_TRY(hipMemcpyAsync(loss, s->st.loss_dev, sizeof(double), hipMemcpyDeviceToHost, S(strm)));
+    HIP_TRY(hipStreamSynchronize(S(strm)));
+  }
+  return OSTC_OK;
+}
+
+int ostc_surrogate_train_step(ostc_surrogate *s, long nb, const double *t, const double *p, const double *y,
+                              const double *dt, const int *idx, const float *target, float lr, double *loss,
+                              void *strm) {
+  if (!s) return fail(OSTC_ERR_ARG, "bad arguments");
+  int st = ostc_surrogate_gradients(s, nb, t, p, y, dt, idx, target, s->st.grad, loss, strm);
+  if (st) return st;
+  st = ostc::sur_adam(s->st, lr, S(strm));
+  return st ? fail(st, "surrogate update launch failed") : OSTC_OK;
+}
+
+int ostc_chem_zone_step(ostc_ctx *c, ostc_zone *z, ostc_surrogate *s, int mode, double *dy, long long stats[4],
+                        void *strm) {
+  if (!c || !z || !dy || (mode != 0 && mode != 1) || (mode == 1 && !s)) return fail(OSTC_ERR_ARG, "bad arguments");
+  if (c->host.n_spec != 9 || c->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
+  const long n = (long)z->desc.mx * z->desc.my * z->desc.mz;
+  int st;
+  if (mode == 0) {
+    st = fp64(z) ? ostc::zone_plain_chem(c->dev, z->vd, dy, S(strm)) : ostc::zone_plain_chem(c->dev, z->vf, dy, S(strm));
+    return st ? fail(st, "plain chemistry launch failed") : OSTC_OK;
+  }
+  st = sur_ensure_fallback(s, n);
+  if (st) return st;
+  st = fp64(z) ? ostc::sur_infer_zone(s->st, c->dev, z->vd, dy, S(strm))
+               : ostc::sur_infer_zone(s->st, c->dev, z->vf, dy, S(strm));
+  if (st) return fail(st, "surrogate inference launch failed");
+  return sur_read_stats(s, stats, S(strm));
 }
 
 }  // extern "C"

@@ -18,9 +18,10 @@ struct CellIdx {
 template <typename R>
 __device__ __forceinline__ CellIdx<R> cell_index(const ZoneView<R> &z, int halo) {
   CellIdx<R> c;
-  c.i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - halo;
-  c.j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - halo;
-  c.k = (int)blockIdx.z - halo;
+  const uint3 b = xcd_block();
+  c.i = (int)(b.x * blockDim.x + threadIdx.x) - halo;
+  c.j = (int)(b.y * blockDim.y + threadIdx.y) - halo;
+  c.k = (int)b.z - halo;
   c.ok = c.i < z.mx + halo && c.j < z.my + halo && c.k < z.mz + halo;
   return c;
 }

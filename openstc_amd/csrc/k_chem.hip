@@ -8,8 +8,7 @@ namespace ostc {
 
 namespace {
 
-// (I - dt*J) d(rhoY) = dt*omega, dY = d(rhoY)/rho: the reference's explicit
-// EPI (src/FiniteRateChem.cu:211-234) applied to the chemical source alone.
+// Batch of plain point-implicit steps (plain_chem_step), AoS [cell][species].
 template <int NS>
 __global__ __launch_bounds__(256) void k_plain_step(const ostc_param *__restrict__ p, long n,
                                                     const double *__restrict__ t, const double *__restrict__ pr,
@@ -17,29 +16,12 @@ __global__ __launch_bounds__(256) void k_plain_step(const ostc_param *__restrict
                                                     double *__restrict__ dy) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  double yy[NS], mw = 0;
+  double yy[NS], out[NS];
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    yy[l] = y[c * NS + l];
-    mw += yy[l] / p->mw[l];
-  }
-  mw = 1 / mw;
-  const double tc = t[c], dtc = dt[c];
-  const double rho = pr[c] * mw / (kRu * tc);
-  double om[NS], a[NS * NS];
-  int piv[NS];
-  chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, tc, yy, om, a, nullptr);
+  for (int l = 0; l < NS; ++l) yy[l] = y[c * NS + l];
+  plain_chem_step<NS>(p, t[c], pr[c], yy, dt[c], out);
 #pragma unroll
-  for (int q = 0; q < NS * NS; ++q) a[q] = -dtc * a[q];
-#pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    a[q * NS + q] += 1.0;
-    om[q] *= dtc;
-  }
-  lu_factor<NS>(a, piv, NS);
-  lu_solve<NS>(a, piv, om, NS);
-#pragma unroll
-  for (int l = 0; l < NS; ++l) dy[c * NS + l] = om[l] / rho;
+  for (int l = 0; l < NS; ++l) dy[c * NS + l] = out[l];
 }
 
 template <int NS>

@@ -291,18 +291,19 @@ __global__ __launch_bounds__(256) void sweep_kernel(ZoneView<R> z, const ostc_pa
   const int ext[3] = {z.mx, z.my, z.mz};
   // tile origin (cells)
   int o[3];
+  const uint3 bk = xcd_block();
   if (DIR == 0) {
-    o[0] = blockIdx.x * L;
-    o[1] = blockIdx.y * TW;
-    o[2] = blockIdx.z;
+    o[0] = bk.x * L;
+    o[1] = bk.y * TW;
+    o[2] = bk.z;
   } else if (DIR == 1) {
-    o[0] = blockIdx.x * TW;
-    o[1] = blockIdx.y * L;
-    o[2] = blockIdx.z;
+    o[0] = bk.x * TW;
+    o[1] = bk.y * L;
+    o[2] = bk.z;
   } else {
-    o[0] = blockIdx.x * TW;
-    o[1] = blockIdx.y;
-    o[2] = blockIdx.z * L;
+    o[0] = bk.x * TW;
+    o[1] = bk.y;
+    o[2] = bk.z * L;
   }
   const int A = DIR == 0 ? 1 : 0;  // across index direction inside the tile
   const int nfaces = TW * (L + 1);

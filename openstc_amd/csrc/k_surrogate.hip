@@ -1,0 +1,675 @@
+// Learned chemistry surrogate (no counterpart in the reference): a
+// 12-128-128-128-128-n_out MLP with GELU that maps a chemistry state
+// (T, p, Y_k, dt) to the plain point-implicit step's dY_k.
+//
+// Inference: BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.
+// The network is evaluated in the transposed form Y^T = W X^T: a weight tile
+// is the A operand (read from LDS, where the whole BF16 network lives, 108 KB)
+// and a 16-cell slab of activations is the B operand.  The C layout of one
+// layer (lane holds units 4g..4g+3 of a 16-unit tile for cell lane%16) is
+// re-used directly as the B operand of the next layer: the contraction order
+// inside each 32-wide k chunk is permuted consistently in the packed weights,
+// so activations never leave registers between layers.
+//
+// Cells that fail the input-range, output-range or element-balance checks are
+// appended to a list and re-done by the FP64 plain integrator in a second
+// kernel (no serialisation inside a wavefront).
+//
+// Training: the same forward kernel (saving pre-activations and BF16-rounded
+// activations), a weighted-MSE gradient kernel, FP32-MFMA
+// (v_mfma_f32_16x16x4f32) GEMMs for the input and weight gradients
+// (straight-through for the BF16 rounding), deterministic split reductions
+// and Adam on FP32 master weights.
+#include <algorithm>
+#include <cmath>
+
+#include "ostc_kernels.h"
+#include "ostc_surrogate.h"
+#include "ostc_thermo.h"
+
+namespace ostc {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNs = 9;
+constexpr int kWaves = 8;              // waves per workgroup of the fused kernel
+constexpr int kTiles = 4;              // 16-cell slabs per wave pass (64 cells)
+constexpr int kStageLd = 17;           // padded LDS row of the output staging
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// GELU, tanh form written as z*sigmoid(u), u = 2*sqrt(2/pi)*(z + 0.044715 z^3)
+__device__ __forceinline__ float gelu_f(float z) {
+  const float u = kGeluC * (z + kGeluA * z * z * z);
+  const float e = fast_exp2(-u * kLog2e);
+  return z * fast_rcp(1.0f + e);
+}
+__device__ __forceinline__ float gelu_grad_f(float z) {
+  const float u = kGeluC * (z + kGeluA * z * z * z);
+  const float s = fast_rcp(1.0f + fast_exp2(-u * kLog2e));
+  return s + z * s * (1.0f - s) * kGeluC * (1.0f + 3.0f * kGeluA * z * z);
+}
+
+__device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
+
+// ---- sources of chemistry states ----------------------------------------
+struct BatchSrc {
+  const double *t, *p, *y, *dt;
+  const int *idx;  // optional gather (training)
+  long n;
+  __device__ long row(long c) const { return idx ? (long)idx[c] : c; }
+  __device__ void load(long c, double &T, double &P, double *Y, double &DT) const {
+    const long r = row(c);
+    T = t[r];
+    P = p[r];
+    DT = dt[r];
+#pragma unroll
+    for (int k = 0; k < kNs; ++k) Y[k] = y[r * kNs + k];
+  }
+};
+
+template <typename R>
+struct ZoneSrc {
+  const R *bv, *sv, *dt;
+  int mx, my, g, nx, ny;
+  long plane, n;
+  __device__ long at(long c) const {
+    const long i = c % mx, jk = c / mx;
+    const long j = jk % my, k = jk / my;
+    return ((k + g) * ny + (j + g)) * (long)nx + (i + g);
+  }
+  __device__ void load(long c, double &T, double &P, double *Y, double &DT) const {
+    const long q = at(c);
+    T = (double)bv[5 * plane + q];
+    P = (double)bv[4 * plane + q];
+    DT = (double)dt[q];
+#pragma unroll
+    for (int k = 0; k < kNs; ++k) Y[k] = (double)sv[k * plane + q];
+  }
+};
+
+// The 12 raw features in FP32, standardised: 1000/T, ln(p/1 atm),
+// BoxCox_0.1(Y_k) = (Y^0.1 - 1)/0.1, ln dt.
+__device__ __forceinline__ float feature(const SurParams &s, int f, double T, double P, const double *Y, double DT) {
+  float v;
+  if (f == 0) {
+    v = 1000.0f / (float)T;
+  } else if (f == 1) {
+    v = logf((float)P * kInvAtm);
+  } else if (f < 11) {
+    const float y = fmaxf((float)Y[f - 2], 0.0f);
+    v = (powf(y, kBoxCox) - 1.0f) * kInvBoxCox;
+  } else {
+    v = logf((float)DT);
+  }
+  return (v - s.in_mean[f]) * s.in_rstd[f];
+}
+
+// ---- fused forward --------------------------------------------------------
+struct TrainOut {
+  float *a0;    // [n][16]  features (BF16-rounded), 1, 0...
+  float *z;     // [4][n][128] pre-activations of the hidden layers
+  float *a;     // [4][n][144] BF16-rounded activations, 1, 0...
+  float *yout;  // [n][16] standardised outputs
+};
+
+struct InferOut {
+  double *dy;                    // [n][9]
+  int *fb_count;                 // fallback list length
+  long *fb_list;                 // fallback cells
+  unsigned long long *stats;     // [4]
+  float *raw;                    // optional [n][16] standardised outputs
+};
+
+// LDS image offsets (in 1 KB A tiles of 64 lanes x 8 BF16)
+__device__ __forceinline__ int tile_l0(int mt) { return mt; }
+__device__ __forceinline__ int tile_h(int l, int mt, int kc) { return 8 + (l - 1) * 32 + mt * 4 + kc; }
+__device__ __forceinline__ int tile_out(int kc) { return 104 + kc; }
+
+template <typename Src, bool TRAIN>
+__global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src src, const bf16x8 *__restrict__ wpk,
+                                                            const float *__restrict__ bias, TrainOut tr,
+                                                            InferOut io) {
+  __shared__ bf16x8 sW[kSurTiles * 64];
+  __shared__ float sB[kSurBias];
+  __shared__ float sStage[kWaves][64 * kStageLd];
+  for (int q = threadIdx.x; q < kSurTiles * 64; q += blockDim.x) sW[q] = wpk[q];
+  for (int q = threadIdx.x; q < kSurBias; q += blockDim.x) sB[q] = bias[q];
+  __syncthreads();
+
+  const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63, g = ln >> 4, cn = ln & 15;
+  const long n = src.n;
+  const long stride = (long)gridDim.x * kWaves * 64;
+  for (long base = ((long)blockIdx.x * kWaves + wave) * 64; base < n; base += stride) {
+    // ---- layer 0 operand: slots 8g..8g+7 of the 32-slot chunk = features
+    bf16x8 b0[kTiles];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+      const long c = base + 16 * t + cn;
+      float f8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (c < n && g < 2) {
+        double T, P, DT, Y[kNs];
+        src.load(c, T, P, Y, DT);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int f = 8 * g + j;
+          if (f < kSurIn) f8[j] = feature(s, f, T, P, Y, DT);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) b0[t][j] = (__bf16)f8[j];
+      if (TRAIN && c < n && g < 2) {
+        float4 *dst = reinterpret_cast<float4 *>(tr.a0 + c * 16 + 8 * g);
+        dst[0] = make_float4((float)b0[t][0], (float)b0[t][1], (float)b0[t][2], (float)b0[t][3]);
+        if (g == 0)
+          dst[1] = make_float4((float)b0[t][4], (float)b0[t][5], (float)b0[t][6], (float)b0[t][7]);
+        else
+          dst[1] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);  // bias column for the weight gradient
+      }
+    }
+
+    bf16x8 bc[4][kTiles];  // current layer input: [k chunk][slab]
+    // ---- layer 0: 12 (padded 32) -> 128
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+      f32x4 acc[2][kTiles];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const bf16x8 a = sW[tile_l0(2 * kc + h) * 64 + ln];
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t)
+          acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0[t], f32x4{0, 0, 0, 0}, 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < kTiles; ++t) {
+        const long c = base + 16 * t + cn;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int u0 = 32 * kc + 16 * h + 4 * g;
+          float zz[4], hh[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            zz[i] = acc[h][t][i] + sB[u0 + i];
+            hh[i] = gelu_f(zz[i]);
+            bc[kc][t][4 * h + i] = (__bf16)hh[i];
+          }
+          if (TRAIN && c < n) {
+            *reinterpret_cast<float4 *>(tr.z + c * 128 + u0) = make_float4(zz[0], zz[1], zz[2], zz[3]);
+            *reinterpret_cast<float4 *>(tr.a + c * 144 + u0) =
+                make_float4((float)bc[kc][t][4 * h], (float)bc[kc][t][4 * h + 1], (float)bc[kc][t][4 * h + 2],
+                            (float)bc[kc][t][4 * h + 3]);
+          }
+        }
+      }
+    }
+    // ---- hidden layers 1..3: 128 -> 128
+#pragma unroll
+    for (int l = 1; l <= 3; ++l) {
+      bf16x8 bn[4][kTiles];
+#pragma unroll
+      for (int kc = 0; kc < 4; ++kc) {
+        f32x4 acc[2][kTiles];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int t = 0; t < kTiles; ++t) acc[h][t] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int ki = 0; ki < 4; ++ki)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const bf16x8 a = sW[tile_h(l, 2 * kc + h, ki) * 64 + ln];
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t)
+              acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[ki][t], acc[h][t], 0, 0, 0);
+          }
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+          const long c = base + 16 * t + cn;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int u0 = 32 * kc + 16 * h + 4 * g;
+            float zz[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              zz[i] = acc[h][t][i] + sB[128 * l + u0 + i];
+              bn[kc][t][4 * h + i] = (__bf16)gelu_f(zz[i]);
+            }
+            if (TRAIN && c < n) {
+              *reinterpret_cast<float4 *>(tr.z + ((long)l * n + c) * 128 + u0) =
+                  make_float4(zz[0], zz[1], zz[2], zz[3]);
+              *reinterpret_cast<float4 *>(tr.a + ((long)l * n + c) * 144 + u0) =
+                  make_float4((float)bn[kc][t][4 * h], (float)bn[kc][t][4 * h + 1], (float)bn[kc][t][4 * h + 2],
+                              (float)bn[kc][t][4 * h + 3]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int kc = 0; kc < 4; ++kc)
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) bc[kc][t] = bn[kc][t];
+    }
+    if (TRAIN && g == 0) {
+      // bias column (1, then zero padding) of the activations feeding layers 1..4
+#pragma unroll
+      for (int t = 0; t < kTiles; ++t) {
+        const long c = base + 16 * t + cn;
+        if (c < n)
+          for (int l = 0; l < 4; ++l) {
+            float4 *dst = reinterpret_cast<float4 *>(tr.a + ((long)l * n + c) * 144 + 128);
+            dst[0] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+            dst[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            dst[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            dst[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          }
+      }
+    }
+    // ---- output layer: 128 -> n_out (padded 16)
+    f32x4 acc[kTiles];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) acc[t] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int ki = 0; ki < 4; ++ki) {
+      const bf16x8 a = sW[tile_out(ki) * 64 + ln];
+#pragma unroll
+      for (int t = 0; t < kTiles; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[ki][t], acc[t], 0, 0, 0);
+    }
+    float *stg = sStage[wave];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+      const long c = base + 16 * t + cn;
+      float o[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) o[i] = acc[t][i] + sB[512 + 4 * g + i];
+      if (TRAIN) {
+        if (c < n) *reinterpret_cast<float4 *>(tr.yout + c * 16 + 4 * g) = make_float4(o[0], o[1], o[2], o[3]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) stg[(16 * t + cn) * kStageLd + 4 * g + i] = o[i];
+      }
+    }
+    if (TRAIN) continue;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    // ---- decode + checks: one lane per cell
+    const long c = base + ln;
+    int why = 0;  // 0 accepted, 1 input range, 2 output range, 3 element balance
+    if (c < n) {
+      double T, P, DT, Y[kNs];
+      src.load(c, T, P, Y, DT);
+      float o[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) o[q] = stg[ln * kStageLd + q];
+      if (io.raw) {
+#pragma unroll
+        for (int q = 0; q < 16; q += 4)
+          *reinterpret_cast<float4 *>(io.raw + c * 16 + q) = make_float4(o[q], o[q + 1], o[q + 2], o[q + 3]);
+      }
+      float dy[kNs];
+      sur_decode(s, o, dy);
+      why = sur_check(s, T, P, DT, Y, dy);
+      if (why == 0) {
+#pragma unroll
+        for (int k = 0; k < kNs; ++k) io.dy[c * kNs + k] = (double)dy[k];
+      }
+    }
+    const unsigned long long fail = __ballot(why != 0);
+    if (fail) {
+      const int first = __ffsll((long long)fail) - 1;
+      int at = 0;
+      if (ln == first) at = atomicAdd(io.fb_count, __popcll(fail));
+      at = __shfl(at, first);
+      if (why != 0) io.fb_list[at + __popcll(fail & ((1ull << ln) - 1ull))] = c;
+    }
+    if (io.stats && ln == 0) {
+      const unsigned long long valid = __ballot(c < n);
+      atomicAdd(&io.stats[0], (unsigned long long)__popcll(valid));
+    }
+    if (io.stats) {
+#pragma unroll
+      for (int r = 1; r <= 3; ++r) {
+        const unsigned long long m = __ballot(why == r);
+        if (m && ln == 0) atomicAdd(&io.stats[r], (unsigned long long)__popcll(m));
+      }
+    }
+  }
+}
+
+// Plain point-implicit step on the fallback list (grid-stride over a count
+// that lives on the device).
+template <typename Src>
+__global__ __launch_bounds__(256) void k_sur_fallback(const ostc_param *__restrict__ p, Src src,
+                                                      const int *__restrict__ count, const long *__restrict__ list,
+                                                      double *__restrict__ dyo) {
+  const int m = *count;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x) {
+    const long c = list[q];
+    double T, P, DT, Y[kNs], dy[kNs];
+    src.load(c, T, P, Y, DT);
+    plain_chem_step<kNs>(p, T, P, Y, DT, dy);
+#pragma unroll
+    for (int k = 0; k < kNs; ++k) dyo[c * kNs + k] = dy[k];
+  }
+}
+
+template <typename Src>
+__global__ __launch_bounds__(256) void k_zone_plain(const ostc_param *__restrict__ p, Src src,
+                                                    double *__restrict__ dyo) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= src.n) return;
+  double T, P, DT, Y[kNs], dy[kNs];
+  src.load(c, T, P, Y, DT);
+  plain_chem_step<kNs>(p, T, P, Y, DT, dy);
+#pragma unroll
+  for (int k = 0; k < kNs; ++k) dyo[c * kNs + k] = dy[k];
+}
+
+__global__ void k_sur_reset(int *count, unsigned long long *stats) {
+  if (threadIdx.x == 0) *count = 0;
+  if (threadIdx.x < 4) stats[threadIdx.x] = 0;
+}
+
+// ---- training kernels -----------------------------------------------------
+// dL/dY of L = sum_b sum_o w_o (y - t)^2 / B; per-block partial losses.
+__global__ __launch_bounds__(256) void k_sur_loss(const float *__restrict__ yout, const float *__restrict__ target,
+                                                  const int *__restrict__ idx, const float *__restrict__ w, int n_out,
+                                                  long nb, float *__restrict__ g5, double *__restrict__ part) {
+  __shared__ double red[256];
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;  // (sample, output) pair
+  double l = 0;
+  if (q < nb * 16) {
+    const long b = q >> 4;
+    const int o = (int)(q & 15);
+    float gq = 0.0f;
+    if (o < n_out) {
+      const long r = idx ? (long)idx[b] : b;
+      const float d = yout[b * 16 + o] - target[r * 16 + o];
+      gq = 2.0f * w[o] * d / (float)nb;
+      l = (double)w[o] * (double)d * (double)d / (double)nb;
+    }
+    g5[b * 16 + o] = gq;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ part, int m,
+                                                      double *__restrict__ out) {
+  __shared__ double red[256];
+  double v = 0;
+  for (int q = threadIdx.x; q < m; q += 256) v += part[q];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// C[M x N] = sum_k A(m,k) B(k,n) on FP32 MFMA 16x16x4, one 16x16 tile per
+// wave.  In each group of 4 MFMAs lane group g supplies k = kb + 4g + s
+// (s = 0..3) to both operands, so a lane reads 4 consecutive k.
+enum { GEMM_DGRAD = 0, GEMM_WGRAD = 1 };
+
+struct GemmArgs {
+  int M, N;        // multiples of 16
+  long K;          // multiple of 16
+  int splits;      // split of K (WGRAD)
+  // DGRAD: A(m,k) = wq[k*lda + m] (k < kmax), B(k,n) = g[n*ldb + k];
+  //        out[n*128 + m] = C * gelu'(z[n*128 + m])
+  // WGRAD: A(m,k) = g[k*lda + m], B(k,n) = act[k*ldb + n]; out[s][m*N + n]
+  const float *a, *b, *z;
+  int lda, ldb, kmax;
+  float *out;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sur_gemm(GemmArgs ga) {
+  const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63, g = ln >> 4, cn = ln & 15;
+  const long tiles_m = ga.M / 16, tiles_n = ga.N / 16;
+  const long w = (long)blockIdx.x * 4 + wave;
+  const long total = tiles_m * tiles_n * ga.splits;
+  if (w >= total) return;
+  const long mt = w % tiles_m, rest = w / tiles_m;
+  const long nt = rest % tiles_n, sp = rest / tiles_n;
+  const long cps = (ga.K / 16 + ga.splits - 1) / ga.splits;  // 16-wide k steps per split
+  const long k0 = sp * cps * 16, k1 = k0 + cps * 16 < ga.K ? k0 + cps * 16 : ga.K;
+  const long m = mt * 16 + cn, nn = nt * 16 + cn;
+  f32x4 acc = {0, 0, 0, 0};
+  for (long kb = k0; kb < k1; kb += 16) {
+    float av[4], bv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const long k = kb + 4 * g + s;
+      if (MODE == GEMM_DGRAD) {
+        av[s] = (k < ga.kmax) ? ga.a[k * ga.lda + m] : 0.0f;
+        bv[s] = ga.b[nn * ga.ldb + k];
+      } else {
+        av[s] = ga.a[k * ga.lda + m];
+        bv[s] = ga.b[k * ga.ldb + nn];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], bv[s], acc, 0, 0, 0);
+  }
+  const long mr = mt * 16 + 4 * g;  // rows of this lane's 4 results; column nn
+  if (MODE == GEMM_DGRAD) {
+    float r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = acc[i] * gelu_grad_f(ga.z[nn * 128 + mr + i]);
+    *reinterpret_cast<float4 *>(ga.out + nn * 128 + mr) = make_float4(r[0], r[1], r[2], r[3]);
+  } else {
+    float *o = ga.out + sp * (long)ga.M * ga.N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[(mr + i) * ga.N + nn] = acc[i];
+  }
+}
+
+// grad[off + o*K + i] = sum_s part[s][o*N + i]; bias from column K.
+__global__ __launch_bounds__(256) void k_sur_reduce(const float *__restrict__ part, int splits, int M, int N,
+                                                    int m_used, int k_used, float *__restrict__ grad) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= m_used * (k_used + 1)) return;
+  const int o = q / (k_used + 1), i = q % (k_used + 1);
+  float v = 0.0f;
+  for (int s = 0; s < splits; ++s) v += part[(long)s * M * N + o * N + i];
+  if (i < k_used)
+    grad[o * k_used + i] = v;
+  else
+    grad[m_used * k_used + o] = v;
+}
+
+__global__ __launch_bounds__(256) void k_sur_adam(float *__restrict__ w, const float *__restrict__ gr,
+                                                  float *__restrict__ m1, float *__restrict__ m2, int n, float lr,
+                                                  float c1, float c2) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  const float gq = gr[q];
+  const float a = kAdamB1 * m1[q] + (1.0f - kAdamB1) * gq;
+  const float b = kAdamB2 * m2[q] + (1.0f - kAdamB2) * gq * gq;
+  m1[q] = a;
+  m2[q] = b;
+  w[q] -= lr * (a * c1) / (sqrtf(b * c2) + kAdamEps);
+}
+
+// master (FP32, [out][in] then bias per layer) -> LDS image of BF16 A tiles,
+// FP32 bias image, and the BF16-rounded weights as FP32 for the backward.
+__global__ __launch_bounds__(64) void k_sur_pack(const float *__restrict__ w, int n_out, bf16x8 *__restrict__ wpk,
+                                                 float *__restrict__ bias, float *__restrict__ wq) {
+  const int tile = blockIdx.x, ln = threadIdx.x, g = ln >> 4;
+  int l, mt, kc;
+  if (tile < 8) {
+    l = 0, mt = tile, kc = 0;
+  } else if (tile < 104) {
+    l = 1 + (tile - 8) / 32;
+    mt = ((tile - 8) % 32) / 4;
+    kc = (tile - 8) % 4;
+  } else {
+    l = 4, mt = 0, kc = tile - 104;
+  }
+  const int kin = sur_in_dim(l), mout = sur_out_dim(l, n_out);
+  const float *wl = w + sur_layer_offset(l);
+  const int m = 16 * mt + (ln & 15);
+  bf16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int f = (l == 0) ? 8 * g + j : 32 * kc + ((j < 4) ? 4 * g + j : 16 + 4 * g + (j - 4));
+    v[j] = (__bf16)((m < mout && f < kin) ? wl[m * kin + f] : 0.0f);
+  }
+  wpk[tile * 64 + ln] = v;
+  if (tile == 0) {
+    for (int q = ln; q < kSurBias; q += 64) {
+      const int ll = q < 512 ? q / 128 : 4, o = q < 512 ? q % 128 : q - 512;
+      bias[q] = (o < sur_out_dim(ll, n_out)) ? w[sur_layer_offset(ll) + sur_out_dim(ll, n_out) * sur_in_dim(ll) + o]
+                                              : 0.0f;
+    }
+  }
+  // weights rounded to BF16 (as the forward sees them), one layer slice per tile
+  if (tile < 5) {
+    const int kk = sur_in_dim(tile), mm = sur_out_dim(tile, n_out);
+    const float *src = w + sur_layer_offset(tile);
+    float *dst = wq + sur_layer_offset(tile);
+    for (int q = ln; q < kk * mm; q += 64) dst[q] = bf16_round(src[q]);
+  }
+}
+
+unsigned grid_for(long waves) { return (unsigned)((waves + 3) / 4); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+int sur_pack(const SurState &st, hipStream_t s) {
+  hipLaunchKernelGGL(k_sur_pack, dim3(kSurTiles), dim3(64), 0, s, st.master, st.params.n_out,
+                     reinterpret_cast<bf16x8 *>(st.packed), st.bias, st.wq);
+  return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;
+}
+
+static unsigned forward_grid(long n, int cus) {
+  const long wgs = (n + kWaves * 64 - 1) / (kWaves * 64);
+  return (unsigned)std::max<long>(1, std::min<long>(wgs, cus));
+}
+
+template <typename Src>
+static int infer_impl(const SurState &st, const ostc_param *dp, const Src &src, double *dy, float *raw,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_sur_reset, dim3(1), dim3(64), 0, s, st.fb_count, st.stats);
+  TrainOut tr{};
+  InferOut io{dy, st.fb_count, st.fb_list, st.stats, raw};
+  hipLaunchKernelGGL((k_sur_forward<Src, false>), dim3(forward_grid(src.n, st.cus)), dim3(kWaves * 64), 0, s,
+                     st.params, src, reinterpret_cast<const bf16x8 *>(st.packed), st.bias, tr, io);
+  const long fb_blocks = std::min<long>((src.n + 255) / 256, 4L * st.cus);
+  hipLaunchKernelGGL(k_sur_fallback<Src>, dim3((unsigned)std::max<long>(1, fb_blocks)), dim3(256), 0, s, dp, src,
+                     st.fb_count, st.fb_list, dy);
+  return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;
+}
+
+int sur_infer_batch(const SurState &st, const ostc_param *dp, long n, const double *t, const double *p,
+                    const double *y, const double *dt, double *dy, float *raw, hipStream_t s) {
+  if (n <= 0) return OSTC_OK;
+  BatchSrc src{t, p, y, dt, nullptr, n};
+  return infer_impl(st, dp, src, dy, raw, s);
+}
+
+template <typename R>
+int sur_infer_zone(const SurState &st, const ostc_param *dp, const ZoneView<R> &z, double *dy, hipStream_t s) {
+  ZoneSrc<R> src{z.bv, z.sv, z.dt, z.mx, z.my, z.g, z.nx, z.ny, z.plane, (long)z.mx * z.my * z.mz};
+  return infer_impl(st, dp, src, dy, nullptr, s);
+}
+
+template <typename R>
+int zone_plain_chem(const ostc_param *dp, const ZoneView<R> &z, double *dy, hipStream_t s) {
+  ZoneSrc<R> src{z.bv, z.sv, z.dt, z.mx, z.my, z.g, z.nx, z.ny, z.plane, (long)z.mx * z.my * z.mz};
+  hipLaunchKernelGGL(k_zone_plain<ZoneSrc<R>>, dim3((unsigned)((src.n + 255) / 256)), dim3(256), 0, s, dp, src, dy);
+  return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;
+}
+
+template int sur_infer_zone<float>(const SurState &, const ostc_param *, const ZoneView<float> &, double *,
+                                   hipStream_t);
+template int sur_infer_zone<double>(const SurState &, const ostc_param *, const ZoneView<double> &, double *,
+                                    hipStream_t);
+template int zone_plain_chem<float>(const ostc_param *, const ZoneView<float> &, double *, hipStream_t);
+template int zone_plain_chem<double>(const ostc_param *, const ZoneView<double> &, double *, hipStream_t);
+
+int sur_gradients(const SurState &st, long nb, const double *t, const double *p, const double *y, const double *dt,
+                  const int *idx, const float *target, float *grad, double *loss_dev, hipStream_t s) {
+  if (nb <= 0 || nb % 16 || nb > st.max_batch) return OSTC_ERR_ARG;
+  BatchSrc src{t, p, y, dt, idx, nb};
+  TrainOut tr{st.a0, st.z, st.a, st.yout};
+  InferOut io{};
+  hipLaunchKernelGGL((k_sur_forward<BatchSrc, true>), dim3(forward_grid(nb, st.cus)), dim3(kWaves * 64), 0, s,
+                     st.params, src, reinterpret_cast<const bf16x8 *>(st.packed), st.bias, tr, io);
+  const unsigned lblocks = (unsigned)((nb * 16 + 255) / 256);
+  hipLaunchKernelGGL(k_sur_loss, dim3(lblocks), dim3(256), 0, s, st.yout, target, idx, st.loss_w,
+                     st.params.n_out, nb, st.g5, st.loss_part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, st.loss_part, (int)lblocks, loss_dev);
+  const int n_out = st.params.n_out;
+  const float *gcur = st.g5;
+  int ldg = 16;
+  for (int l = 4; l >= 0; --l) {
+    const int kin = sur_in_dim(l), mout = sur_out_dim(l, n_out);
+    const int M = (mout + 15) / 16 * 16;        // 128 or 16
+    const int N = (kin + 1 + 15) / 16 * 16;     // 144 or 16
+    const float *act = (l == 0) ? st.a0 : st.a + (long)(l - 1) * nb * 144;
+    const int ldh = (l == 0) ? 16 : 144;
+    // weight gradient (split over the batch)
+    GemmArgs wg{};
+    wg.M = M;
+    wg.N = N;
+    wg.K = nb;
+    wg.splits = (int)std::max<long>(1, std::min<long>(nb / 256, kSurSplits));
+    wg.a = gcur;
+    wg.lda = ldg;
+    wg.b = act;
+    wg.ldb = ldh;
+    wg.out = st.part;
+    hipLaunchKernelGGL(k_sur_gemm<GEMM_WGRAD>, dim3(grid_for((long)(M / 16) * (N / 16) * wg.splits)), dim3(256), 0,
+                       s, wg);
+    hipLaunchKernelGGL(k_sur_reduce, dim3((unsigned)((mout * (kin + 1) + 255) / 256)), dim3(256), 0, s, st.part,
+                       wg.splits, M, N, mout, kin, grad + sur_layer_offset(l));
+    if (l == 0) break;
+    // input gradient through W (BF16-rounded) and GELU'
+    GemmArgs dg{};
+    dg.M = 128;
+    dg.N = (int)nb;
+    dg.K = M;
+    dg.splits = 1;
+    dg.a = st.wq + sur_layer_offset(l);
+    dg.lda = kin;
+    dg.kmax = mout;
+    dg.b = gcur;
+    dg.ldb = ldg;
+    dg.z = st.z + (long)(l - 1) * nb * 128;
+    float *gnext = st.gbuf[l & 1];
+    dg.out = gnext;
+    hipLaunchKernelGGL(k_sur_gemm<GEMM_DGRAD>, dim3(grid_for((long)(128 / 16) * (nb / 16))), dim3(256), 0, s, dg);
+    gcur = gnext;
+    ldg = 128;
+  }
+  return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;
+}
+
+int sur_adam(SurState &st, float lr, hipStream_t s) {
+  st.adam_step += 1;
+  const double c1 = 1.0 / (1.0 - std::pow((double)kAdamB1, (double)st.adam_step));
+  const double c2 = 1.0 / (1.0 - std::pow((double)kAdamB2, (double)st.adam_step));
+  const int n = st.n_params;
+  hipLaunchKernelGGL(k_sur_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, st.master, st.grad, st.m1,
+                     st.m2, n, lr, (float)c1, (float)c2);
+  if (hipGetLastError() != hipSuccess) return OSTC_ERR_HIP;
+  return sur_pack(st, s);
+}
+
+}  // namespace ostc

@@ -93,4 +93,15 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
   return (b % kXcd) * per + b / kXcd;
 }
 
+// Workgroup coordinates after the remap above: workgroups are dispatched
+// round-robin over the XCDs, so spreading contiguous ranges of the linear
+// workgroup id over them keeps spatial neighbours (the j and k stencil rows)
+// in one XCD's L2.
+__device__ __forceinline__ uint3 xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned r = xcd_remap(b, n);
+  return make_uint3(r % gx, (r / gx) % gy, r / (gx * gy));
+}
+
 }  // namespace ostc

@@ -309,4 +309,31 @@ __device__ __forceinline__ void lu_solve(const double *a, const int *piv, double
   }
 }
 
+// The plain point-implicit chemistry step at fixed (T, p):
+// (I - dt*J) d(rhoY) = dt*omega, dY = d(rhoY)/rho -- the reference's explicit
+// EPI (src/FiniteRateChem.cu:211-234) applied to the chemical source alone.
+template <int NS>
+__device__ __forceinline__ void plain_chem_step(const ostc_param *__restrict__ p, double tc, double pc,
+                                                const double *yy, double dtc, double *dy) {
+  double mw = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) mw += yy[l] / p->mw[l];
+  mw = 1 / mw;
+  const double rho = pc * mw / (kRu * tc);
+  double om[NS], a[NS * NS];
+  int piv[NS];
+  chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, tc, yy, om, a, nullptr);
+#pragma unroll
+  for (int q = 0; q < NS * NS; ++q) a[q] = -dtc * a[q];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    a[q * NS + q] += 1.0;
+    om[q] *= dtc;
+  }
+  lu_factor<NS>(a, piv, NS);
+  lu_solve<NS>(a, piv, om, NS);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) dy[l] = om[l] / rho;
+}
+
 }  // namespace ostc

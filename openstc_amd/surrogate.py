@@ -1,0 +1,374 @@
+"""Learned chemistry surrogate: data, training and inference through libostc.
+
+No counterpart in the reference.  The surrogate stands in for the plain
+point-implicit chemistry step (``ostc_chem_plain_step``: (I - dt J) d(rhoY) =
+dt omega at fixed T, p) of the 9-species H2-air mechanism.  States are
+sampled from the solver's own trajectories of the finite-rate mixing-layer
+workload, labelled by that step on the GPU, and the network (12-128x4-n_out,
+GELU) is trained on the GPU by the library's forward/backward kernels with
+Adam on FP32 master weights.  Inference runs on BF16 MFMA with per-cell
+fallback to the plain step.
+
+Output encoding (kind 0): o_0 = log10(max(max_k |dY_k|, floor)),
+o_{k+1} = dY_k / 10**o_0 -- one magnitude and a direction bounded by 1, so the
+network's error is relative to the size of the change in each cell.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib as L
+
+KIND_LOGMAG, KIND_ASINH = 0, 1
+N_SPEC = 9
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2air9.npz")
+ATM = 101325.0
+
+
+class DeviceArray:
+    """A device copy of a host array (float64/float32/int32)."""
+
+    def __init__(self, a: np.ndarray | None = None, nbytes: int | None = None):
+        self.lib = L.load()
+        self.p = C.c_void_p()
+        if a is not None:
+            self.a = np.ascontiguousarray(a)
+            nbytes = self.a.nbytes
+        else:
+            self.a = None
+        self.nbytes = max(16, int(nbytes))
+        L.check(self.lib.ostc_device_alloc(C.byref(self.p), self.nbytes), "device alloc")
+        if a is not None and self.a.nbytes:
+            L.check(self.lib.ostc_memcpy_h2d(self.p, self.a.ctypes.data_as(C.c_void_p), self.a.nbytes), "h2d")
+
+    def put(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        L.check(self.lib.ostc_memcpy_h2d(self.p, a.ctypes.data_as(C.c_void_p), a.nbytes), "h2d")
+
+    def get(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        L.check(self.lib.ostc_device_sync(), "sync")
+        L.check(self.lib.ostc_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.p, out.nbytes), "d2h")
+        return out
+
+    def offset(self, nbytes: int) -> C.c_void_p:
+        return C.c_void_p(self.p.value + nbytes)
+
+    def free(self):
+        if self.p:
+            self.lib.ostc_device_free(self.p)
+            self.p = C.c_void_p()
+
+
+# ---- features and targets (host copies, for statistics) -------------------
+def raw_features(t, p, y, dt) -> np.ndarray:
+    """(n, 12) FP32 features before standardisation, as the kernel forms them."""
+    t32 = np.asarray(t, np.float32)
+    f = np.empty((len(t32), 12), np.float32)
+    f[:, 0] = np.float32(1000.0) / t32
+    f[:, 1] = np.log(np.asarray(p, np.float32) * np.float32(1.0 / ATM))
+    yy = np.maximum(np.asarray(y, np.float32), np.float32(0))
+    f[:, 2:11] = (np.power(yy, np.float32(0.1)) - np.float32(1)) * np.float32(10)
+    f[:, 11] = np.log(np.asarray(dt, np.float32))
+    return f
+
+
+def encode(dy: np.ndarray, kind: int, floor: float) -> np.ndarray:
+    dy = np.asarray(dy, np.float64)
+    if kind == KIND_LOGMAG:
+        mag = np.maximum(np.abs(dy).max(1), floor)
+        lg = np.log10(mag)
+        return np.column_stack([lg, dy / (10.0 ** lg)[:, None]])
+    return np.arcsinh(dy / floor)
+
+
+def decode(o: np.ndarray, kind: int, floor: float) -> np.ndarray:
+    """Inverse of ``encode`` on de-standardised outputs."""
+    if kind == KIND_LOGMAG:
+        return o[:, 1:10] * (10.0 ** o[:, :1])
+    return floor * np.sinh(o[:, :9])
+
+
+def element_weights(param: L.Param, info: L.MechanismInfo) -> np.ndarray:
+    """(n_elem, 9): kmol of element j per kg of species k (a_jk / W_k, W in kg/kmol)."""
+    ne = info.n_elem
+    w = np.zeros((ne, N_SPEC))
+    for k in range(N_SPEC):
+        for j in range(ne):
+            w[j, k] = info.elem_comp[k * L.MAX_ELEM + j] / param.mw[k]
+    return w
+
+
+def make_desc(t, p, y, dt, dy, param: L.Param, info: L.MechanismInfo, kind=KIND_LOGMAG, floor=1e-12,
+              y_tol=1e-5, elem_tol=1e-2, elem_abs=1e-9, mag_weight=4.0) -> L.SurDesc:
+    """Standardisation, fallback box and balance weights from a training set."""
+    d = L.SurDesc()
+    d.kind = kind
+    d.n_out = 10 if kind == KIND_LOGMAG else 9
+    f = raw_features(t, p, y, dt).astype(np.float64)
+    for q in range(12):
+        d.in_mean[q] = float(f[:, q].mean())
+        d.in_std[q] = float(max(f[:, q].std(), 1e-6))
+    o = encode(dy, kind, floor)
+    for q in range(16):
+        d.out_mean[q], d.out_std[q], d.loss_w[q] = 0.0, 1.0, 0.0
+    for q in range(d.n_out):
+        d.loss_w[q] = 1.0
+    if kind == KIND_LOGMAG:
+        d.out_mean[0] = float(o[:, 0].mean())
+        d.out_std[0] = float(max(o[:, 0].std(), 1e-6))
+        d.loss_w[0] = mag_weight
+    else:
+        for q in range(9):
+            d.out_std[q] = float(max(o[:, q].std(), 1e-6))
+    ew = element_weights(param, info)
+    d.n_elem = ew.shape[0]
+    for j in range(ew.shape[0]):
+        for k in range(N_SPEC):
+            d.elem_w[j * 9 + k] = float(ew[j, k])
+    d.floor = floor
+    d.t_min, d.t_max = float(np.min(t)), float(np.max(t))
+    d.p_min, d.p_max = float(np.min(p)), float(np.max(p))
+    d.dt_min, d.dt_max = float(np.min(dt)), float(np.max(dt))
+    d.y_tol, d.elem_tol, d.elem_abs = y_tol, elem_tol, elem_abs
+    return d
+
+
+def standardised_targets(dy, desc: L.SurDesc) -> np.ndarray:
+    o = encode(dy, desc.kind, desc.floor)
+    out = np.zeros((len(o), 16), np.float32)
+    for q in range(desc.n_out):
+        out[:, q] = (o[:, q] - desc.out_mean[q]) / desc.out_std[q]
+    return out
+
+
+def init_weights(n_out: int, seed: int = 0) -> np.ndarray:
+    """Uniform(+-1/sqrt(fan_in)) for weights and biases, per layer."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l in range(5):
+        fin = 12 if l == 0 else 128
+        fout = n_out if l == 4 else 128
+        b = 1.0 / math.sqrt(fin)
+        parts.append(rng.uniform(-b, b, fout * fin))
+        parts.append(rng.uniform(-b, b, fout))
+    w = np.concatenate(parts).astype(np.float32)
+    assert w.size == L.load().ostc_surrogate_n_params(n_out)
+    return w
+
+
+def desc_to_dict(d: L.SurDesc) -> dict:
+    out = {}
+    for name, typ in d._fields_:
+        v = getattr(d, name)
+        out[name] = np.array(list(v)) if hasattr(v, "__len__") else np.array(v)
+    return out
+
+
+def desc_from_dict(m) -> L.SurDesc:
+    d = L.SurDesc()
+    for name, typ in d._fields_:
+        v = m[name]
+        if v.ndim:
+            arr = getattr(d, name)
+            for i, x in enumerate(v.tolist()):
+                arr[i] = x
+        else:
+            setattr(d, name, v.item())
+    return d
+
+
+class Surrogate:
+    def __init__(self, ctx, desc: L.SurDesc, weights: np.ndarray | None = None, max_batch: int = 0, seed: int = 0):
+        self.ctx, self.lib, self.desc = ctx, ctx.lib, desc
+        self.n_params = self.lib.ostc_surrogate_n_params(desc.n_out)
+        w = init_weights(desc.n_out, seed) if weights is None else np.ascontiguousarray(weights, np.float32)
+        assert w.size == self.n_params
+        h = C.c_void_p()
+        L.check(self.lib.ostc_surrogate_create(ctx.h, C.byref(desc), w.ctypes.data_as(C.POINTER(C.c_float)),
+                                               max_batch, C.byref(h)), "ostc_surrogate_create")
+        self.h = h
+        self.max_batch = max_batch
+
+    @classmethod
+    def load(cls, ctx, path: str = DEFAULT_WEIGHTS, max_batch: int = 0, **overrides):
+        m = np.load(path, allow_pickle=False)
+        desc = desc_from_dict({k[5:]: m[k] for k in m.files if k.startswith("desc_")})
+        for k, v in overrides.items():
+            setattr(desc, k, v)
+        return cls(ctx, desc, m["weights"], max_batch)
+
+    def weights(self) -> np.ndarray:
+        w = np.empty(self.n_params, np.float32)
+        L.check(self.lib.ostc_surrogate_get_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float))), "get_weights")
+        return w
+
+    def set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        L.check(self.lib.ostc_surrogate_set_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float))), "set_weights")
+
+    def save(self, path: str, **extra):
+        arrays = {"weights": self.weights()}
+        arrays.update({"desc_" + k: v for k, v in desc_to_dict(self.desc).items()})
+        arrays.update(extra)
+        np.savez(path, **arrays)
+
+    def step(self, n, t, p, y, dt, dy, stream=None):
+        st = (C.c_longlong * 4)()
+        L.check(self.lib.ostc_surrogate_step(self.h, n, t, p, y, dt, dy, st, stream), "surrogate_step")
+        return list(st)
+
+    def forward(self, n, t, p, y, dt, raw, dy):
+        L.check(self.lib.ostc_surrogate_forward(self.h, n, t, p, y, dt, raw, dy, None), "surrogate_forward")
+
+    def gradients(self, nb, t, p, y, dt, idx, target, grad) -> float:
+        loss = C.c_double()
+        L.check(self.lib.ostc_surrogate_gradients(self.h, nb, t, p, y, dt, idx, target, grad, C.byref(loss), None),
+                "surrogate_gradients")
+        return loss.value
+
+    def train_step(self, nb, t, p, y, dt, idx, target, lr, want_loss=False):
+        loss = C.c_double()
+        L.check(self.lib.ostc_surrogate_train_step(self.h, nb, t, p, y, dt, idx, target, lr,
+                                                   C.byref(loss) if want_loss else None, None), "train_step")
+        return loss.value if want_loss else None
+
+    def close(self):
+        if self.h:
+            self.lib.ostc_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- data ------------------------------------------------------------------
+def interior_states(sim, b: int = 0):
+    """(T, p, Y[n,9], dt) of the interior cells of block b, i fastest."""
+    z = sim.zones[b]
+    g = z.ngg
+    sl = (slice(g, -g),) * 3
+    bv = z.download("BV", 6)
+    sv = z.download("SV", N_SPEC)
+    dt = z.download("DT", 1)[0]
+    t = bv[5][sl].ravel()
+    p = bv[4][sl].ravel()
+    y = np.stack([sv[k][sl].ravel() for k in range(N_SPEC)], 1)
+    return t, p, y, dt[sl].ravel()
+
+
+def harvest(case, n_steps: int, every: int, per_snapshot: int, seed: int = 0, log=print):
+    """Run the solver on the GPU and sample interior states every `every`
+    iterations (the dt is the local time step the iteration used)."""
+    from .runtime import Simulation
+    rng = np.random.default_rng(seed)
+    sim = Simulation(case)
+    sim.initialize_computation()
+    out = [[], [], [], [], []]
+    t0 = time.time()
+    for s in range(1, n_steps + 1):
+        sim.step(output_screen=1 << 30)
+        if s % every == 0:
+            t, p, y, dt = interior_states(sim)
+            if not (np.isfinite(t).all() and np.isfinite(y).all()):
+                log(f"[harvest] non-finite state at iteration {s}; stopping")
+                break
+            pick = rng.choice(len(t), size=min(per_snapshot, len(t)), replace=False)
+            for lst, a in zip(out, (t, p, y, dt, np.full(len(t), s))):
+                lst.append(a[pick])
+            log(f"[harvest] iteration {s}: T {t.min():.0f}..{t.max():.0f} K, dt {dt.min():.2e}..{dt.max():.2e} s, "
+                f"{time.time() - t0:.1f}s")
+    sim.close()
+    t, p, y, dt, it = (np.concatenate(v) for v in out)
+    y = np.maximum(y, 0.0)
+    y /= y.sum(1, keepdims=True)
+    return t, p, y, dt, it
+
+
+def plain_step(ctx, t, p, y, dt) -> np.ndarray:
+    """dY of the plain point-implicit step on the GPU (FP64)."""
+    n = len(t)
+    arrs = [DeviceArray(np.ascontiguousarray(a, np.float64)) for a in (t, p, y, dt)]
+    out = DeviceArray(nbytes=n * N_SPEC * 8)
+    L.check(ctx.lib.ostc_chem_plain_step(ctx.h, n, *(a.p for a in arrs), out.p, None), "plain step")
+    dy = out.get(np.float64, (n, N_SPEC))
+    for a in arrs + [out]:
+        a.free()
+    return dy
+
+
+def train(sur: Surrogate, t, p, y, dt, target, epochs: int, batch: int, lr_max: float, seed: int = 0,
+          val=None, log=print, warmup_frac=0.03, lr_min_frac=0.01):
+    """Adam with linear warm-up and cosine decay; data resident on the GPU.
+    val = (n, t, p, y, dt, dy) device arrays + host dy for a per-epoch check."""
+    n = len(t)
+    assert batch % 16 == 0 and batch <= sur.max_batch
+    dev = [DeviceArray(np.ascontiguousarray(a, np.float64)) for a in (t, p, y, dt)]
+    dtar = DeviceArray(np.ascontiguousarray(target, np.float32))
+    nb = n // batch
+    perm_dev = DeviceArray(nbytes=nb * batch * 4)
+    rng = np.random.default_rng(seed)
+    total = epochs * nb
+    step = 0
+    hist = []
+    t0 = time.time()
+    for ep in range(epochs):
+        perm = rng.permutation(n)[: nb * batch].astype(np.int32)
+        perm_dev.put(perm)
+        loss = None
+        for b in range(nb):
+            frac = step / max(1, total - 1)
+            if frac < warmup_frac:
+                lr = lr_max * (0.1 + 0.9 * frac / warmup_frac)
+            else:
+                q = (frac - warmup_frac) / (1 - warmup_frac)
+                lr = lr_max * (lr_min_frac + (1 - lr_min_frac) * 0.5 * (1 + math.cos(math.pi * q)))
+            want = b == nb - 1
+            l_ = sur.train_step(batch, *(a.p for a in dev), perm_dev.offset(4 * b * batch), dtar.p, lr, want)
+            if want:
+                loss = l_
+            step += 1
+        rec = {"epoch": ep, "loss": loss, "lr": lr, "seconds": time.time() - t0}
+        if val is not None and (ep % 5 == 4 or ep == epochs - 1):
+            rec.update(evaluate(sur, *val))
+        hist.append(rec)
+        log("[train] " + " ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}" for k, v in rec.items()))
+    for a in dev + [dtar, perm_dev]:
+        a.free()
+    return hist
+
+
+def evaluate(sur: Surrogate, n, t, p, y, dt, dy_ref: np.ndarray, out=None) -> dict:
+    """Surrogate-with-fallback vs the plain step on a resident set."""
+    dyd = out if out is not None else DeviceArray(nbytes=n * N_SPEC * 8)
+    st = sur.step(n, t.p, p.p, y.p, dt.p, dyd.p)
+    dy = dyd.get(np.float64, (n, N_SPEC))
+    if out is None:
+        dyd.free()
+    return error_report(dy, dy_ref, st)
+
+
+def error_report(dy: np.ndarray, ref: np.ndarray, stats) -> dict:
+    """Relative error of dY (= of the effective production rate dY/dt):
+    rel_l2 = ||dY - dY_ref|| / ||dY_ref|| over all cells and species; per-cell
+    errors max_k|dY_k - ref_k| / max_k|ref_k| over cells with max|ref| > 1e-8."""
+    rel = float(np.linalg.norm(dy - ref) / max(np.linalg.norm(ref), 1e-300))
+    mag = np.abs(ref).max(1)
+    big = mag > 1e-8
+    cell = np.abs(dy - ref).max(1)[big] / mag[big]
+    n = max(1, stats[0])
+    return {"rel_l2": rel,
+            "cell_p50": float(np.median(cell)) if cell.size else 0.0,
+            "cell_p99": float(np.percentile(cell, 99)) if cell.size else 0.0,
+            "fallback": float((stats[1] + stats[2] + stats[3]) / n),
+            "fb_range": float(stats[1] / n), "fb_output": float(stats[2] / n), "fb_element": float(stats[3] / n)}

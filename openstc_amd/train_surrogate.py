@@ -1,0 +1,85 @@
+"""Train the chemistry surrogate on the GPU and write its weights file.
+
+    python -m openstc_amd.train_surrogate --out openstc_amd/data/surrogate_h2air9.npz
+
+1. Run the finite-rate mixing-layer workload (192x96x96, FP32 bulk, DPLUR)
+   on the GPU and sample interior states (T, p, Y, local dt) every `every`
+   iterations (training seed) and, separately, from a run with another
+   initial perturbation (held-out seed).
+2. Label every state with the plain point-implicit step on the GPU (FP64).
+3. Train the MLP with the library's kernels (Adam, warm-up + cosine).
+4. Save weights, descriptor, training history and a held-out sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import cases, surrogate as S
+from .runtime import Context
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--steps", type=int, default=400, help="solver iterations per harvest run")
+    ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--per-snapshot", type=int, default=100000)
+    ap.add_argument("--val-per-snapshot", type=int, default=10000)
+    ap.add_argument("--cfl", type=float, default=0.2)
+    ap.add_argument("--grid", type=int, nargs=3, default=[192, 96, 96])
+    ap.add_argument("--epochs", type=int, default=100)
+    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--keep-val", type=int, default=8192, help="held-out states stored with the weights")
+    a = ap.parse_args(argv)
+    mx, my, mz = a.grid
+    t0 = time.time()
+    log(f"[train_surrogate] harvest {mx}x{my}x{mz}, {a.steps} iterations, CFL {a.cfl}")
+    tr = S.harvest(cases.config3(mx=mx, my=my, mz=mz, cfl=a.cfl, seed=3), a.steps, a.every, a.per_snapshot,
+                   seed=a.seed, log=log)
+    va = S.harvest(cases.config3(mx=mx, my=my, mz=mz, cfl=a.cfl, seed=4), a.steps, a.every, a.val_per_snapshot,
+                   seed=a.seed + 1, log=log)
+    mech, info = cases.load_mechanism(True)
+    param = cases.mixture_param(mech, False, True, 4, a.cfl, 1,
+                                cases.InflowState(1.0, 1.0, 0, 0, 1e5, 1000.0, 1.0, 0.0, [0.0] * 9))
+    ctx = Context(param)
+    t, p, y, dt, it = tr
+    dy = S.plain_step(ctx, t, p, y, dt)
+    vt, vp, vy, vdt, vit = va
+    vdy = S.plain_step(ctx, vt, vp, vy, vdt)
+    log(f"[train_surrogate] {len(t)} training / {len(vt)} held-out states labelled ({time.time() - t0:.1f}s)")
+    desc = S.make_desc(t, p, y, dt, dy, param, info)
+    target = S.standardised_targets(dy, desc)
+    sur = S.Surrogate(ctx, desc, max_batch=a.batch, seed=a.seed)
+    vdev = [S.DeviceArray(np.ascontiguousarray(v)) for v in (vt, vp, vy, vdt)]
+    val = (len(vt), *vdev, vdy)
+    hist = S.train(sur, t, p, y, dt, target, a.epochs, a.batch, a.lr, seed=a.seed, val=val, log=log)
+    final = S.evaluate(sur, *val)
+    log("[train_surrogate] held-out:", json.dumps(final))
+    keep = np.random.default_rng(a.seed).choice(len(vt), size=min(a.keep_val, len(vt)), replace=False)
+    sur.save(a.out, history=np.array(json.dumps(hist)), heldout_t=vt[keep], heldout_p=vp[keep],
+             heldout_y=vy[keep], heldout_dt=vdt[keep], heldout_dy=vdy[keep],
+             info=np.array(json.dumps({"grid": a.grid, "cfl": a.cfl, "steps": a.steps, "every": a.every,
+                                       "n_train": int(len(t)), "n_heldout": int(len(vt)), "epochs": a.epochs,
+                                       "batch": a.batch, "lr": a.lr, "heldout": final})))
+    log(f"[train_surrogate] wrote {a.out} ({time.time() - t0:.1f}s)")
+    for d in vdev:
+        d.free()
+    sur.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -89,6 +89,16 @@ class OrcZone(C.Structure):
                 + [(n, _DP) for n in ZONE_ARRAYS])
 
 
+class OrcSurDesc(C.Structure):
+    _fields_ = [("n_out", C.c_int), ("kind", C.c_int), ("n_elem", C.c_int), ("reserved", C.c_int),
+                ("in_mean", C.c_float * 12), ("in_std", C.c_float * 12),
+                ("out_mean", C.c_float * 16), ("out_std", C.c_float * 16),
+                ("loss_w", C.c_float * 16), ("elem_w", C.c_float * 36),
+                ("floor", C.c_float), ("t_min", C.c_float), ("t_max", C.c_float), ("p_min", C.c_float),
+                ("p_max", C.c_float), ("dt_min", C.c_float), ("dt_max", C.c_float),
+                ("y_tol", C.c_float), ("elem_tol", C.c_float), ("elem_abs", C.c_float)]
+
+
 def build(quiet=True):
     """Compile the oracle with its Makefile (gcc)."""
     r = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
@@ -143,6 +153,14 @@ def load():
             "orc_plain_chem_batch": (None, [Pp, C.c_long, _DP, _DP, _DP, _DP, _DP]),
             "orc_newton_temperature": (I, [Pp, _DP, D, D, _DP]),
             "orc_omp_threads": (I, []),
+            "orc_sur_n_params": (I, [I]),
+            "orc_bf16": (D, [D]),
+            "orc_sur_features": (None, [C.POINTER(OrcSurDesc), D, D, _DP, D, C.POINTER(C.c_float)]),
+            "orc_sur_forward": (None, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), I, C.c_long, _DP, _DP, _DP,
+                                       _DP, _DP]),
+            "orc_sur_loss_grad": (D, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), I, C.c_long, _DP, _DP, _DP,
+                                      _DP, C.POINTER(C.c_float), _DP]),
+            "orc_sur_decode_check": (I, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), D, D, _DP, D, _DP]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)

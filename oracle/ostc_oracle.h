@@ -171,6 +171,40 @@ int orc_newton_temperature(const orc_param *p, const double *y, double e, double
 
 int orc_omp_threads(void);
 
+/* ---- learned chemistry surrogate (no reference counterpart) ----
+ * FP64 restatement of the product's MLP 12-128-128-128-128-n_out (GELU, tanh
+ * form): features, forward, weighted-MSE gradient, output decoding and the
+ * fallback checks.  With quant = 1 the BF16 rounding points of the product's
+ * inference (features, weights, hidden activations; round to nearest even)
+ * are emulated and the backward treats them as identity (straight-through);
+ * with quant = 0 everything is plain FP64 (for finite-difference checks).
+ * The descriptor has the layout of ostc_surrogate_desc. */
+typedef struct orc_sur_desc {
+  int n_out, kind, n_elem, reserved;
+  float in_mean[12], in_std[12];
+  float out_mean[16], out_std[16];
+  float loss_w[16];
+  float elem_w[4 * 9];
+  float floor;
+  float t_min, t_max, p_min, p_max, dt_min, dt_max;
+  float y_tol, elem_tol, elem_abs;
+} orc_sur_desc;
+
+int orc_sur_n_params(int n_out);
+double orc_bf16(double x);
+/* standardised FP32 features of one state */
+void orc_sur_features(const orc_sur_desc *d, double t, double p, const double *y, double dt, float *f);
+/* standardised outputs out[n*16] (columns >= n_out are 0) */
+void orc_sur_forward(const orc_sur_desc *d, const float *w, int quant, long n, const double *t, const double *p,
+                     const double *y, const double *dt, double *out);
+/* L = sum_b sum_o loss_w[o] (out_bo - target[b*16+o])^2 / n; grad[n_params] */
+double orc_sur_loss_grad(const orc_sur_desc *d, const float *w, int quant, long n, const double *t, const double *p,
+                         const double *y, const double *dt, const float *target, double *grad);
+/* decode standardised outputs o[16] into dy[9] and return the fallback
+ * reason (0 accepted, 1 input range, 2 output range, 3 element balance) */
+int orc_sur_decode_check(const orc_sur_desc *d, const float *o, double t, double p, const double *y, double dt,
+                         double *dy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,182 @@
+"""GPU parity of the learned chemistry surrogate against the FP64 oracle
+(oracle/orc_sur.c, BF16 rounding points emulated with quant = 1).
+
+Tolerances: raw standardised outputs 2e-3 absolute (FP32 vs FP64
+accumulation can flip a BF16 rounding of a hidden activation: one BF16 ulp
+of one unit), gradients 1e-3 relative per layer; fallback cells equal the
+plain step bit for bit; accepted cells equal the oracle's decoding of the
+GPU's own raw outputs to 1e-5 relative."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from openstc_amd import _lib as L
+from openstc_amd import cases, surrogate as S
+from openstc_amd.runtime import Context
+
+pytestmark = pytest.mark.gpu
+FP = C.POINTER(C.c_float)
+
+
+def say(*a):
+    print("[gpu-sur]", *a, flush=True)
+
+
+def fptr(a):
+    return a.ctypes.data_as(FP)
+
+
+@pytest.fixture(scope="module")
+def env():
+    mech, info = cases.load_mechanism(True)
+    param = cases.mixture_param(mech, False, True, 4, 1.0, 1,
+                                cases.InflowState(1.0, 1.0, 0, 0, 1e5, 1000.0, 1.0, 0.0, [0.0] * 9))
+    ctx = Context(param)
+    yield ctx, param, info
+    ctx.close()
+
+
+def states(n, seed):
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(950, 2450, n)
+    p = rng.uniform(0.9, 1.5, n) * 101325.0
+    y = rng.dirichlet(np.ones(9) * 0.5, n)
+    dt = 10 ** rng.uniform(-7.7, -7.2, n)
+    return t, p, y, dt
+
+
+def make(env, n=4096, seed=1, max_batch=1024, wseed=3):
+    ctx, param, info = env
+    t, p, y, dt = states(n, seed)
+    dy = S.plain_step(ctx, t, p, y, dt)
+    desc = S.make_desc(t, p, y, dt, dy, param, info)
+    sur = S.Surrogate(ctx, desc, S.init_weights(desc.n_out, wseed), max_batch=max_batch)
+    od = O.OrcSurDesc()
+    C.memmove(C.addressof(od), C.addressof(desc), C.sizeof(od))
+    return sur, od, (t, p, y, dt, dy)
+
+
+def test_forward_matches_oracle(env):
+    sur, od, (t, p, y, dt, dy) = make(env)
+    n = 2000
+    say("forward n=%d" % n)
+    d = [S.DeviceArray(np.ascontiguousarray(a[:n])) for a in (t, p, y, dt)]
+    raw = S.DeviceArray(nbytes=n * 16 * 4)
+    dyd = S.DeviceArray(nbytes=n * 9 * 8)
+    sur.forward(n, *(a.p for a in d), raw.p, dyd.p)
+    g = raw.get(np.float32, (n, 16))
+    ref = np.zeros((n, 16))
+    yy = np.ascontiguousarray(y[:n])
+    O.load().orc_sur_forward(C.byref(od), fptr(S.init_weights(sur.desc.n_out, 3)), 1, n, O.dp(t[:n].copy()),
+                             O.dp(p[:n].copy()), O.dp(yy), O.dp(dt[:n].copy()), O.dp(ref))
+    err = np.abs(g - ref)[:, :sur.desc.n_out]
+    say("raw output max abs diff", err.max(), "median", np.median(err))
+    assert err.max() < 2e-3 and np.median(err) < 1e-5
+    for a in d + [raw, dyd]:
+        a.free()
+    sur.close()
+
+
+def test_gradients_match_oracle(env):
+    sur, od, (t, p, y, dt, dy) = make(env, max_batch=512)
+    nb = 256
+    say("gradients nb=%d" % nb)
+    tgt = S.standardised_targets(dy, sur.desc)
+    d = [S.DeviceArray(np.ascontiguousarray(a)) for a in (t, p, y, dt)]
+    dtg = S.DeviceArray(tgt)
+    idx = np.random.default_rng(4).choice(len(t), nb, replace=False).astype(np.int32)
+    didx = S.DeviceArray(idx)
+    grad = S.DeviceArray(nbytes=sur.n_params * 4)
+    loss = sur.gradients(nb, *(a.p for a in d), didx.p, dtg.p, grad.p)
+    gg = grad.get(np.float32, (sur.n_params,)).astype(np.float64)
+    go = np.zeros(sur.n_params)
+    yy = np.ascontiguousarray(y[idx])
+    lo = O.load().orc_sur_loss_grad(C.byref(od), fptr(S.init_weights(sur.desc.n_out, 3)), 1, nb,
+                                    O.dp(t[idx].copy()), O.dp(p[idx].copy()), O.dp(yy), O.dp(dt[idx].copy()),
+                                    fptr(np.ascontiguousarray(tgt[idx])), O.dp(go))
+    say("loss gpu %.8g oracle %.8g" % (loss, lo))
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    offs = [0, 1664, 18176, 34688, 51200, sur.n_params]
+    for l in range(5):
+        a, b = offs[l], offs[l + 1]
+        e = np.linalg.norm(gg[a:b] - go[a:b]) / np.linalg.norm(go[a:b])
+        say("layer", l, "grad rel err", e)
+        assert e < 1e-3, (l, e)
+    for a in d + [dtg, didx, grad]:
+        a.free()
+    sur.close()
+
+
+def test_training_reduces_loss(env):
+    sur, od, (t, p, y, dt, dy) = make(env, n=4096, max_batch=1024)
+    tgt = S.standardised_targets(dy, sur.desc)
+    say("training 12 epochs x 4 batches")
+    hist = S.train(sur, t, p, y, dt, tgt, epochs=12, batch=1024, lr_max=2e-3, log=say)
+    assert hist[-1]["loss"] < 0.5 * hist[0]["loss"], hist
+    sur.close()
+
+
+def test_fallback_paths(env):
+    ctx = env[0]
+    sur, od, (t, p, y, dt, dy) = make(env)
+    n = 1024
+    t, p, y, dt = (a[:n].copy() for a in (t, p, y, dt))
+    t[:100] = 5000.0          # input outside the trained box -> reason 1
+    ref = S.plain_step(ctx, t, p, y, dt)
+    d = [S.DeviceArray(np.ascontiguousarray(a)) for a in (t, p, y, dt)]
+    out = S.DeviceArray(nbytes=n * 9 * 8)
+    raw = S.DeviceArray(nbytes=n * 16 * 4)
+    say("fallback: 100 hot cells, untrained weights")
+    st = sur.step(n, *(a.p for a in d), out.p)
+    sur.forward(n, *(a.p for a in d), raw.p, out.p)
+    got = out.get(np.float64, (n, 9))
+    rawh = raw.get(np.float32, (n, 16))
+    say("stats", st)
+    assert st[0] == n and st[1] >= 100
+    lib = O.load()
+    reasons = np.zeros(n, int)
+    for c in range(n):
+        dyo = np.zeros(9)
+        reasons[c] = lib.orc_sur_decode_check(C.byref(od), fptr(np.ascontiguousarray(rawh[c])), t[c], p[c],
+                                              O.dp(np.ascontiguousarray(y[c])), dt[c], O.dp(dyo))
+        if reasons[c] == 0:
+            assert np.allclose(got[c], dyo, rtol=1e-5, atol=1e-30), c
+        else:
+            assert np.array_equal(got[c], ref[c]), c
+    assert np.all(reasons[:100] == 1)
+    counts = [int(np.sum(reasons == r)) for r in (1, 2, 3)]
+    say("oracle reasons", counts)
+    assert sum(abs(a - b) for a, b in zip(counts, st[1:])) <= 2
+    # poisoned weights: a huge output bias on the magnitude -> dY out of range
+    w = S.init_weights(sur.desc.n_out, 3)
+    w[sur.n_params - sur.desc.n_out] = 1e3
+    sur.set_weights(w)
+    st = sur.step(n, *(a.p for a in d), out.p)
+    got = out.get(np.float64, (n, 9))
+    say("poisoned stats", st)
+    assert st[1] + st[2] + st[3] == n and st[2] >= n - 100
+    assert np.array_equal(got, ref)
+    for a in d + [out, raw]:
+        a.free()
+    sur.close()
+
+
+def test_zone_plain_step_equals_batch(env):
+    ctx = env[0]
+    case = cases.config3(mx=24, my=16, mz=8)
+    say("zone plain chemistry on config3 24x16x8 FP64")
+    from openstc_amd.runtime import Simulation
+    sim = Simulation(case, precision=L.FP64)
+    sim.initialize_computation()
+    sim.step()
+    t, p, y, dt = S.interior_states(sim)
+    n = len(t)
+    out = S.DeviceArray(nbytes=n * 9 * 8)
+    L.check(sim.ctx.lib.ostc_chem_zone_step(sim.ctx.h, sim.zones[0].h, None, 0, out.p, None, None), "zone step")
+    got = out.get(np.float64, (n, 9))
+    ref = S.plain_step(sim.ctx, t, p, y, dt)
+    assert np.array_equal(got, ref)
+    out.free()
+    sim.close()

@@ -1,0 +1,184 @@
+"""Pins of the surrogate oracle (oracle/orc_sur.c) on the CPU.
+
+* BF16 rounding: against torch's float32 -> bfloat16 conversion (round to
+  nearest even) on random and tie values.
+* Features: against an independent numpy FP32 evaluation.
+* Forward (quant = 0): against a torch float64 MLP with the same weights
+  (nn.functional.linear + GELU tanh form).
+* Gradient (quant = 0): against central finite differences of the oracle's
+  own loss.
+* Decode/checks: closed cases (balanced change accepted, out-of-box input,
+  Y+dY out of range, unbalanced change)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_ffi as O
+from openstc_amd import cases, surrogate as S
+
+FP = C.POINTER(C.c_float)
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(FP)
+
+
+@pytest.fixture(scope="module")
+def mech():
+    return cases.load_mechanism(True)
+
+
+def states(n, seed):
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(900, 2500, n)
+    p = rng.uniform(0.8, 1.6, n) * 101325.0
+    y = rng.dirichlet(np.ones(9) * 0.5, n)
+    dt = 10 ** rng.uniform(-8, -7, n)
+    return t, p, y, dt
+
+
+def make_desc(mech, n=2048):
+    param, info = mech
+    t, p, y, dt = states(n, 1)
+    dy = (np.random.default_rng(2).normal(size=(n, 9)) * 1e-5)
+    d = S.make_desc(t, p, y, dt, dy, param, info)
+    od = O.OrcSurDesc()
+    C.memmove(C.addressof(od), C.addressof(d), C.sizeof(od))
+    return d, od
+
+
+def test_bf16_rounding_matches_torch():
+    lib = O.load()
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.normal(size=4000) * 10 ** rng.uniform(-6, 6, 4000),
+                        # exact ties between two BF16 neighbours (round to even)
+                        np.array([1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -(1.0 + 2 ** -8), 3.0 + 2 ** -7])])
+    ref = torch.tensor(x.astype(np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+    got = np.array([lib.orc_bf16(float(v)) for v in x.astype(np.float32)])
+    assert np.array_equal(got, ref)
+
+
+def test_features_match_numpy(mech):
+    lib = O.load()
+    d, od = make_desc(mech)
+    t, p, y, dt = states(500, 3)
+    raw = S.raw_features(t, p, y, dt)
+    mean = np.array(d.in_mean, np.float32)
+    rstd = np.array([np.float32(1.0 / np.float64(np.float32(s))) for s in d.in_std], np.float32)
+    ref = (raw - mean) * rstd
+    got = np.empty((500, 12), np.float32)
+    for c in range(500):
+        yy = np.ascontiguousarray(y[c])
+        lib.orc_sur_features(C.byref(od), t[c], p[c], O.dp(yy), dt[c], fptr(got[c]))
+    # numpy and libm FP32 log/pow may differ in the last place of the raw
+    # feature; standardisation scales that place by 1/std
+    eps = np.finfo(np.float32).eps
+    tol = 4 * eps * (np.abs(raw) + np.abs(mean)) * rstd
+    assert np.all(np.abs(got - ref) <= tol)
+    assert np.mean(got == ref) > 0.5
+
+
+def torch_forward(w, feat, n_out):
+    wt = torch.tensor(w.astype(np.float64))
+    h = torch.tensor(feat.astype(np.float64))
+    off = 0
+    for l in range(5):
+        fin = 12 if l == 0 else 128
+        fout = n_out if l == 4 else 128
+        W = wt[off:off + fout * fin].reshape(fout, fin)
+        b = wt[off + fout * fin: off + fout * fin + fout]
+        off += fout * fin + fout
+        h = torch.nn.functional.linear(h, W, b)
+        if l < 4:
+            h = torch.nn.functional.gelu(h, approximate="tanh")
+    return h.numpy()
+
+
+def test_forward_matches_torch_fp64(mech):
+    lib = O.load()
+    d, od = make_desc(mech)
+    w = S.init_weights(d.n_out, seed=5)
+    n = 300
+    t, p, y, dt = states(n, 4)
+    y = np.ascontiguousarray(y)
+    out = np.zeros((n, 16))
+    lib.orc_sur_forward(C.byref(od), fptr(w), 0, n, O.dp(t), O.dp(p), O.dp(y), O.dp(dt), O.dp(out))
+    feat = np.empty((n, 12), np.float32)
+    for c in range(n):
+        lib.orc_sur_features(C.byref(od), t[c], p[c], O.dp(np.ascontiguousarray(y[c])), dt[c], fptr(feat[c]))
+    ref = torch_forward(w, feat, d.n_out)
+    assert np.abs(out[:, :d.n_out] - ref).max() < 1e-12
+    assert np.all(out[:, d.n_out:] == 0)
+
+
+def test_gradient_matches_finite_differences(mech):
+    lib = O.load()
+    d, od = make_desc(mech)
+    w = S.init_weights(d.n_out, seed=7)
+    n = 24
+    t, p, y, dt = states(n, 8)
+    y = np.ascontiguousarray(y)
+    tgt = np.random.default_rng(9).normal(size=(n, 16)).astype(np.float32)
+    g = np.zeros(w.size)
+    args = (n, O.dp(t), O.dp(p), O.dp(y), O.dp(dt), fptr(tgt))
+    lib.orc_sur_loss_grad(C.byref(od), fptr(w), 0, *args, O.dp(g))
+    rng = np.random.default_rng(10)
+    # parameters from every layer: weights and biases
+    offs = [0, 1664, 18176, 34688, 51200]
+    picks = []
+    for l, o in enumerate(offs):
+        size = (1664 if l == 0 else 16512) if l < 4 else 129 * d.n_out
+        picks += list(o + rng.choice(size, 6, replace=False))
+    scratch = np.zeros(w.size)
+    for q in picks:
+        h = 1e-3 * max(1.0, abs(float(w[q])))
+        # the weights are FP32: step by an FP32-representable amount
+        wp32, wm32 = w.copy(), w.copy()
+        wp32[q] = np.float32(w[q] + h)
+        wm32[q] = np.float32(w[q] - h)
+        lp = lib.orc_sur_loss_grad(C.byref(od), fptr(wp32), 0, *args, O.dp(scratch))
+        lm = lib.orc_sur_loss_grad(C.byref(od), fptr(wm32), 0, *args, O.dp(scratch))
+        fd = (lp - lm) / (float(wp32[q]) - float(wm32[q]))
+        assert abs(fd - g[q]) <= 1e-5 * max(abs(g[q]), 1e-4), (q, fd, g[q])
+
+
+def test_decode_and_checks(mech):
+    lib = O.load()
+    d, od = make_desc(mech)
+    names = cases.species_names(mech[1])
+    ih2, io2, ih2o = names.index("H2"), names.index("O2"), names.index("H2O")
+    mw = np.array(mech[0].mw[:9])
+    # 2 H2 + O2 -> 2 H2O: balanced in H, O and N
+    dy = np.zeros(9)
+    rate = 1e-5
+    dy[ih2] = -2 * rate * mw[ih2]
+    dy[io2] = -rate * mw[io2]
+    dy[ih2o] = 2 * rate * mw[ih2o]
+    mag = np.abs(dy).max()
+    o_de = np.zeros(16)
+    o_de[0] = np.log10(mag)
+    o_de[1:10] = dy / mag
+    o = np.zeros(16, np.float32)
+    for q in range(d.n_out):
+        o[q] = (o_de[q] - d.out_mean[q]) / d.out_std[q]
+    y = np.full(9, 0.1)
+    y[ih2] = 0.2
+    y /= y.sum()
+    out = np.zeros(9)
+    st = lib.orc_sur_decode_check(C.byref(od), fptr(o), 1200.0, 101325.0, O.dp(y), 3e-8, O.dp(out))
+    assert st == 0
+    assert np.allclose(out, dy, rtol=1e-5, atol=0)
+    # input temperature outside the trained box
+    assert lib.orc_sur_decode_check(C.byref(od), fptr(o), 50.0, 101325.0, O.dp(y), 3e-8, O.dp(out)) == 1
+    # Y + dY below zero: a species that is absent but predicted to be consumed
+    y2 = y.copy()
+    y2[io2] = 0.0
+    y2 /= y2.sum()
+    assert lib.orc_sur_decode_check(C.byref(od), fptr(o), 1200.0, 101325.0, O.dp(y2), 3e-8, O.dp(out)) == 2
+    # unbalanced: only H2 consumed
+    o3 = o.copy()
+    o3[1 + ih2o] = (0 - d.out_mean[1 + ih2o]) / d.out_std[1 + ih2o]
+    assert lib.orc_sur_decode_check(C.byref(od), fptr(o3), 1200.0, 101325.0, O.dp(y), 3e-8, O.dp(out)) == 3
