@@ -1,6 +1,8 @@
 // Cell-parallel stages of the per-iteration path: properties, sources,
 // local time step, point-implicit / DPLUR treatment, update with the
 // T-from-e Newton solve, limit_flow, residual reduction, wall distance.
+#include <algorithm>
+
 #include "ostc_kernels.h"
 #include "ostc_thermo.h"
 
@@ -383,76 +385,13 @@ __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__
   }
 }
 
-// compute_jacobian_times_dq (src/DPLURHPP.cuh:78-125) at neighbour nb,
-// accumulated into acc with weight sgn*0.5
-template <typename R, int NS, int NT>
-__device__ __forceinline__ void jac_times_dq(const ZoneView<R> &z, const ostc_param *__restrict__ p,
-                                             const R *__restrict__ din, int64_t nb, int dir, R pm_sr, R w,
-                                             R *acc) {
-  constexpr int NV = 5 + NS + NT;
-  const int64_t P = z.plane;
-  const R xi_x = z.metric[(dir * 3) * P + nb], xi_y = z.metric[(dir * 3 + 1) * P + nb],
-          xi_z = z.metric[(dir * 3 + 2) * P + nb];
-  const R u = z.bv[P + nb], v = z.bv[2 * P + nb], wv = z.bv[3 * P + nb];
-  const R U = xi_x * u + xi_y * v + xi_z * wv;
-  const R lmd1 = U + pm_sr;
-  const R vel = z.vel[nb];
-  const R e = (R)0.5 * vel * vel;
-  R d[NV];
-#pragma unroll
-  for (int l = 0; l < NV; ++l) d[l] = din[l * P + nb];
-  R gamma = (R)kGammaAir, b3 = 0, b4 = 0, h = 0;
-  if constexpr (NS > 0) {
-    R hk[NS];
-    const R t = z.bv[5 * P + nb];
-    enthalpy<R>(t, hk, p, NS);
-    gamma = z.gamma[nb];
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      b3 += (R)kRu * t / (R)p->mw[l] * d[5 + l];
-      b4 += hk[l] * d[5 + l];
-      h += z.sv[l * P + nb] * hk[l];
-    }
-    b3 *= gamma;
-    b4 *= gamma - (R)1;
-    h += e;
-  } else {
-    h = gamma / (gamma - (R)1) * z.bv[4 * P + nb] / z.bv[nb] + e;
-  }
-  const R b1 = xi_x * d[1] + xi_y * d[2] + xi_z * d[3] - U * d[0];
-  const R b2 = (gamma - (R)1) * (e * d[0] - u * d[1] - v * d[2] - wv * d[3] + d[4]);
-  acc[0] += w * (b1 + lmd1 * d[0]);
-  acc[1] += w * (u * b1 + xi_x * b2 + lmd1 * d[1] + xi_x * (b3 - b4));
-  acc[2] += w * (v * b1 + xi_y * b2 + lmd1 * d[2] + xi_y * (b3 - b4));
-  acc[3] += w * (wv * b1 + xi_z * b2 + lmd1 * d[3] + xi_z * (b3 - b4));
-  acc[4] += w * (h * b1 + U * b2 + lmd1 * d[4] + U * (b3 - b4));
-#pragma unroll
-  for (int l = 0; l < NS + NT; ++l) acc[5 + l] += w * (lmd1 * d[5 + l] + z.sv[l * P + nb] * b1);
-}
-
-// DPLUR_inner_iteration (src/DPLURHPP.cuh:127-249): one Jacobi sweep,
-// din -> dout (dq0 read for the cell itself); EPI_for_dqk
+// Diagonal solve and write of one cell after a DPLUR sweep has gathered its
+// off-diagonal sum acc (src/DPLURHPP.cuh:193-249); EPI_for_dqk
 // (src/FiniteRateChem.cu:255-284) with the LU factors from k_dq0.
 template <typename R, int NS, int NT, bool CHEM>
-__global__ __launch_bounds__(256) void k_dplur(ZoneView<R> z, const ostc_param *__restrict__ p,
-                                               const R *__restrict__ din, R *__restrict__ dout) {
-  constexpr int NV = 5 + NS + NT;
-  const auto c = cell_index(z, 0);
-  if (!c.ok) return;
-  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
-  const int64_t s[3] = {1, (int64_t)z.nx, (int64_t)z.nx * z.ny};
-  R acc[NV];
-#pragma unroll
-  for (int l = 0; l < NV; ++l) acc[l] = 0;
-  const R *sr = z.spectr;
-  if (c.i > 0) jac_times_dq<R, NS, NT>(z, p, din, id - s[0], 0, sr[id - s[0]], (R)0.5, acc);
-  if (c.j > 0) jac_times_dq<R, NS, NT>(z, p, din, id - s[1], 1, sr[P + id - s[1]], (R)0.5, acc);
-  if (c.k > 0) jac_times_dq<R, NS, NT>(z, p, din, id - s[2], 2, sr[2 * P + id - s[2]], (R)0.5, acc);
-  if (c.i != z.mx - 1) jac_times_dq<R, NS, NT>(z, p, din, id + s[0], 0, -sr[id + s[0]], (R)-0.5, acc);
-  if (c.j != z.my - 1) jac_times_dq<R, NS, NT>(z, p, din, id + s[1], 1, -sr[P + id + s[1]], (R)-0.5, acc);
-  if (c.k != z.mz - 1) jac_times_dq<R, NS, NT>(z, p, din, id + s[2], 2, -sr[2 * P + id + s[2]], (R)-0.5, acc);
-  const R dt = z.dt[id];
-  const R diag = (R)1 + dt * (sr[id] + sr[P + id] + sr[2 * P + id]);
+__device__ __forceinline__ void dplur_finish(const ZoneView<R> &z, const ostc_param *__restrict__ p, int64_t id,
+                                             const R *acc, R dt, R diag, R *__restrict__ dout) {
+  const int64_t P = z.plane;
   const int nflow = (CHEM && NS > 0) ? 5 : 5 + NS;
   for (int l = 0; l < nflow; ++l) dout[l * P + id] = z.dq0[l * P + id] + dt * acc[l] / diag;
   if constexpr (CHEM && NS > 0) {
@@ -489,6 +428,209 @@ __global__ __launch_bounds__(256) void k_dplur(ZoneView<R> z, const ostc_param *
       dout[(NS + 5) * P + id] = z.dq0[(NS + 5) * P + id] + dt * acc[5 + NS] / diag;
       dout[(NS + 6) * P + id] = z.dq0[(NS + 6) * P + id] + dt * acc[6 + NS] / diag;
     }
+  }
+}
+
+// Per-cell inputs of the convective-Jacobian products of one DPLUR sweep
+// (the neighbour part of src/DPLURHPP.cuh:127-191), loaded once per cell.
+template <typename R, int NS, int NT>
+struct JdCell {
+  static constexpr int NV = 5 + NS + NT;
+  R d[NV];
+  R sv[NS + NT];
+  R u, v, w, e, h, b2, bb;
+};
+
+template <typename R, int NS, int NT>
+__device__ __forceinline__ void jd_load(const ZoneView<R> &z, const ostc_param *__restrict__ p,
+                                        const R *__restrict__ din, int64_t q, JdCell<R, NS, NT> &c) {
+  constexpr int NV = 5 + NS + NT;
+  const int64_t P = z.plane;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) c.d[l] = din[l * P + q];
+#pragma unroll
+  for (int l = 0; l < NS + NT; ++l) c.sv[l] = z.sv[l * P + q];
+  c.u = z.bv[P + q];
+  c.v = z.bv[2 * P + q];
+  c.w = z.bv[3 * P + q];
+  const R vel = z.vel[q];
+  c.e = (R)0.5 * vel * vel;
+  R gamma = (R)kGammaAir, b3 = 0, b4 = 0;
+  if constexpr (NS > 0) {
+    R hk[NS];
+    const R t = z.bv[5 * P + q];
+    enthalpy<R>(t, hk, p, NS);
+    gamma = z.gamma[q];
+    R h = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      b3 += (R)kRu * t / (R)p->mw[l] * c.d[5 + l];
+      b4 += hk[l] * c.d[5 + l];
+      h += c.sv[l] * hk[l];
+    }
+    b3 *= gamma;
+    b4 *= gamma - (R)1;
+    c.h = h + c.e;
+  } else {
+    c.h = gamma / (gamma - (R)1) * z.bv[4 * P + q] / z.bv[q] + c.e;
+  }
+  c.bb = b3 - b4;
+  c.b2 = (gamma - (R)1) * (c.e * c.d[0] - c.u * c.d[1] - c.v * c.d[2] - c.w * c.d[3] + c.d[4]);
+}
+
+// Contribution of cell q, seen as a neighbour across direction `dir`:
+// w * (A_dir(q) dq(q) + pm_sr dq(q)); out = 0.5 * (JD + sr*d) (sign +1, the
+// minus-side neighbour) or 0.5 * (JD - sr*d) (sign -1, the plus-side one).
+template <typename R, int NS, int NT>
+__device__ __forceinline__ void jd_dir(const ZoneView<R> &z, int64_t q, int dir, const JdCell<R, NS, NT> &c,
+                                       R sr, R *plus, R *minus) {
+  constexpr int NV = 5 + NS + NT;
+  const int64_t P = z.plane;
+  const R xi_x = z.metric[(dir * 3) * P + q], xi_y = z.metric[(dir * 3 + 1) * P + q],
+          xi_z = z.metric[(dir * 3 + 2) * P + q];
+  const R U = xi_x * c.u + xi_y * c.v + xi_z * c.w;
+  const R b1 = xi_x * c.d[1] + xi_y * c.d[2] + xi_z * c.d[3] - U * c.d[0];
+  R jd[NV];
+  jd[0] = b1 + U * c.d[0];
+  jd[1] = c.u * b1 + xi_x * c.b2 + U * c.d[1] + xi_x * c.bb;
+  jd[2] = c.v * b1 + xi_y * c.b2 + U * c.d[2] + xi_y * c.bb;
+  jd[3] = c.w * b1 + xi_z * c.b2 + U * c.d[3] + xi_z * c.bb;
+  jd[4] = c.h * b1 + U * c.b2 + U * c.d[4] + U * c.bb;
+#pragma unroll
+  for (int l = 0; l < NS + NT; ++l) jd[5 + l] = U * c.d[5 + l] + c.sv[l] * b1;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    if (plus) plus[l] = (R)0.5 * (jd[l] + sr * c.d[l]);
+    if (minus) minus[l] = (R)0.5 * (jd[l] - sr * c.d[l]);
+  }
+}
+
+// DPLUR_inner_iteration (src/DPLURHPP.cuh:127-249): one Jacobi sweep din ->
+// dout.  A workgroup owns a 64 x 4 (i, j) tile and marches through a chunk of
+// k planes: each cell's Jacobian products are formed once per direction and
+// handed to its neighbours -- along i by lane shuffles inside the wave (one
+// wave = one i row), along j through LDS, along k in registers from the
+// previous plane.  Tile-edge neighbours outside the tile are evaluated by
+// the edge lanes/rows themselves.  Off-diagonal sums are accumulated in the
+// reference's order (i-, j-, k-, i+, j+, k+).
+template <typename R, int NS, int NT, bool CHEM>
+__global__ __launch_bounds__(256) void k_dplur(ZoneView<R> z, const ostc_param *__restrict__ p,
+                                               const R *__restrict__ din, R *__restrict__ dout, int kc) {
+  constexpr int NV = 5 + NS + NT;
+  __shared__ R sj[2][4][NV][64];  // [plus/minus][row][var][lane]
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const uint3 bk = xcd_block();
+  const int i = (int)bk.x * 64 + tx, j = (int)bk.y * 4 + ty;
+  const int k0 = (int)bk.z * kc, k1 = min(k0 + kc, z.mz);
+  const bool ok = i < z.mx && j < z.my;
+  const int ii = min(i, z.mx - 1), jj = min(j, z.my - 1);  // clamped (loads of idle lanes stay in bounds)
+  const int64_t P = z.plane;
+  const int64_t sk = (int64_t)z.nx * z.ny;
+  const R *sr = z.spectr;
+  JdCell<R, NS, NT> c;
+  R pk[NV], acc[NV], accp[NV];
+  R dtp = 0, diagp = 1;
+  // P_k of the plane below the chunk
+  if (k0 > 0) {
+    const int64_t q = z.idx(ii, jj, k0 - 1);
+    jd_load<R, NS, NT>(z, p, din, q, c);
+    jd_dir<R, NS, NT>(z, q, 2, c, sr[2 * P + q], pk, nullptr);
+  }
+  for (int k = k0; k < k1; ++k) {
+    const int64_t id = z.idx(ii, jj, k);
+    jd_load<R, NS, NT>(z, p, din, id, c);
+    R mk[NV], pnext[NV];
+    jd_dir<R, NS, NT>(z, id, 2, c, sr[2 * P + id], pnext, mk);
+    // finish the cell below: its plus-side k neighbour is this cell
+    if (k > k0) {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) accp[l] -= mk[l];
+      if (ok) dplur_finish<R, NS, NT, CHEM>(z, p, id - sk, accp, dtp, diagp, dout);
+    }
+    // minus-side terms in the reference order: i-, j-, k-
+    R pi[NV], mi[NV];
+    jd_dir<R, NS, NT>(z, id, 0, c, sr[id], pi, mi);
+    R pj[NV], mj[NV];
+    jd_dir<R, NS, NT>(z, id, 1, c, sr[P + id], pj, mj);
+    __syncthreads();  // previous plane's j exchange is consumed
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      sj[0][ty][l][tx] = pj[l];
+      sj[1][ty][l][tx] = mj[l];
+    }
+    // i: neighbours' P_i / M_i from the adjacent lanes
+    R ip[NV], im[NV];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      ip[l] = __shfl_up(pi[l], 1);
+      im[l] = __shfl_down(mi[l], 1);
+    }
+    if (tx == 0 && i > 0 && i - 1 < z.mx) {
+      const int64_t q = z.idx(i - 1, jj, k);
+      JdCell<R, NS, NT> e;
+      jd_load<R, NS, NT>(z, p, din, q, e);
+      jd_dir<R, NS, NT>(z, q, 0, e, sr[q], ip, nullptr);
+    }
+    if (tx == 63 && i + 1 < z.mx) {
+      const int64_t q = z.idx(i + 1, jj, k);
+      JdCell<R, NS, NT> e;
+      jd_load<R, NS, NT>(z, p, din, q, e);
+      jd_dir<R, NS, NT>(z, q, 0, e, sr[q], nullptr, im);
+    }
+    // j: rows at the tile edge evaluate the neighbour row themselves
+    R jp[NV], jm[NV];
+    if (ty == 0 && j > 0) {
+      const int64_t q = z.idx(ii, j - 1, k);
+      JdCell<R, NS, NT> e;
+      jd_load<R, NS, NT>(z, p, din, q, e);
+      jd_dir<R, NS, NT>(z, q, 1, e, sr[P + q], jp, nullptr);
+    }
+    if (ty == 3 && j + 1 < z.my) {
+      const int64_t q = z.idx(ii, j + 1, k);
+      JdCell<R, NS, NT> e;
+      jd_load<R, NS, NT>(z, p, din, q, e);
+      jd_dir<R, NS, NT>(z, q, 1, e, sr[P + q], nullptr, jm);
+    }
+    __syncthreads();
+    if (ty > 0)
+#pragma unroll
+      for (int l = 0; l < NV; ++l) jp[l] = sj[0][ty - 1][l][tx];
+    if (ty < 3)
+#pragma unroll
+      for (int l = 0; l < NV; ++l) jm[l] = sj[1][ty + 1][l][tx];
+    const bool hi = i > 0, hj = j > 0, hk = k > 0;
+    const bool lo_i = i + 1 < z.mx, lo_j = j + 1 < z.my;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      R a = 0;
+      if (hi) a += ip[l];
+      if (hj) a += jp[l];
+      if (hk) a += pk[l];
+      if (lo_i) a -= im[l];
+      if (lo_j) a -= jm[l];
+      acc[l] = a;
+    }
+    // this plane's plus-side k term arrives with the next plane
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      accp[l] = acc[l];
+      pk[l] = pnext[l];
+    }
+    dtp = z.dt[id];
+    diagp = (R)1 + dtp * (sr[id] + sr[P + id] + sr[2 * P + id]);
+  }
+  // last plane of the chunk: plus-side k neighbour from the plane above
+  if (k1 > k0) {
+    const int kl = k1 - 1;
+    if (k1 < z.mz) {
+      const int64_t q = z.idx(ii, jj, k1);
+      jd_load<R, NS, NT>(z, p, din, q, c);
+      R mk[NV];
+      jd_dir<R, NS, NT>(z, q, 2, c, sr[2 * P + q], nullptr, mk);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) accp[l] -= mk[l];
+    }
+    if (ok) dplur_finish<R, NS, NT, CHEM>(z, p, z.idx(ii, jj, kl), accp, dtp, diagp, dout);
   }
 }
 
@@ -955,15 +1097,22 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
   }
   prof_mark(P_DQ0, st, false);
   if (last_err()) return OSTC_ERR_HIP;
+  // k-marching sweeps: (64 x 4) tiles, k split into chunks so that about
+  // 1024 workgroups (4 per CU) are in flight; at least 4 planes per chunk
+  const int tiles = ((z.mx + 63) / 64) * ((z.my + 3) / 4);
+  const int nkc = std::max(1, std::min((1024 + tiles - 1) / tiles, (z.mz + 3) / 4));
+  const int kc = (z.mz + nkc - 1) / nkc;
+  const dim3 sgrid((unsigned)((z.mx + 63) / 64), (unsigned)((z.my + 3) / 4), (unsigned)((z.mz + kc - 1) / kc));
+  const dim3 sblock(64, 4, 1);
   R *in = z.dq0, *out = z.dqk;
   for (int s = 0; s < o.dplur_steps; ++s) {
     prof_mark(P_DPLUR, st, true);
     if (chem) {
-      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp,
-                                               (const R *)in, out));
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true>), sgrid, sblock, 0, st, z, dp,
+                                               (const R *)in, out, kc));
     } else {
-      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp,
-                                               (const R *)in, out));
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false>), sgrid, sblock, 0, st, z, dp,
+                                               (const R *)in, out, kc));
     }
     prof_mark(P_DPLUR, st, false);
     if (last_err()) return OSTC_ERR_HIP;

@@ -328,11 +328,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
       at = __shfl(at, first);
       if (why != 0) io.fb_list[at + __popcll(fail & ((1ull << ln) - 1ull))] = c;
     }
-    if (io.stats && ln == 0) {
-      const unsigned long long valid = __ballot(c < n);
-      atomicAdd(&io.stats[0], (unsigned long long)__popcll(valid));
-    }
     if (io.stats) {
+      const unsigned long long valid = __ballot(c < n);
+      if (ln == 0) atomicAdd(&io.stats[0], (unsigned long long)__popcll(valid));
 #pragma unroll
       for (int r = 1; r <= 3; ++r) {
         const unsigned long long m = __ballot(why == r);
