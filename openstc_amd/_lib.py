@@ -104,7 +104,7 @@ class SurDesc(C.Structure):
                 ("loss_w", C.c_float * SUR_MAX_OUT), ("elem_w", C.c_float * (SUR_MAX_ELEM * 9)),
                 ("floor", C.c_float), ("t_min", C.c_float), ("t_max", C.c_float), ("p_min", C.c_float),
                 ("p_max", C.c_float), ("dt_min", C.c_float), ("dt_max", C.c_float),
-                ("y_tol", C.c_float), ("elem_tol", C.c_float), ("elem_abs", C.c_float)]
+                ("y_tol", C.c_float), ("elem_tol", C.c_float), ("elem_abs", C.c_float), ("dy_max", C.c_float)]
 
 
 class ZoneDesc(C.Structure):

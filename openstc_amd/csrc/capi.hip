@@ -715,6 +715,7 @@ int sur_check_desc(const ostc_surrogate_desc &d) {
   if (d.kind == 1 && d.n_out != 9) return fail(OSTC_ERR_ARG, "surrogate kind 1 has 9 outputs");
   if (d.kind < 0 || d.kind > 1) return fail(OSTC_ERR_ARG, "surrogate kind must be 0 or 1");
   if (d.n_elem < 0 || d.n_elem > OSTC_SUR_MAX_ELEM) return fail(OSTC_ERR_ARG, "n_elem out of range");
+  if (!(d.dy_max > 0)) return fail(OSTC_ERR_ARG, "dy_max must be positive");
   for (int f = 0; f < OSTC_SUR_IN; ++f)
     if (!(d.in_std[f] > 0)) return fail(OSTC_ERR_ARG, "in_std must be positive");
   return OSTC_OK;
@@ -738,6 +739,7 @@ ostc::SurParams sur_params(const ostc_surrogate_desc &d) {
   p.t_min = d.t_min, p.t_max = d.t_max, p.p_min = d.p_min, p.p_max = d.p_max;
   p.dt_min = d.dt_min, p.dt_max = d.dt_max;
   p.y_tol = d.y_tol, p.elem_tol = d.elem_tol, p.elem_abs = d.elem_abs;
+  p.dy_max = d.dy_max;
   return p;
 }
 

@@ -42,16 +42,26 @@ constexpr int kStageLd = 17;           // padded LDS row of the output staging
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// GELU, tanh form written as z*sigmoid(u), u = 2*sqrt(2/pi)*(z + 0.044715 z^3)
-__device__ __forceinline__ float gelu_f(float z) {
-  const float u = kGeluC * (z + kGeluA * z * z * z);
-  const float e = fast_exp2(-u * kLog2e);
-  return z * fast_rcp(1.0f + e);
+// GELU, tanh form written as z*sigmoid(u), u = 2*sqrt(2/pi)*(z + 0.044715 z^3):
+// exp(-u) = 2^(z*(C + C*0.044715*z^2)) with C = -2*sqrt(2/pi)*log2(e), so one
+// activation is 5 VALU operations and two transcendentals.
+constexpr float kGeluE = -2.3022081983490023f;             // -2*sqrt(2/pi)*log2(e)
+constexpr float kGeluE3 = -2.3022081983490023f * 0.044715f;
+__device__ __forceinline__ float gelu_sig(float z) {
+  const float q = fmaf(z * z, kGeluE3, kGeluE);
+  return fast_rcp(1.0f + fast_exp2(z * q));
 }
+__device__ __forceinline__ float gelu_f(float z) { return z * gelu_sig(z); }
 __device__ __forceinline__ float gelu_grad_f(float z) {
-  const float u = kGeluC * (z + kGeluA * z * z * z);
-  const float s = fast_rcp(1.0f + fast_exp2(-u * kLog2e));
+  const float s = gelu_sig(z);
   return s + z * s * (1.0f - s) * kGeluC * (1.0f + 3.0f * kGeluA * z * z);
+}
+
+// two FP32 values -> packed BF16 pair (round to nearest even)
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  const bf16x2 v = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(unsigned, v);
 }
 
 __device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
@@ -93,20 +103,21 @@ struct ZoneSrc {
 };
 
 // The 12 raw features in FP32, standardised: 1000/T, ln(p/1 atm),
-// BoxCox_0.1(Y_k) = (Y^0.1 - 1)/0.1, ln dt.
-__device__ __forceinline__ float feature(const SurParams &s, int f, double T, double P, const double *Y, double DT) {
-  float v;
-  if (f == 0) {
-    v = 1000.0f / (float)T;
-  } else if (f == 1) {
-    v = logf((float)P * kInvAtm);
-  } else if (f < 11) {
-    const float y = fmaxf((float)Y[f - 2], 0.0f);
-    v = (powf(y, kBoxCox) - 1.0f) * kInvBoxCox;
-  } else {
-    v = logf((float)DT);
+// BoxCox_0.1(Y_k) = (Y^0.1 - 1)/0.1, ln dt (logarithms and powers through the
+// hardware log2/exp2).
+constexpr float kLn2 = 0.6931471805599453f;
+__device__ __forceinline__ void features(const SurParams &s, double T, double P, const double *Y, double DT,
+                                         float *f) {
+  f[0] = 1000.0f / (float)T;
+  f[1] = __builtin_amdgcn_logf((float)P * kInvAtm) * kLn2;
+#pragma unroll
+  for (int k = 0; k < kNs; ++k) {
+    const float y = fmaxf((float)Y[k], 0.0f);
+    f[2 + k] = (fast_exp2(kBoxCox * __builtin_amdgcn_logf(y)) - 1.0f) * kInvBoxCox;
   }
-  return (v - s.in_mean[f]) * s.in_rstd[f];
+  f[11] = __builtin_amdgcn_logf((float)DT) * kLn2;
+#pragma unroll
+  for (int q = 0; q < kSurIn; ++q) f[q] = (f[q] - s.in_mean[q]) * s.in_rstd[q];
 }
 
 // ---- fused forward --------------------------------------------------------
@@ -125,6 +136,29 @@ struct InferOut {
   float *raw;                    // optional [n][16] standardised outputs
 };
 
+// GELU of two accumulator quads (units 4g..4g+3 of two 16-unit tiles) packed
+// into the next layer's B operand
+__device__ __forceinline__ bf16x8 gelu_pack(const f32x4 &a0, const f32x4 &a1) {
+  uint4 u;
+  u.x = pack_bf16(gelu_f(a0[0]), gelu_f(a0[1]));
+  u.y = pack_bf16(gelu_f(a0[2]), gelu_f(a0[3]));
+  u.z = pack_bf16(gelu_f(a1[0]), gelu_f(a1[1]));
+  u.w = pack_bf16(gelu_f(a1[2]), gelu_f(a1[3]));
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+// training: pre-activations z and BF16-rounded activations of hidden layer l
+// for cell c, units u0..u0+3 (first tile) and u0+16..u0+19 (second tile)
+__device__ __forceinline__ void save_layer(const TrainOut &tr, int l, long n, long c, int u0, const f32x4 &z0,
+                                           const f32x4 &z1, const bf16x8 &h) {
+  if (c >= n) return;
+  float *zr = tr.z + ((long)l * n + c) * 128, *ar = tr.a + ((long)l * n + c) * 144;
+  *reinterpret_cast<float4 *>(zr + u0) = make_float4(z0[0], z0[1], z0[2], z0[3]);
+  *reinterpret_cast<float4 *>(zr + u0 + 16) = make_float4(z1[0], z1[1], z1[2], z1[3]);
+  *reinterpret_cast<float4 *>(ar + u0) = make_float4((float)h[0], (float)h[1], (float)h[2], (float)h[3]);
+  *reinterpret_cast<float4 *>(ar + u0 + 16) = make_float4((float)h[4], (float)h[5], (float)h[6], (float)h[7]);
+}
+
 // LDS image offsets (in 1 KB A tiles of 64 lanes x 8 BF16)
 __device__ __forceinline__ int tile_l0(int mt) { return mt; }
 __device__ __forceinline__ int tile_h(int l, int mt, int kc) { return 8 + (l - 1) * 32 + mt * 4 + kc; }
@@ -135,7 +169,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
                                                             const float *__restrict__ bias, TrainOut tr,
                                                             InferOut io) {
   __shared__ bf16x8 sW[kSurTiles * 64];
-  __shared__ float sB[kSurBias];
+  __shared__ __attribute__((aligned(16))) float sB[kSurBias];
   __shared__ float sStage[kWaves][64 * kStageLd];
   for (int q = threadIdx.x; q < kSurTiles * 64; q += blockDim.x) sW[q] = wpk[q];
   for (int q = threadIdx.x; q < kSurBias; q += blockDim.x) sB[q] = bias[q];
@@ -145,23 +179,42 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
   const long n = src.n;
   const long stride = (long)gridDim.x * kWaves * 64;
   for (long base = ((long)blockIdx.x * kWaves + wave) * 64; base < n; base += stride) {
+    float *stg = sStage[wave];
+    // ---- features, one lane per cell, staged through LDS
+    {
+      const long c = base + ln;
+      float f[kSurIn];
+#pragma unroll
+      for (int q = 0; q < kSurIn; ++q) f[q] = 0.0f;
+      if (c < n) {
+        double T, P, DT, Y[kNs];
+        src.load(c, T, P, Y, DT);
+        features(s, T, P, Y, DT, f);
+      }
+#pragma unroll
+      for (int q = 0; q < kSurIn; ++q) stg[ln * kStageLd + q] = f[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // ---- layer 0 operand: slots 8g..8g+7 of the 32-slot chunk = features
     bf16x8 b0[kTiles];
 #pragma unroll
     for (int t = 0; t < kTiles; ++t) {
-      const long c = base + 16 * t + cn;
-      float f8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (c < n && g < 2) {
-        double T, P, DT, Y[kNs];
-        src.load(c, T, P, Y, DT);
+      const float *row = stg + (16 * t + cn) * kStageLd;
+      float f8[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int f = 8 * g + j;
-          if (f < kSurIn) f8[j] = feature(s, f, T, P, Y, DT);
-        }
+      for (int j = 0; j < 8; ++j) {
+        const int q = 8 * g + j;
+        f8[j] = q < kSurIn ? row[q < kSurIn ? q : 0] : 0.0f;
       }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) b0[t][j] = (__bf16)f8[j];
+      uint4 u;
+      u.x = pack_bf16(f8[0], f8[1]);
+      u.y = pack_bf16(f8[2], f8[3]);
+      u.z = pack_bf16(f8[4], f8[5]);
+      u.w = pack_bf16(f8[6], f8[7]);
+      b0[t] = __builtin_bit_cast(bf16x8, u);
+      const long c = base + 16 * t + cn;
       if (TRAIN && c < n && g < 2) {
         float4 *dst = reinterpret_cast<float4 *>(tr.a0 + c * 16 + 8 * g);
         dst[0] = make_float4((float)b0[t][0], (float)b0[t][1], (float)b0[t][2], (float)b0[t][3]);
@@ -171,39 +224,26 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
           dst[1] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);  // bias column for the weight gradient
       }
     }
+    // the staging rows are rewritten with the outputs below
+    __builtin_amdgcn_wave_barrier();
 
     bf16x8 bc[4][kTiles];  // current layer input: [k chunk][slab]
-    // ---- layer 0: 12 (padded 32) -> 128
+    // ---- layer 0: 12 (padded 32) -> 128; the bias enters as the MFMA's C
 #pragma unroll
     for (int kc = 0; kc < 4; ++kc) {
       f32x4 acc[2][kTiles];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
+        const f32x4 bz = *reinterpret_cast<const f32x4 *>(&sB[32 * kc + 16 * h + 4 * g]);
         const bf16x8 a = sW[tile_l0(2 * kc + h) * 64 + ln];
 #pragma unroll
         for (int t = 0; t < kTiles; ++t)
-          acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0[t], f32x4{0, 0, 0, 0}, 0, 0, 0);
+          acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0[t], bz, 0, 0, 0);
       }
 #pragma unroll
       for (int t = 0; t < kTiles; ++t) {
-        const long c = base + 16 * t + cn;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int u0 = 32 * kc + 16 * h + 4 * g;
-          float zz[4], hh[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            zz[i] = acc[h][t][i] + sB[u0 + i];
-            hh[i] = gelu_f(zz[i]);
-            bc[kc][t][4 * h + i] = (__bf16)hh[i];
-          }
-          if (TRAIN && c < n) {
-            *reinterpret_cast<float4 *>(tr.z + c * 128 + u0) = make_float4(zz[0], zz[1], zz[2], zz[3]);
-            *reinterpret_cast<float4 *>(tr.a + c * 144 + u0) =
-                make_float4((float)bc[kc][t][4 * h], (float)bc[kc][t][4 * h + 1], (float)bc[kc][t][4 * h + 2],
-                            (float)bc[kc][t][4 * h + 3]);
-          }
-        }
+        bc[kc][t] = gelu_pack(acc[0][t], acc[1][t]);
+        if (TRAIN) save_layer(tr, 0, n, base + 16 * t + cn, 32 * kc + 4 * g, acc[0][t], acc[1][t], bc[kc][t]);
       }
     }
     // ---- hidden layers 1..3: 128 -> 128
@@ -213,10 +253,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
 #pragma unroll
       for (int kc = 0; kc < 4; ++kc) {
         f32x4 acc[2][kTiles];
+        f32x4 bz[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-          for (int t = 0; t < kTiles; ++t) acc[h][t] = f32x4{0, 0, 0, 0};
+        for (int h = 0; h < 2; ++h) bz[h] = *reinterpret_cast<const f32x4 *>(&sB[128 * l + 32 * kc + 16 * h + 4 * g]);
 #pragma unroll
         for (int ki = 0; ki < 4; ++ki)
 #pragma unroll
@@ -224,28 +263,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
             const bf16x8 a = sW[tile_h(l, 2 * kc + h, ki) * 64 + ln];
 #pragma unroll
             for (int t = 0; t < kTiles; ++t)
-              acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[ki][t], acc[h][t], 0, 0, 0);
+              acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[ki][t], ki == 0 ? bz[h] : acc[h][t], 0, 0, 0);
           }
 #pragma unroll
         for (int t = 0; t < kTiles; ++t) {
-          const long c = base + 16 * t + cn;
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int u0 = 32 * kc + 16 * h + 4 * g;
-            float zz[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              zz[i] = acc[h][t][i] + sB[128 * l + u0 + i];
-              bn[kc][t][4 * h + i] = (__bf16)gelu_f(zz[i]);
-            }
-            if (TRAIN && c < n) {
-              *reinterpret_cast<float4 *>(tr.z + ((long)l * n + c) * 128 + u0) =
-                  make_float4(zz[0], zz[1], zz[2], zz[3]);
-              *reinterpret_cast<float4 *>(tr.a + ((long)l * n + c) * 144 + u0) =
-                  make_float4((float)bn[kc][t][4 * h], (float)bn[kc][t][4 * h + 1], (float)bn[kc][t][4 * h + 2],
-                              (float)bn[kc][t][4 * h + 3]);
-            }
-          }
+          bn[kc][t] = gelu_pack(acc[0][t], acc[1][t]);
+          if (TRAIN) save_layer(tr, l, n, base + 16 * t + cn, 32 * kc + 4 * g, acc[0][t], acc[1][t], bn[kc][t]);
         }
       }
 #pragma unroll
@@ -270,22 +293,20 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
     }
     // ---- output layer: 128 -> n_out (padded 16)
     f32x4 acc[kTiles];
-#pragma unroll
-    for (int t = 0; t < kTiles; ++t) acc[t] = f32x4{0, 0, 0, 0};
+    const f32x4 bo = *reinterpret_cast<const f32x4 *>(&sB[512 + 4 * g]);
 #pragma unroll
     for (int ki = 0; ki < 4; ++ki) {
       const bf16x8 a = sW[tile_out(ki) * 64 + ln];
 #pragma unroll
       for (int t = 0; t < kTiles; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[ki][t], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[ki][t], ki == 0 ? bo : acc[t], 0, 0, 0);
     }
-    float *stg = sStage[wave];
 #pragma unroll
     for (int t = 0; t < kTiles; ++t) {
       const long c = base + 16 * t + cn;
       float o[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) o[i] = acc[t][i] + sB[512 + 4 * g + i];
+      for (int i = 0; i < 4; ++i) o[i] = acc[t][i];
       if (TRAIN) {
         if (c < n) *reinterpret_cast<float4 *>(tr.yout + c * 16 + 4 * g) = make_float4(o[0], o[1], o[2], o[3]);
       } else {

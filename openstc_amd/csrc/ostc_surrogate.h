@@ -42,6 +42,7 @@ struct SurParams {
   float floor;
   float t_min, t_max, p_min, p_max, dt_min, dt_max;
   float y_tol, elem_tol, elem_abs;
+  float dy_max;
 };
 
 struct SurState {
@@ -79,7 +80,8 @@ __device__ __forceinline__ void sur_decode(const SurParams &s, const float *o, f
 }
 
 // 0 = accepted; 1 = input outside the trained box; 2 = Y+dY outside
-// [-y_tol, 1+y_tol]; 3 = element balance sum_k a_jk dY_k/W_k violated.
+// [-y_tol, 1+y_tol] or |dY_k| > dy_max; 3 = element balance
+// sum_k a_jk dY_k/W_k violated.
 __device__ __forceinline__ int sur_check(const SurParams &s, double T, double P, double DT, const double *Y,
                                          const float *dy) {
   bool in = (T >= s.t_min && T <= s.t_max && P >= s.p_min && P <= s.p_max && DT >= s.dt_min && DT <= s.dt_max);
@@ -90,7 +92,7 @@ __device__ __forceinline__ int sur_check(const SurParams &s, double T, double P,
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     const float yn = (float)Y[k] + dy[k];
-    out = out && (yn >= -s.y_tol && yn <= 1.0f + s.y_tol);
+    out = out && (yn >= -s.y_tol && yn <= 1.0f + s.y_tol) && fabsf(dy[k]) <= s.dy_max;
   }
   if (!out) return 2;
   float tot = 0.0f, worst = 0.0f;

@@ -138,6 +138,7 @@ def make_desc(t, p, y, dt, dy, param: L.Param, info: L.MechanismInfo, kind=KIND_
     d.p_min, d.p_max = float(np.min(p)), float(np.max(p))
     d.dt_min, d.dt_max = float(np.min(dt)), float(np.max(dt))
     d.y_tol, d.elem_tol, d.elem_abs = y_tol, elem_tol, elem_abs
+    d.dy_max = float(np.finfo(np.float32).max)
     return d
 
 
@@ -174,7 +175,10 @@ def desc_to_dict(d: L.SurDesc) -> dict:
 
 def desc_from_dict(m) -> L.SurDesc:
     d = L.SurDesc()
+    d.dy_max = float(np.finfo(np.float32).max)
     for name, typ in d._fields_:
+        if name not in m:
+            continue
         v = m[name]
         if v.ndim:
             arr = getattr(d, name)
@@ -267,14 +271,43 @@ def interior_states(sim, b: int = 0):
     return t, p, y, dt[sl].ravel()
 
 
-def harvest(case, n_steps: int, every: int, per_snapshot: int, seed: int = 0, log=print):
+def zone_plain_dy(sim, b: int = 0) -> np.ndarray:
+    """Plain-step dY of every interior cell of block b (device kernel)."""
+    z = sim.zones[b]
+    n = z.mx * z.my * z.mz
+    out = DeviceArray(nbytes=n * N_SPEC * 8)
+    L.check(sim.ctx.lib.ostc_chem_zone_step(sim.ctx.h, z.h, None, 0, out.p, None, None), "zone chemistry")
+    dy = out.get(np.float64, (n, N_SPEC))
+    out.free()
+    return dy
+
+
+def stratified_pick(dy: np.ndarray, count: int, rng, lo=-12.0, hi=0.0, bins=12) -> np.ndarray:
+    """Indices drawn evenly over decades of max_k|dY_k| (cells with large
+    changes are rare but carry most of the production rates)."""
+    mag = np.log10(np.maximum(np.abs(dy).max(1), 10.0 ** lo))
+    edges = np.linspace(lo, hi, bins + 1)
+    which = np.clip(np.searchsorted(edges, mag, side="right") - 1, 0, bins - 1)
+    groups = [np.flatnonzero(which == b) for b in range(bins)]
+    groups = [g for g in groups if g.size]
+    per = count // max(1, len(groups))
+    picks, spare = [], 0
+    for g in sorted(groups, key=len):
+        take = min(g.size, per + spare)
+        spare = per + spare - take
+        picks.append(rng.choice(g, size=take, replace=False))
+    return np.concatenate(picks)
+
+
+def harvest(case, n_steps: int, every: int, per_snapshot: int, seed: int = 0, log=print, stratify=True):
     """Run the solver on the GPU and sample interior states every `every`
-    iterations (the dt is the local time step the iteration used)."""
+    iterations together with their plain-step labels (the dt is the local time
+    step the iteration used).  Returns t, p, y, dt, dy, iteration."""
     from .runtime import Simulation
     rng = np.random.default_rng(seed)
     sim = Simulation(case)
     sim.initialize_computation()
-    out = [[], [], [], [], []]
+    out = [[], [], [], [], [], []]
     t0 = time.time()
     for s in range(1, n_steps + 1):
         sim.step(output_screen=1 << 30)
@@ -283,16 +316,15 @@ def harvest(case, n_steps: int, every: int, per_snapshot: int, seed: int = 0, lo
             if not (np.isfinite(t).all() and np.isfinite(y).all()):
                 log(f"[harvest] non-finite state at iteration {s}; stopping")
                 break
-            pick = rng.choice(len(t), size=min(per_snapshot, len(t)), replace=False)
-            for lst, a in zip(out, (t, p, y, dt, np.full(len(t), s))):
+            dy = zone_plain_dy(sim)
+            m = min(per_snapshot, len(t))
+            pick = stratified_pick(dy, m, rng) if stratify else rng.choice(len(t), size=m, replace=False)
+            for lst, a in zip(out, (t, p, y, dt, dy, np.full(len(t), s))):
                 lst.append(a[pick])
             log(f"[harvest] iteration {s}: T {t.min():.0f}..{t.max():.0f} K, dt {dt.min():.2e}..{dt.max():.2e} s, "
-                f"{time.time() - t0:.1f}s")
+                f"max|dY| {np.abs(dy).max():.2e}, {len(pick)} picked, {time.time() - t0:.1f}s")
     sim.close()
-    t, p, y, dt, it = (np.concatenate(v) for v in out)
-    y = np.maximum(y, 0.0)
-    y /= y.sum(1, keepdims=True)
-    return t, p, y, dt, it
+    return tuple(np.concatenate(v) for v in out)
 
 
 def plain_step(ctx, t, p, y, dt) -> np.ndarray:
@@ -356,6 +388,39 @@ def evaluate(sur: Surrogate, n, t, p, y, dt, dy_ref: np.ndarray, out=None) -> di
     if out is None:
         dyd.free()
     return error_report(dy, dy_ref, st)
+
+
+def calibrate_dy_max(sur: Surrogate, t, p, y, dt, dy_ref, target: float = 8e-4):
+    """Largest trusted output range dy_max for which the surrogate with
+    fallback meets rel_l2 <= target on a calibration set (cells whose
+    predicted max|dY_k| exceeds dy_max go to the plain step).  Returns
+    (dy_max, table of (dy_max, rel_l2, fallback))."""
+    n = len(t)
+    d = [DeviceArray(np.ascontiguousarray(a, np.float64)) for a in (t, p, y, dt)]
+    raw = DeviceArray(nbytes=n * 16 * 4)
+    out = DeviceArray(nbytes=n * N_SPEC * 8)
+    sur.forward(n, *(a.p for a in d), raw.p, out.p)
+    o = raw.get(np.float32, (n, 16)).astype(np.float64)
+    dy_dev = out.get(np.float64, (n, N_SPEC))
+    for a in d + [raw, out]:
+        a.free()
+    desc = sur.desc
+    de = o[:, :desc.n_out] * np.array(desc.out_std[:desc.n_out]) + np.array(desc.out_mean[:desc.n_out])
+    pred = decode(de, desc.kind, desc.floor)
+    rejected = np.all(dy_dev == dy_ref, axis=1)  # the device's own checks fell back to the plain step
+    base = np.where(rejected[:, None], dy_ref, pred)
+    mag = np.abs(pred).max(1)
+    norm = np.linalg.norm(dy_ref)
+    table = []
+    best = None
+    for thr in 10.0 ** np.arange(-10.0, 0.01, 0.25):
+        use_ref = rejected | (mag > thr)
+        final = np.where(use_ref[:, None], dy_ref, base)
+        rel = float(np.linalg.norm(final - dy_ref) / norm)
+        table.append((float(thr), rel, float(use_ref.mean())))
+        if rel <= target:
+            best = float(thr)
+    return (best if best is not None else 1e-10), table
 
 
 def error_report(dy: np.ndarray, ref: np.ndarray, stats) -> dict:

@@ -6,9 +6,12 @@
    on the GPU and sample interior states (T, p, Y, local dt) every `every`
    iterations (training seed) and, separately, from a run with another
    initial perturbation (held-out seed).
-2. Label every state with the plain point-implicit step on the GPU (FP64).
+2. Label every cell of each snapshot with the plain point-implicit step on
+   the GPU (FP64) and draw training states evenly over decades of max|dY|.
 3. Train the MLP with the library's kernels (Adam, warm-up + cosine).
-4. Save weights, descriptor, training history and a held-out sample.
+4. Calibrate the trusted output range dy_max on half of the held-out run
+   (largest value meeting the error target) and report the other half.
+5. Save weights, descriptor, training history and a held-out sample.
 """
 from __future__ import annotations
 
@@ -34,7 +37,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=400, help="solver iterations per harvest run")
     ap.add_argument("--every", type=int, default=20)
     ap.add_argument("--per-snapshot", type=int, default=100000)
-    ap.add_argument("--val-per-snapshot", type=int, default=10000)
+    ap.add_argument("--val-per-snapshot", type=int, default=20000)
     ap.add_argument("--cfl", type=float, default=0.2)
     ap.add_argument("--grid", type=int, nargs=3, default=[192, 96, 96])
     ap.add_argument("--epochs", type=int, default=100)
@@ -42,37 +45,57 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--keep-val", type=int, default=8192, help="held-out states stored with the weights")
+    ap.add_argument("--uniform", action="store_true", help="sample training states uniformly, not by |dY| decade")
+    ap.add_argument("--mag-weight", type=float, default=16.0)
+    ap.add_argument("--target", type=float, default=8e-4, help="rel_l2 the output gate is calibrated to")
     a = ap.parse_args(argv)
     mx, my, mz = a.grid
     t0 = time.time()
     log(f"[train_surrogate] harvest {mx}x{my}x{mz}, {a.steps} iterations, CFL {a.cfl}")
     tr = S.harvest(cases.config3(mx=mx, my=my, mz=mz, cfl=a.cfl, seed=3), a.steps, a.every, a.per_snapshot,
-                   seed=a.seed, log=log)
+                   seed=a.seed, log=log, stratify=not a.uniform)
+    # held-out: another initial perturbation, sampled uniformly (the solver's own distribution)
     va = S.harvest(cases.config3(mx=mx, my=my, mz=mz, cfl=a.cfl, seed=4), a.steps, a.every, a.val_per_snapshot,
-                   seed=a.seed + 1, log=log)
+                   seed=a.seed + 1, log=log, stratify=False)
     mech, info = cases.load_mechanism(True)
     param = cases.mixture_param(mech, False, True, 4, a.cfl, 1,
                                 cases.InflowState(1.0, 1.0, 0, 0, 1e5, 1000.0, 1.0, 0.0, [0.0] * 9))
     ctx = Context(param)
-    t, p, y, dt, it = tr
-    dy = S.plain_step(ctx, t, p, y, dt)
-    vt, vp, vy, vdt, vit = va
-    vdy = S.plain_step(ctx, vt, vp, vy, vdt)
+    t, p, y, dt, dy, it = tr
+    vt, vp, vy, vdt, vdy, vit = va
     log(f"[train_surrogate] {len(t)} training / {len(vt)} held-out states labelled ({time.time() - t0:.1f}s)")
-    desc = S.make_desc(t, p, y, dt, dy, param, info)
+    desc = S.make_desc(t, p, y, dt, dy, param, info, mag_weight=a.mag_weight)
     target = S.standardised_targets(dy, desc)
     sur = S.Surrogate(ctx, desc, max_batch=a.batch, seed=a.seed)
     vdev = [S.DeviceArray(np.ascontiguousarray(v)) for v in (vt, vp, vy, vdt)]
     val = (len(vt), *vdev, vdy)
     hist = S.train(sur, t, p, y, dt, target, a.epochs, a.batch, a.lr, seed=a.seed, val=val, log=log)
-    final = S.evaluate(sur, *val)
-    log("[train_surrogate] held-out:", json.dumps(final))
-    keep = np.random.default_rng(a.seed).choice(len(vt), size=min(a.keep_val, len(vt)), replace=False)
+    ungated = S.evaluate(sur, *val)
+    log("[train_surrogate] held-out, no output gate:", json.dumps(ungated))
+    # trusted output range: calibrated on half of the held-out run, reported on the other half
+    half = np.random.default_rng(a.seed + 7).permutation(len(vt))
+    cal, tst = half[: len(vt) // 2], half[len(vt) // 2:]
+    dy_max, table = S.calibrate_dy_max(sur, vt[cal], vp[cal], vy[cal], vdt[cal], vdy[cal], target=a.target)
+    log(f"[train_surrogate] dy_max = {dy_max:.3e}")
+    for thr, rel, fb in table:
+        log(f"[calibrate] dy_max {thr:.2e} rel_l2 {rel:.3e} fallback {fb:.4f}")
+    w = sur.weights()
+    sur.close()
+    desc.dy_max = dy_max
+    sur = S.Surrogate(ctx, desc, w)
+    tdev = [S.DeviceArray(np.ascontiguousarray(v[tst])) for v in (vt, vp, vy, vdt)]
+    final = S.evaluate(sur, len(tst), *tdev, vdy[tst])
+    log("[train_surrogate] held-out test half, gated:", json.dumps(final))
+    for d in tdev:
+        d.free()
+    keep = tst[np.random.default_rng(a.seed).choice(len(tst), size=min(a.keep_val, len(tst)), replace=False)]
     sur.save(a.out, history=np.array(json.dumps(hist)), heldout_t=vt[keep], heldout_p=vp[keep],
              heldout_y=vy[keep], heldout_dt=vdt[keep], heldout_dy=vdy[keep],
              info=np.array(json.dumps({"grid": a.grid, "cfl": a.cfl, "steps": a.steps, "every": a.every,
                                        "n_train": int(len(t)), "n_heldout": int(len(vt)), "epochs": a.epochs,
-                                       "batch": a.batch, "lr": a.lr, "heldout": final})))
+                                       "batch": a.batch, "lr": a.lr, "mag_weight": a.mag_weight,
+                                       "heldout_ungated": ungated, "dy_max": dy_max, "target": a.target,
+                                       "heldout_test_gated": final})))
     log(f"[train_surrogate] wrote {a.out} ({time.time() - t0:.1f}s)")
     for d in vdev:
         d.free()

@@ -180,3 +180,46 @@ def test_zone_plain_step_equals_batch(env):
     assert np.array_equal(got, ref)
     out.free()
     sim.close()
+
+
+def test_trained_surrogate_gain(env):
+    """The committed network (trained on the GPU by these kernels) against the
+    plain integrator on held-out solver states stored with it: error and
+    fallback rate at the calibrated output gate, and the time of both on the
+    same 1M-state batch."""
+    import time
+
+    ctx = env[0]
+    m = np.load(S.DEFAULT_WEIGHTS, allow_pickle=False)
+    sur = S.Surrogate.load(ctx)
+    t, p, y, dt, dy = (np.ascontiguousarray(m["heldout_" + k]) for k in ("t", "p", "y", "dt", "dy"))
+    n = len(t)
+    say("held-out states", n, "dy_max", sur.desc.dy_max)
+    d = [S.DeviceArray(a) for a in (t, p, y, dt)]
+    rep = S.evaluate(sur, n, *d, dy)
+    say("held-out", rep)
+    assert rep["rel_l2"] <= 1e-3, rep
+    assert rep["fallback"] <= 0.1, rep
+    # timing on a 1M-state batch (the held-out set repeated)
+    reps = (1 << 20) // n
+    big = [S.DeviceArray(np.ascontiguousarray(np.concatenate([a] * reps))) for a in (t, p, y, dt)]
+    nb = n * reps
+    out = S.DeviceArray(nbytes=nb * 9 * 8)
+    lib = ctx.lib
+
+    def timed(fn, k=5):
+        fn()
+        L.check(lib.ostc_device_sync(), "sync")
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        L.check(lib.ostc_device_sync(), "sync")
+        return (time.perf_counter() - t0) / k
+
+    tp = timed(lambda: L.check(lib.ostc_chem_plain_step(ctx.h, nb, *(a.p for a in big), out.p, None), "plain"))
+    ts = timed(lambda: L.check(lib.ostc_surrogate_step(sur.h, nb, *(a.p for a in big), out.p, None, None), "sur"))
+    say("1M states: plain %.3f ms, surrogate %.3f ms, speedup %.2f" % (tp * 1e3, ts * 1e3, tp / ts))
+    assert ts < tp
+    for a in d + big + [out]:
+        a.free()
+    sur.close()
