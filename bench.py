@@ -32,13 +32,13 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE on this workload (FETCH_SIZE scaled x2.0 and
 # WRITE_SIZE x1.0, the factors a 512 MiB copy kernel gives).  Keyed by
 # (workload, stage); the summary with the commands is
-# profiles/config2_sst_dplur_256x128x64_kernels.md.  Re-measured whenever the
-# kernel changes; a stage without an entry reports null.
+# profiles/kernels_rans_256x128x64_and_fr_192x96x96.md.  Re-measured whenever
+# the kernel changes; a stage without an entry reports null.
 MEASURED_TRAFFIC = {
-    ("config2_air_sst_dplur", "dplur_sweep"): 1108.0e6,
-    ("config2_air_sst_dplur", "flux_i"): 492.0e6,
-    ("config2_air_sst_dplur", "flux_j"): 547.0e6,
-    ("config2_air_sst_dplur", "flux_k"): 722.0e6,
+    ("config2_air_sst_dplur", "dplur_sweep"): 496.6e6,
+    ("config2_air_sst_dplur", "flux_i"): 382.5e6,
+    ("config2_air_sst_dplur", "flux_j"): 425.8e6,
+    ("config2_air_sst_dplur", "flux_k"): 379.0e6,
 }
 
 
