@@ -139,9 +139,11 @@ __device__ __forceinline__ double arrhenius(double t, double log_t, double A, do
 }
 
 // g_k/(R T) (src/Thermo.cu:63-79, evaluated at T_low below T_low)
-__device__ __forceinline__ void gibbs_div_rt(double t, const ostc_param *__restrict__ p, int ns, double *g) {
+template <int NS>
+__device__ __forceinline__ void gibbs_div_rt(double t, const ostc_param *__restrict__ p, double *g) {
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t_inv = 1 / t, log_t = log(t);
-  for (int i = 0; i < ns; ++i) {
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
     if (t < p->t_low[i]) {
       const double tt = p->t_low[i];
       const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt_inv = 1 / tt, log_tt = log(tt);
@@ -163,149 +165,200 @@ __device__ __forceinline__ double ipow(double c, int n) {
   return r;
 }
 
-// Rates of progress q1 (forward), q2 (backward) in mol/cm3/s at (rho, T, Y):
-// src/FiniteRateChem.cu:7-153.
-template <int NS, int NR>
-__device__ __forceinline__ void rates_of_progress(const ostc_param *__restrict__ p, int ns, int nr, double rho,
-                                                  double t, const double *y, double *q1, double *q2) {
-  double c[NS];
-  for (int l = 0; l < ns; ++l) c[l] = rho * y[l] / p->mw[l] * 1e-3;
-  double g[NS];
-  gibbs_div_rt(t, p, ns, g);
-  const double log_t = log(t);
-  const double temp_t = kPatm / kRu * 1e-3 / t;
-  for (int i = 0; i < nr; ++i) {
-    const int type = p->reac_type[i];
-    double kf = arrhenius(t, log_t, p->A[i], p->b[i], p->Ea[i]);
-    if (type == 3) kf += arrhenius(t, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
-    if (type > 3) {
-      double cc = 0;
-      for (int l = 0; l < ns; ++l) cc += c[l] * p->third_body_coeff[i * kMaxSpec + l];
-      if (type == 4) {
-        kf *= cc;
-      } else {
-        const double kf_low = arrhenius(t, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
-        const double kf_high = kf;
-        const double pr = kf_low * cc / kf_high;
-        double fv = 1.0;
-        if (type > 5) {
-          double f_cent = (1 - p->troe_alpha[i]) * exp(-t / p->troe_t3[i]) + p->troe_alpha[i] * exp(-t / p->troe_t1[i]);
-          if (type == 7) f_cent += exp(-p->troe_t2[i] / t);
-          const double logFc = log10(f_cent);
-          const double cc1 = -0.4 - 0.67 * logFc;
-          const double n = 0.75 - 1.27 * logFc;
-          const double logPr = log10(pr);
-          const double tempo = (logPr + cc1) / (n - 0.14 * (logPr + cc1));
-          fv = exp10(logFc / (1.0 + tempo * tempo));
-        }
-        kf = kf_high * pr / (1.0 + pr) * fv;
-      }
-    }
-    double kb;
-    if (type != 2) {
-      double d_gibbs = 0;
-      for (int l = 0; l < ns; ++l) {
-        const int dn = p->stoi_b[i * kMaxSpec + l] - p->stoi_f[i * kMaxSpec + l];
-        if (dn != 0) d_gibbs += g[l] * dn;
-      }
-      const int order = p->reac_order[i];
-      // kb = kf / Kc with Kc = temp_t^order * exp(-d_gibbs)
-      kb = kf * exp(d_gibbs) / ipow(order >= 0 ? temp_t : 1.0 / temp_t, order >= 0 ? order : -order);
+// Forward / backward rate of progress (mol/cm3/s) of reaction i at
+// concentrations c (mol/cm3) and g_k/RT: one iteration of
+// src/FiniteRateChem.cu:7-153.  nf/nb: the reaction's stoichiometric
+// coefficients (uniform across the wavefront).
+template <int NS>
+__device__ __forceinline__ void reaction_rates(const ostc_param *__restrict__ p, int i, double t, double log_t,
+                                               double temp_t, const double *c, const double *g, const int *nf,
+                                               const int *nb, double &q1, double &q2) {
+  const int type = p->reac_type[i];
+  double kf = arrhenius(t, log_t, p->A[i], p->b[i], p->Ea[i]);
+  if (type == 3) kf += arrhenius(t, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
+  if (type > 3) {
+    double cc = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) cc += c[l] * p->third_body_coeff[i * kMaxSpec + l];
+    if (type == 4) {
+      kf *= cc;
     } else {
-      kb = arrhenius(t, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
+      const double kf_low = arrhenius(t, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
+      const double kf_high = kf;
+      const double pr = kf_low * cc / kf_high;
+      double fv = 1.0;
+      if (type > 5) {
+        double f_cent = (1 - p->troe_alpha[i]) * exp(-t / p->troe_t3[i]) + p->troe_alpha[i] * exp(-t / p->troe_t1[i]);
+        if (type == 7) f_cent += exp(-p->troe_t2[i] / t);
+        const double logFc = log10(f_cent);
+        const double cc1 = -0.4 - 0.67 * logFc;
+        const double n = 0.75 - 1.27 * logFc;
+        const double logPr = log10(pr);
+        const double tempo = (logPr + cc1) / (n - 0.14 * (logPr + cc1));
+        fv = exp10(logFc / (1.0 + tempo * tempo));
+      }
+      kf = kf_high * pr / (1.0 + pr) * fv;
     }
-    double a = kf, b = kb;
-    for (int l = 0; l < ns; ++l) {
-      const int nf = p->stoi_f[i * kMaxSpec + l], nb = p->stoi_b[i * kMaxSpec + l];
-      if (nf) a *= ipow(c[l], nf);
-      if (nb) b *= ipow(c[l], nb);
-    }
-    q1[i] = a;
-    q2[i] = b;
   }
+  double kb;
+  if (type != 2) {
+    double d_gibbs = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const int dn = nb[l] - nf[l];
+      if (dn != 0) d_gibbs += g[l] * dn;
+    }
+    const int order = p->reac_order[i];
+    // kb = kf / Kc with Kc = temp_t^order * exp(-d_gibbs)
+    kb = kf * exp(d_gibbs) / ipow(order >= 0 ? temp_t : 1.0 / temp_t, order >= 0 ? order : -order);
+  } else {
+    kb = arrhenius(t, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
+  }
+  double a = kf, b = kb;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    if (nf[l]) a *= ipow(c[l], nf[l]);
+    if (nb[l]) b *= ipow(c[l], nb[l]);
+  }
+  q1 = a;
+  q2 = b;
 }
 
-// omega_k [kg/m3/s] and optionally the EPI Jacobian (row-major m*ns+n) or the
-// DA diagonal (src/FiniteRateChem.cu:155-209)
+// omega_k [kg/m3/s] and optionally the EPI Jacobian (row-major m*NS+n) or the
+// DA diagonal (src/FiniteRateChem.cu:155-209).  One pass over the reactions:
+// each reaction's q1/q2 is formed and immediately added to the production
+// sums and the Jacobian (same summation order over reactions as the
+// reference), so no per-reaction array is kept and every per-species array
+// has compile-time indices (registers, no scratch).
 template <int NS, int NR>
 __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, int ns, int nr, double rho, double t,
                                             const double *y, double *omega, double *jac_epi, double *jac_da) {
-  double q1[NR], q2[NR];
-  rates_of_progress<NS, NR>(p, ns, nr, rho, t, y, q1, q2);
-  for (int i = 0; i < ns; ++i) {
-    double creation = 0, od = 0;
-    for (int j = 0; j < nr; ++j) {
-      const int nf = p->stoi_f[j * kMaxSpec + i], nb = p->stoi_b[j * kMaxSpec + i];
-      creation += q2[j] * nf + q1[j] * nb;
-      od += q1[j] * nf + q2[j] * nb;
+  (void)ns;
+  double c[NS], g[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c[l] = rho * y[l] / p->mw[l] * 1e-3;
+  gibbs_div_rt<NS>(t, p, g);
+  const double log_t = log(t);
+  const double temp_t = kPatm / kRu * 1e-3 / t;
+  double creation[NS], od[NS], zz[NS * NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) creation[l] = od[l] = 0;
+  if (jac_epi) {
+#pragma unroll
+    for (int q = 0; q < NS * NS; ++q) zz[q] = 0;
+  }
+  for (int r = 0; r < nr && r < NR; ++r) {
+    int nf[NS], nb[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      nf[l] = p->stoi_f[r * kMaxSpec + l];
+      nb[l] = p->stoi_b[r * kMaxSpec + l];
     }
-    creation *= p->mw[i] * 1e+3;
-    od *= p->mw[i] * 1e+3;
-    omega[i] = creation - od;
-    if (jac_da) jac_da[i] = y[i] > 1e-25 ? -od / (y[i] * rho) : 0.0;
+    double q1, q2;
+    reaction_rates<NS>(p, r, t, log_t, temp_t, c, g, nf, nb, q1, q2);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      creation[i] += q2 * nf[i] + q1 * nb[i];
+      od[i] += q1 * nf[i] + q2 * nb[i];
+    }
+    if (jac_epi) {
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        const int dm = nb[m] - nf[m];
+        if (dm != 0)
+#pragma unroll
+          for (int n = 0; n < NS; ++n) zz[m * NS + n] += dm * (nf[n] * q1 - nb[n] * q2);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double cr = creation[i] * (p->mw[i] * 1e+3), d = od[i] * (p->mw[i] * 1e+3);
+    omega[i] = cr - d;
+    if (jac_da) jac_da[i] = y[i] > 1e-25 ? -d / (y[i] * rho) : 0.0;
   }
   if (jac_epi) {
-    for (int m = 0; m < ns; ++m)
-      for (int n = 0; n < ns; ++n) {
-        double zz = 0;
-        if (y[n] > 1e-25) {
-          for (int r = 0; r < nr; ++r) {
-            const int dm = p->stoi_b[r * kMaxSpec + m] - p->stoi_f[r * kMaxSpec + m];
-            if (dm != 0) zz += dm * (p->stoi_f[r * kMaxSpec + n] * q1[r] - p->stoi_b[r * kMaxSpec + n] * q2[r]);
-          }
-          zz /= rho * y[n];
-        }
-        jac_epi[m * ns + n] = p->mw[m] * zz * 1e+3;
-      }
+#pragma unroll
+    for (int m = 0; m < NS; ++m)
+#pragma unroll
+      for (int n = 0; n < NS; ++n)
+        jac_epi[m * NS + n] = y[n] > 1e-25 ? p->mw[m] * (zz[m * NS + n] / (rho * y[n])) * 1e+3 : 0.0;
   }
 }
 
-// Pivoted LU factorisation in place, as solve_chem_system
-// (src/FiniteRateChem.cu:286-318): Crout-style column pivoting.
+// In-place LU factorisation with partial pivoting of the reference
+// (src/FiniteRateChem.cu:286-318): Crout order, pivot = first row of maximal
+// |column entry|.  Row exchanges are written as selects over compile-time rows
+// so the matrix stays in registers.
 template <int NS>
 __device__ __forceinline__ void lu_factor(double *a, int *piv, int dim) {
-  for (int n = 0; n < dim; ++n) {
+  (void)dim;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
     int ik = n;
-    for (int m = n; m < dim; ++m) {
-      double v = a[m * dim + n];
-      for (int t = 0; t < n; ++t) v -= a[m * dim + t] * a[t * dim + n];
-      a[m * dim + n] = v;
-      if (fabs(v) > fabs(a[ik * dim + n])) ik = m;
+    double best = 0;
+#pragma unroll
+    for (int m = n; m < NS; ++m) {
+      double v = a[m * NS + n];
+#pragma unroll
+      for (int t = 0; t < n; ++t) v -= a[m * NS + t] * a[t * NS + n];
+      a[m * NS + n] = v;
+      if (m == n) {
+        best = fabs(v);
+      } else if (fabs(v) > best) {
+        best = fabs(v);
+        ik = m;
+      }
     }
     piv[n] = ik;
-    if (ik != n)
-      for (int t = 0; t < dim; ++t) {
-        const double mid = a[ik * dim + t];
-        a[ik * dim + t] = a[n * dim + t];
-        a[n * dim + t] = mid;
+#pragma unroll
+    for (int m = n + 1; m < NS; ++m) {
+      const bool sw = ik == m;
+#pragma unroll
+      for (int t = 0; t < NS; ++t) {
+        const double x = a[m * NS + t], y = a[n * NS + t];
+        a[m * NS + t] = sw ? y : x;
+        a[n * NS + t] = sw ? x : y;
       }
-    for (int q = n + 1; q < dim; ++q) {
-      double v = a[n * dim + q];
-      for (int t = 0; t < n; ++t) v -= a[n * dim + t] * a[t * dim + q];
-      a[n * dim + q] = v;
     }
-    for (int m = n + 1; m < dim; ++m) a[m * dim + n] /= a[n * dim + n];
+#pragma unroll
+    for (int q = n + 1; q < NS; ++q) {
+      double v = a[n * NS + q];
+#pragma unroll
+      for (int t = 0; t < n; ++t) v -= a[n * NS + t] * a[t * NS + q];
+      a[n * NS + q] = v;
+    }
+#pragma unroll
+    for (int m = n + 1; m < NS; ++m) a[m * NS + n] /= a[n * NS + n];
   }
 }
 
 // Forward/back substitution with the factors above (src/FiniteRateChem.cu:
-// 320-341 / 377-397)
+// 320-341 / 377-397); the row exchanges as selects.
 template <int NS>
 __device__ __forceinline__ void lu_solve(const double *a, const int *piv, double *b, int dim) {
-  for (int m = 0; m < dim; ++m) {
+  (void)dim;
+#pragma unroll
+  for (int m = 0; m < NS; ++m) {
     const int t = piv[m];
-    if (t != m) {
-      const double mid = b[t];
-      b[t] = b[m];
-      b[m] = mid;
+#pragma unroll
+    for (int r = m + 1; r < NS; ++r) {
+      const bool sw = t == r;
+      const double x = b[r], y = b[m];
+      b[r] = sw ? y : x;
+      b[m] = sw ? x : y;
     }
   }
-  for (int m = 1; m < dim; ++m)
-    for (int t = 0; t < m; ++t) b[m] -= a[m * dim + t] * b[t];
-  b[dim - 1] /= a[dim * dim - 1];
-  for (int m = dim - 2; m >= 0; --m) {
-    for (int t = m + 1; t < dim; ++t) b[m] -= a[m * dim + t] * b[t];
-    b[m] /= a[m * dim + m];
+#pragma unroll
+  for (int m = 1; m < NS; ++m)
+#pragma unroll
+    for (int t = 0; t < m; ++t) b[m] -= a[m * NS + t] * b[t];
+  b[NS - 1] /= a[NS * NS - 1];
+#pragma unroll
+  for (int m = NS - 2; m >= 0; --m) {
+#pragma unroll
+    for (int t = m + 1; t < NS; ++t) b[m] -= a[m * NS + t] * b[t];
+    b[m] /= a[m * NS + m];
   }
 }
 
