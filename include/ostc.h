@@ -155,6 +155,28 @@ int ostc_sizeof_bc(void);
 int ostc_block_geometry(int mx, int my, int mz, int ngg, const double *x_in, const double *y_in, const double *z_in,
                         double *x_out, double *y_out, double *z_out, double *jac, double *metric, int *n_neg_jac);
 
+/* InnerFace ctor + register_boundary (src/Mesh.cpp:34-178), 3-D: raw[13] is
+ * one record of the reference's inner{rank}.txt read by
+ * Mesh::read_inner_interface (src/Mesh.cpp:709-735): the 1-based ranges
+ * i1 i2 j1 j2 k1 k2 of this block, those of the target block and the 1-based
+ * target block id. */
+int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_face *out);
+
+/* Geometry of a multi-block mesh (host, FP64): per block ghost grids
+ * (Mesh::init_ghost_grid, src/Mesh.cpp:789-943), then the ghost layers across
+ * inner faces copied from the neighbour block (Mesh::init_inner_ghost_grid,
+ * src/Mesh.cpp:940-985), then Jacobian and metrics (src/Mesh.cpp:259-521).
+ * Inner faces of block b are inner[off_b .. off_b+n_inner[b]). */
+typedef struct ostc_block_coords {
+  int mx, my, mz;
+  const double *x, *y, *z;      /* interior coordinates, i fastest */
+  double *x_out, *y_out, *z_out; /* optional: with ngg ghost layers */
+  double *jac, *metric;          /* ngg-padded; metric 9 planes */
+  int n_neg_jac;                 /* out: ghost points repaired */
+} ostc_block_coords;
+int ostc_mesh_geometry(int n_blocks, int ngg, ostc_block_coords *blocks, const ostc_inner_face *inner,
+                       const int *n_inner);
+
 /* CHEMKIN inputs (Species/Reaction constructors, src/ChemData.cpp:11-614):
  * fills mw, NASA-7 tables, T ranges, Lennard-Jones/viscosity constants, the
  * Wilke tables and (with_reactions != 0) the reaction arrays of *param, and
@@ -261,7 +283,8 @@ int ostc_chem_zone_step(ostc_ctx *ctx, ostc_zone *zone, ostc_surrogate *sur, int
  * (n_out 9): asinh(dY_k/floor).  Inference: BF16 weights and activations,
  * FP32 accumulation (MFMA).  A cell falls back to the FP64 plain step when
  * its input lies outside [t_min,t_max] x [p_min,p_max] x [dt_min,dt_max]
- * (or Y_k < -y_tol), when Y_k+dY_k leaves [-y_tol, 1+y_tol], or when an
+ * (or Y_k < -y_tol), when Y_k+dY_k leaves [-y_tol, 1+y_tol] or a |dY_k|
+ * exceeds dy_max (the output range the network is trusted with), or when an
  * element balance |sum_k elem_w[j][k] dY_k| exceeds
  * elem_tol * sum_jk |elem_w[j][k] dY_k| + elem_abs (elem_w = atoms/W_k).
  * Master weights are FP32, per layer W[out][in] row-major then b[out]. */
@@ -277,6 +300,7 @@ typedef struct ostc_surrogate_desc {
   float floor;
   float t_min, t_max, p_min, p_max, dt_min, dt_max;
   float y_tol, elem_tol, elem_abs;
+  float dy_max;                               /* trusted output range: |dY_k| <= dy_max */
 } ostc_surrogate_desc;
 
 int ostc_surrogate_n_params(int n_out);

@@ -88,6 +88,14 @@ class InnerFace(C.Structure):
 MAX_ELEM = 8
 
 
+class BlockCoords(C.Structure):
+    """struct ostc_block_coords."""
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
+                ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+                ("x_out", C.c_void_p), ("y_out", C.c_void_p), ("z_out", C.c_void_p),
+                ("jac", C.c_void_p), ("metric", C.c_void_p), ("n_neg_jac", C.c_int)]
+
+
 class MechanismInfo(C.Structure):
     _fields_ = [("n_spec", C.c_int), ("n_elem", C.c_int), ("spec_names", (C.c_char * 24) * MAX_SPEC),
                 ("elem_names", (C.c_char * 4) * MAX_ELEM), ("elem_comp", C.c_int * (MAX_SPEC * MAX_ELEM))]
@@ -172,6 +180,8 @@ SIGNATURES = {
     "ostc_memcpy_h2d": (C.c_int, [_P, _P, C.c_long]),
     "ostc_memcpy_d2h": (C.c_int, [_P, _P, C.c_long]),
     "ostc_stream_copy": (C.c_int, [_P, _P, C.c_long, _P]),
+    "ostc_register_inner_face": (C.c_int, [C.POINTER(C.c_int), C.c_int, C.POINTER(InnerFace)]),
+    "ostc_mesh_geometry": (C.c_int, [C.c_int, C.c_int, C.POINTER(BlockCoords), C.POINTER(InnerFace), _I]),
     "ostc_read_chemkin": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(Param),
                                     C.POINTER(MechanismInfo)]),
     "ostc_chemkin_error": (C.c_char_p, []),

@@ -288,6 +288,49 @@ def config1(mx=64, my=64, mz=8, steps=100, precision=L.FP64, seed=1) -> Case:
                 dict(mach=2.0, p=101325.0, T=300.0))
 
 
+def config1_split(mx=64, my=64, mz=8, split=None, steps=100, precision=L.FP64, seed=1) -> Case:
+    """The configuration-1 warped box cut at i = split into two blocks that
+    share the node plane i = split (block 1 = i <= split, block 2 = i >= split),
+    connected by an inner face each way (the reference's inner{rank}.txt
+    records, registered by libostc); inflow on block 1's i = 0, outflow on block
+    2's i = max, symmetry elsewhere; same perturbed Mach 2 field."""
+    ngg = 2
+    split = mx // 2 if split is None else split
+    full = config1(mx, my, mz, steps, precision, seed)
+    gb = full.blocks[0]
+    fs = air_inflow(2.0, 101325.0, 300.0)
+    from .runtime import mesh_geometry, register_inner_face
+    spans = [(0, split), (split, mx - 1)]
+    blocks = []
+    for b, (i0, i1) in enumerate(spans):
+        n = i1 - i0 + 1
+        blk = Block(n, my, mz, gb.x[:, :, i0:i1 + 1].copy(), gb.y[:, :, i0:i1 + 1].copy(), gb.z[:, :, i0:i1 + 1].copy())
+        rows = [(0, n - 1, 0, 0, 0, mz - 1, 3), (0, n - 1, my - 1, my - 1, 0, mz - 1, 3),
+                (0, n - 1, 0, my - 1, 0, 0, 3), (0, n - 1, 0, my - 1, mz - 1, mz - 1, 3)]
+        if b == 0:
+            rows.insert(0, (0, 0, 0, my - 1, 0, mz - 1, 5))
+        else:
+            rows.insert(0, (n - 1, n - 1, 0, my - 1, 0, mz - 1, 6))
+        blk.bc = bc_program(rows, [(5, "inflow")], [fs], [], ngg)
+        blocks.append(blk)
+    n0, n1 = blocks[0].mx, blocks[1].mx
+    raw = [  # i1 i2 j1 j2 k1 k2 | target ranges | target block (1-based)
+        [n0, n0, -1, -my, 1, mz, 1, 1, -1, -my, 1, mz, 2],
+        [1, 1, -1, -my, 1, mz, n0, n0, -1, -my, 1, mz, 1],
+    ]
+    for blk, r in zip(blocks, raw):
+        blk.inner = [register_inner_face(r, ngg)]
+    for blk, geo in zip(blocks, mesh_geometry(blocks, ngg)):
+        blk.xg, blk.yg, blk.zg, blk.jac, blk.metric, _ = geo
+    # the perturbed single-block field, restricted to each block (ghosts too)
+    for blk, (i0, i1) in zip(blocks, spans):
+        lo, hi = i0, i1 + 2 * ngg + 1
+        blk.bv0 = full.blocks[0].bv0[:, :, :, lo:hi].copy()
+        blk.sv0 = full.blocks[0].sv0[:, :, :, lo:hi].copy()
+    return Case("config1_split_2blocks", full.param, blocks, ngg, precision, steps, steps, None, seed,
+                dict(split=split))
+
+
 def _tanh_stretch(n, height, first):
     """y_j = H*(1 - tanh(beta*(1-s))/tanh(beta)), s=j/(n-1), beta such that y_1 = first."""
     s1 = 1.0 / (n - 1)

@@ -204,3 +204,185 @@ extern "C" int ostc_block_geometry(int mx, int my, int mz, int ngg, const double
   }
   return OSTC_OK;
 }
+
+// ---------------------------------------------------------------- multi-block
+
+namespace {
+
+int sgn(int v) { return (v > 0) - (v < 0); }
+
+// copy the neighbour block's coordinates into the ghost layers across one
+// inner face (Mesh::init_inner_ghost_grid, src/Mesh.cpp:940-985); arrays are
+// (ngg+1)-padded
+void inner_ghost(const ostc_inner_face &fc, Grid &B, const Grid &N) {
+  int min_[3] = {fc.range_start[0], fc.range_start[1], fc.range_start[2]};
+  int max_[3] = {fc.range_end[0], fc.range_end[1], fc.range_end[2]};
+  int mint_[3] = {fc.target_start[0], fc.target_start[1], fc.target_start[2]};
+  const int shift = fc.direction < 0 ? -1 : 1;
+  min_[fc.face] += shift;
+  max_[fc.face] += shift;
+  mint_[fc.target_face] += fc.target_direction > 0 ? -1 : 1;
+  const int di1 = fc.loop_dir[0], dj1 = fc.loop_dir[1], dk1 = fc.loop_dir[2];
+  int di[3];
+  for (int i1 = min_[0]; di1 * (i1 - max_[0]) != 1; i1 += di1) {
+    di[0] = i1 - min_[0];
+    for (int j1 = min_[1]; dj1 * (j1 - max_[1]) != 1; j1 += dj1) {
+      di[1] = j1 - min_[1];
+      for (int k1 = min_[2]; dk1 * (k1 - max_[2]) != 1; k1 += dk1) {
+        di[2] = k1 - min_[2];
+        const int i2 = mint_[0] + fc.target_loop_dir[0] * di[fc.src_tar[0]];
+        const int j2 = mint_[1] + fc.target_loop_dir[1] * di[fc.src_tar[1]];
+        const int k2 = mint_[2] + fc.target_loop_dir[2] * di[fc.src_tar[2]];
+        const int64_t d = B.at(i1, j1, k1), s = N.at(i2, j2, k2);
+        B.x[d] = N.x[s];
+        B.y[d] = N.y[s];
+        B.z[d] = N.z[s];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// InnerFace::register_boundary (src/Mesh.cpp:44-178), 3-D: the raw record is
+// the reference's inner{rank}.txt pair of lines (1-based ranges; the range
+// with start == end is the face, a negative pair marks the second direction;
+// the target block id is 1-based).
+extern "C" int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_face *out) {
+  if (!raw || !out || ngg < 1) return OSTC_ERR_ARG;
+  ostc_inner_face f{};
+  for (int i = 0; i < 3; ++i) {
+    f.range_start[i] = raw[2 * i];
+    f.range_end[i] = raw[2 * i + 1];
+    f.target_start[i] = raw[6 + 2 * i];
+    f.target_end[i] = raw[6 + 2 * i + 1];
+  }
+  f.target_block = raw[12] - 1;
+  f.face = -1;
+  f.direction = 1;
+  f.target_face = -1;
+  f.target_direction = 1;
+  for (int i = 0; i < 3; ++i)
+    if (f.range_start[i] == f.range_end[i]) {
+      f.face = i;
+      if (f.range_start[i] == 1) f.direction = -1;
+      f.range_start[i] -= 1;
+      f.range_end[i] -= 1;
+      break;
+    }
+  for (int i = 0; i < 3; ++i)
+    if (f.target_start[i] == f.target_end[i]) {
+      f.target_face = i;
+      if (f.target_start[i] == 1) f.target_direction = -1;
+      f.target_start[i] -= 1;
+      f.target_end[i] -= 1;
+      break;
+    }
+  if (f.face < 0 || f.target_face < 0) return OSTC_ERR_ARG;
+  f.src_tar[f.target_face] = -(f.direction * f.target_direction) * (f.face + 1);
+  int next_face = -1, target_next_face = -1;
+  for (int i = 0; i < 3; ++i)
+    if (f.range_start[i] < 0) {
+      next_face = i;
+      f.range_start[i] = -f.range_start[i] - 1;
+      f.range_end[i] = -f.range_end[i] - 1;
+      break;
+    }
+  for (int i = 0; i < 3; ++i)
+    if (f.target_start[i] < 0) {
+      target_next_face = i;
+      f.target_start[i] = -f.target_start[i] - 1;
+      f.target_end[i] = -f.target_end[i] - 1;
+      break;
+    }
+  if (next_face < 0 || target_next_face < 0) return OSTC_ERR_ARG;
+  f.src_tar[target_next_face] = sgn((f.target_end[target_next_face] - f.target_start[target_next_face]) *
+                                    (f.range_end[next_face] - f.range_start[next_face])) *
+                                (next_face + 1);
+  next_face = 3 - f.face - next_face;
+  target_next_face = 3 - f.target_face - target_next_face;
+  f.range_start[next_face] -= 1;
+  f.range_end[next_face] -= 1;
+  f.target_start[target_next_face] -= 1;
+  f.target_end[target_next_face] -= 1;
+  f.src_tar[target_next_face] = sgn((f.target_end[target_next_face] - f.target_start[target_next_face]) *
+                                    (f.range_end[next_face] - f.range_start[next_face])) *
+                                (next_face + 1);
+  // loop ranges on the ghost side (corners not communicated)
+  f.range_start[f.face] = f.direction < 0 ? -ngg : f.range_end[f.face];
+  f.range_end[f.face] = f.direction < 0 ? 0 : f.range_end[f.face] + ngg;
+  const int sgn_tar = sgn(f.src_tar[f.target_face]);
+  if (f.target_direction > 0)
+    f.target_start[f.target_face] = sgn_tar > 0 ? f.target_end[f.target_face] - ngg : f.target_end[f.target_face];
+  else
+    f.target_start[f.target_face] = sgn_tar > 0 ? 0 : ngg;
+  for (int i = 0; i < 3; ++i) {
+    f.loop_dir[i] = f.range_start[i] <= f.range_end[i] ? 1 : -1;
+    f.target_loop_dir[i] = sgn(f.src_tar[i]);
+    f.src_tar[i] = (f.src_tar[i] < 0 ? -f.src_tar[i] : f.src_tar[i]) - 1;
+  }
+  for (int i = 0; i < 3; ++i) {
+    const int d = f.range_start[i] - f.range_end[i];
+    f.n_point[i] = (d < 0 ? -d : d) + 1;
+  }
+  *out = f;
+  return OSTC_OK;
+}
+
+// Geometry of a multi-block mesh: ghost grids by reflection, then the ghost
+// layers across inner faces copied from the neighbour blocks, then metrics
+// (src/Mesh.cpp:789-943, 940-985, 259-521, in that order).
+extern "C" int ostc_mesh_geometry(int n_blocks, int ngg, ostc_block_coords *blocks, const ostc_inner_face *inner,
+                                  const int *n_inner) {
+  if (n_blocks < 1 || ngg < 1 || !blocks || (!inner && n_inner)) return OSTC_ERR_ARG;
+  const int G = ngg + 1;
+  std::vector<std::vector<double>> X(n_blocks), Y(n_blocks), Z(n_blocks);
+  std::vector<Grid> grid(n_blocks);
+  for (int b = 0; b < n_blocks; ++b) {
+    const ostc_block_coords &c = blocks[b];
+    if (c.mx < 3 || c.my < 3 || c.mz < 3 || !c.x || !c.y || !c.z || !c.jac || !c.metric) return OSTC_ERR_ARG;
+    const int64_t NX = c.mx + 2 * G, NY = c.my + 2 * G, NZ = c.mz + 2 * G;
+    X[b].assign(NX * NY * NZ, 0.0);
+    Y[b].assign(X[b].size(), 0.0);
+    Z[b].assign(X[b].size(), 0.0);
+    grid[b] = Grid{c.mx, c.my, c.mz, G, NX, NY, X[b].data(), Y[b].data(), Z[b].data()};
+    for (int k = 0; k < c.mz; ++k)
+      for (int j = 0; j < c.my; ++j)
+        for (int i = 0; i < c.mx; ++i) {
+          const int64_t s = ((int64_t)k * c.my + j) * c.mx + i, d = grid[b].at(i, j, k);
+          X[b][d] = c.x[s];
+          Y[b][d] = c.y[s];
+          Z[b][d] = c.z[s];
+        }
+    ostc_host::ghost_faces(c.mx, c.my, c.mz, ngg, X[b].data(), Y[b].data(), Z[b].data());
+    ostc_host::ghost_edges_corners(c.mx, c.my, c.mz, ngg, X[b].data(), Y[b].data(), Z[b].data());
+  }
+  if (n_inner) {
+    int off = 0;
+    for (int b = 0; b < n_blocks; ++b) {
+      for (int f = 0; f < n_inner[b]; ++f) {
+        const ostc_inner_face &fc = inner[off + f];
+        if (fc.target_block < 0 || fc.target_block >= n_blocks) return OSTC_ERR_ARG;
+        inner_ghost(fc, grid[b], grid[fc.target_block]);
+      }
+      off += n_inner[b];
+    }
+  }
+  for (int b = 0; b < n_blocks; ++b) {
+    ostc_block_coords &c = blocks[b];
+    if (ostc_host::jac_metric(c.mx, c.my, c.mz, ngg, X[b].data(), Y[b].data(), Z[b].data(), c.jac, c.metric,
+                              &c.n_neg_jac) != 0)
+      return OSTC_ERR_ARG;
+    if (c.x_out && c.y_out && c.z_out)
+      for (int k = -ngg; k < c.mz + ngg; ++k)
+        for (int j = -ngg; j < c.my + ngg; ++j)
+          for (int i = -ngg; i < c.mx + ngg; ++i) {
+            const int64_t d = ((int64_t)(k + ngg) * (c.my + 2 * ngg) + (j + ngg)) * (c.mx + 2 * ngg) + (i + ngg);
+            const int64_t s = grid[b].at(i, j, k);
+            c.x_out[d] = X[b][s];
+            c.y_out[d] = Y[b][s];
+            c.z_out[d] = Z[b][s];
+          }
+  }
+  return OSTC_OK;
+}

@@ -240,7 +240,9 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
   gibbs_div_rt<NS>(t, p, g);
   const double log_t = log(t);
   const double temp_t = kPatm / kRu * 1e-3 / t;
-  double creation[NS], od[NS], zz[NS * NS];
+  // the Jacobian sums accumulate in the caller's array and are scaled in place
+  double creation[NS], od[NS];
+  double *zz = jac_epi;
 #pragma unroll
   for (int l = 0; l < NS; ++l) creation[l] = od[l] = 0;
   if (jac_epi) {
@@ -265,9 +267,10 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
 #pragma unroll
       for (int m = 0; m < NS; ++m) {
         const int dm = nb[m] - nf[m];
-        if (dm != 0)
+        if (dm != 0) {
 #pragma unroll
           for (int n = 0; n < NS; ++n) zz[m * NS + n] += dm * (nf[n] * q1 - nb[n] * q2);
+        }
       }
     }
   }
@@ -279,10 +282,12 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
   }
   if (jac_epi) {
 #pragma unroll
-    for (int m = 0; m < NS; ++m)
+    for (int n = 0; n < NS; ++n) {
+      const bool on = y[n] > 1e-25;
+      const double ry = rho * y[n];
 #pragma unroll
-      for (int n = 0; n < NS; ++n)
-        jac_epi[m * NS + n] = y[n] > 1e-25 ? p->mw[m] * (zz[m * NS + n] / (rho * y[n])) * 1e+3 : 0.0;
+      for (int m = 0; m < NS; ++m) zz[m * NS + n] = on ? p->mw[m] * (zz[m * NS + n] / ry) * 1e+3 : 0.0;
+    }
   }
 }
 

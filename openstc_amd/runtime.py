@@ -108,6 +108,43 @@ def block_geometry(x: np.ndarray, y: np.ndarray, z: np.ndarray, ngg: int = 2):
     return xo, yo, zo, jac, met, nneg.value
 
 
+def register_inner_face(raw, ngg: int = 2) -> L.InnerFace:
+    """One record of the reference's inner{rank}.txt -> registered face
+    (libostc; src/Mesh.cpp:44-178)."""
+    lib = L.load()
+    r = (C.c_int * 13)(*raw)
+    out = L.InnerFace()
+    L.check(lib.ostc_register_inner_face(r, ngg, C.byref(out)), "ostc_register_inner_face")
+    return out
+
+
+def mesh_geometry(blocks, ngg: int = 2):
+    """Ghost grids (incl. across inner faces) + metrics of all blocks on the
+    host (libostc).  blocks: objects with mx, my, mz, x, y, z, inner.
+    Returns [(xg, yg, zg, jac, metric, n_neg)] per block."""
+    lib = L.load()
+    n = len(blocks)
+    keep, outs = [], []
+    arr = (L.BlockCoords * n)()
+    for b, blk in enumerate(blocks):
+        shape = (blk.mz + 2 * ngg, blk.my + 2 * ngg, blk.mx + 2 * ngg)
+        xs, ys, zs = (np.ascontiguousarray(a, dtype=np.float64) for a in (blk.x, blk.y, blk.z))
+        xo, yo, zo, jac = (np.zeros(shape) for _ in range(4))
+        met = np.zeros((9,) + shape)
+        keep += [xs, ys, zs]
+        outs.append([xo, yo, zo, jac, met])
+        c = arr[b]
+        c.mx, c.my, c.mz = blk.mx, blk.my, blk.mz
+        c.x, c.y, c.z = xs.ctypes.data, ys.ctypes.data, zs.ctypes.data
+        c.x_out, c.y_out, c.z_out = xo.ctypes.data, yo.ctypes.data, zo.ctypes.data
+        c.jac, c.metric = jac.ctypes.data, met.ctypes.data
+    faces = [f for blk in blocks for f in blk.inner]
+    inner = (L.InnerFace * max(1, len(faces)))(*faces)
+    n_inner = (C.c_int * n)(*[len(blk.inner) for blk in blocks])
+    L.check(lib.ostc_mesh_geometry(n, ngg, arr, inner, n_inner), "ostc_mesh_geometry")
+    return [tuple(o) + (arr[b].n_neg_jac,) for b, o in enumerate(outs)]
+
+
 class Simulation:
     """One multi-block steady case on the GPU (Driver<mix,turb> equivalent)."""
 

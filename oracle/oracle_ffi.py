@@ -96,7 +96,7 @@ class OrcSurDesc(C.Structure):
                 ("loss_w", C.c_float * 16), ("elem_w", C.c_float * 36),
                 ("floor", C.c_float), ("t_min", C.c_float), ("t_max", C.c_float), ("p_min", C.c_float),
                 ("p_max", C.c_float), ("dt_min", C.c_float), ("dt_max", C.c_float),
-                ("y_tol", C.c_float), ("elem_tol", C.c_float), ("elem_abs", C.c_float)]
+                ("y_tol", C.c_float), ("elem_tol", C.c_float), ("elem_abs", C.c_float), ("dy_max", C.c_float)]
 
 
 def build(quiet=True):
@@ -122,6 +122,8 @@ def load():
             "orc_ghost_grid": (None, [I, I, I, I, _DP, _DP, _DP]),
             "orc_jac_metric": (I, [I, I, I, I, _DP, _DP, _DP, _DP, _DP, C.POINTER(I)]),
             "orc_trim_ghost": (None, [I, I, I, I, _DP, _DP]),
+            "orc_inner_ghost_grid": (None, [C.POINTER(OrcInnerFace), I, I, I, I, _DP, _DP, _DP, I, I, I, _DP, _DP,
+                                            _DP]),
             "orc_enthalpy": (None, [D, _DP, Pp]),
             "orc_enthalpy_and_cp": (None, [D, _DP, _DP, Pp]),
             "orc_cp": (None, [D, _DP, Pp]),
@@ -202,8 +204,8 @@ class OracleBlock:
         for k in ZONE_ARRAYS:
             setattr(self.s, k, dp(self.arr[k]))
 
-    def geometry(self, x, y, z):
-        """Ghost grid + metrics by the oracle's own restatement."""
+    def ghost_grid(self, x, y, z):
+        """Padded (ngg+1) coordinates with the single-block ghost grid."""
         lib = load()
         mx, my, mz, g = self.mx, self.my, self.mz, self.ngg
         G = g + 1
@@ -213,6 +215,14 @@ class OracleBlock:
         Y[G:G + mz, G:G + my, G:G + mx] = y
         Z[G:G + mz, G:G + my, G:G + mx] = z
         lib.orc_ghost_grid(mx, my, mz, g, dp(X), dp(Y), dp(Z))
+        self.big = (X, Y, Z)
+        return self.big
+
+    def metrics(self):
+        """Jacobian/metrics from the padded coordinates; trims the ghosts."""
+        lib = load()
+        mx, my, mz, g = self.mx, self.my, self.mz, self.ngg
+        X, Y, Z = self.big
         nneg = C.c_int()
         r = lib.orc_jac_metric(mx, my, mz, g, dp(X), dp(Y), dp(Z), dp(self.arr["jac"]), dp(self.arr["metric"]),
                                C.byref(nneg))
@@ -221,6 +231,11 @@ class OracleBlock:
         for name, src in (("x", X), ("y", Y), ("z", Z)):
             lib.orc_trim_ghost(mx, my, mz, g, dp(src), dp(self.arr[name]))
         return nneg.value
+
+    def geometry(self, x, y, z):
+        """Ghost grid + metrics by the oracle's own restatement."""
+        self.ghost_grid(x, y, z)
+        return self.metrics()
 
     @property
     def ptr(self):
@@ -241,11 +256,23 @@ class OracleSimulation:
         self.blocks = []
         for blk in case.blocks:
             ob = OracleBlock(blk.mx, blk.my, blk.mz, case.ngg, self.param)
-            ob.geometry(blk.x, blk.y, blk.z)
+            ob.ghost_grid(blk.x, blk.y, blk.z)
+            self.blocks.append(ob)
+        # ghost layers across inner faces from the neighbours (src/Mesh.cpp:940-985)
+        lib = self.lib
+        for b, blk in enumerate(case.blocks):
+            ob = self.blocks[b]
+            for f in blk.inner:
+                t = self.blocks[f.target_block]
+                X, Y, Z = ob.big
+                TX, TY, TZ = t.big
+                lib.orc_inner_ghost_grid(C.byref(copy_struct(OrcInnerFace, f)), ob.mx, ob.my, ob.mz, case.ngg,
+                                         dp(X), dp(Y), dp(Z), t.mx, t.my, t.mz, dp(TX), dp(TY), dp(TZ))
+        for blk, ob in zip(case.blocks, self.blocks):
+            ob.metrics()
             ob.arr["bv"][:] = blk.bv0
             if case.param.n_scalar:
                 ob.arr["sv"][:case.param.n_scalar] = blk.sv0
-            self.blocks.append(ob)
         self.bcs = (OrcBC * len(case.blocks))(*[oracle_bc(b.bc) for b in case.blocks])
         self.zones = (OrcZone * len(self.blocks))(*[b.s for b in self.blocks])
         n_inner = [len(blk.inner) for blk in case.blocks]

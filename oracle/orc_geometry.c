@@ -215,3 +215,47 @@ void orc_trim_ghost(int mx, int my, int mz, int ngg, const double *src, double *
     for (int j = -ngg; j < my + ngg; ++j)
       for (int i = -ngg; i < mx + ngg; ++i) dst[gidx(mx, my, ngg, i, j, k)] = src[gidx(mx, my, ngg + 1, i, j, k)];
 }
+
+/* Mesh::init_inner_ghost_grid for one inner face (src/Mesh.cpp:940-985):
+ * ghost coordinates of this block across the face copied from the target
+ * block's points; both arrays padded by ngg+1. */
+void orc_inner_ghost_grid(const orc_inner_face *fc, int mx, int my, int mz, int ngg, double *x, double *y, double *z,
+                          int tmx, int tmy, int tmz, const double *tx, const double *ty, const double *tz) {
+  const int G = ngg + 1;
+  (void)mz;
+  (void)tmz;
+  int lo[3], hi[3], t0[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = fc->range_start[d];
+    hi[d] = fc->range_end[d];
+    t0[d] = fc->target_start[d];
+  }
+  if (fc->direction < 0) {
+    lo[fc->face] -= 1;
+    hi[fc->face] -= 1;
+  } else {
+    lo[fc->face] += 1;
+    hi[fc->face] += 1;
+  }
+  t0[fc->target_face] += fc->target_direction > 0 ? -1 : 1;
+  int off[3];
+  for (int i = lo[0];; i += fc->loop_dir[0]) {
+    off[0] = i - lo[0];
+    for (int j = lo[1];; j += fc->loop_dir[1]) {
+      off[1] = j - lo[1];
+      for (int k = lo[2];; k += fc->loop_dir[2]) {
+        off[2] = k - lo[2];
+        const int ti = t0[0] + fc->target_loop_dir[0] * off[fc->src_tar[0]];
+        const int tj = t0[1] + fc->target_loop_dir[1] * off[fc->src_tar[1]];
+        const int tk = t0[2] + fc->target_loop_dir[2] * off[fc->src_tar[2]];
+        const long d = gidx(mx, my, G, i, j, k), s = gidx(tmx, tmy, G, ti, tj, tk);
+        x[d] = tx[s];
+        y[d] = ty[s];
+        z[d] = tz[s];
+        if (k == hi[2]) break;
+      }
+      if (j == hi[1]) break;
+    }
+    if (i == hi[0]) break;
+  }
+}

@@ -158,7 +158,7 @@ int orc_sur_decode_check(const orc_sur_desc *d, const float *o, double t, double
   if (!in) return 1;
   for (int k = 0; k < 9; ++k) {
     const double yn = y[k] + dy[k];
-    if (!(yn >= -d->y_tol && yn <= 1.0 + d->y_tol)) return 2;
+    if (!(yn >= -d->y_tol && yn <= 1.0 + d->y_tol && fabs(dy[k]) <= d->dy_max)) return 2;
   }
   double tot = 0, worst = 0;
   for (int j = 0; j < d->n_elem; ++j) {

@@ -114,6 +114,10 @@ void orc_ghost_grid(int mx, int my, int mz, int ngg, double *x, double *y, doubl
  * points repaired from their inner neighbour. */
 int orc_jac_metric(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
                    double *jac, double *metric, int *n_neg);
+/* Ghost coordinates across one inner face copied from the target block
+ * (src/Mesh.cpp:940-985); both arrays (ngg+1)-padded. */
+void orc_inner_ghost_grid(const orc_inner_face *fc, int mx, int my, int mz, int ngg, double *x, double *y, double *z,
+                          int tmx, int tmy, int tmz, const double *tx, const double *ty, const double *tz);
 /* Copy the (ngg+1)-padded coordinates into ngg-padded arrays. */
 void orc_trim_ghost(int mx, int my, int mz, int ngg, const double *src, double *dst);
 
@@ -188,6 +192,7 @@ typedef struct orc_sur_desc {
   float floor;
   float t_min, t_max, p_min, p_max, dt_min, dt_max;
   float y_tol, elem_tol, elem_abs;
+  float dy_max;
 } orc_sur_desc;
 
 int orc_sur_n_params(int n_out);
