@@ -2,6 +2,7 @@
 // per-stage entry points declared in include/ostc.h.
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -231,6 +232,12 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   o.turb_implicit = param->turb_implicit;
   o.implicit_method = param->implicit_method;
   o.dplur_steps = param->dplur_inner_step;
+  // launch tuning of the DPLUR sweep: tile height (4 or 8 rows) and the number
+  // of workgroups the k chunks are sized for (environment overrides)
+  o.dplur_tj = 4;
+  o.dplur_wg = 1024;
+  if (const char *e = std::getenv("OSTC_DPLUR_TJ")) o.dplur_tj = std::atoi(e) == 4 ? 4 : 8;
+  if (const char *e = std::getenv("OSTC_DPLUR_WG")) o.dplur_wg = std::max(64, std::atoi(e));
   *out = c;
   return OSTC_OK;
 }

@@ -513,14 +513,14 @@ __device__ __forceinline__ void jd_dir(const ZoneView<R> &z, int64_t q, int dir,
 // previous plane.  Tile-edge neighbours outside the tile are evaluated by
 // the edge lanes/rows themselves.  Off-diagonal sums are accumulated in the
 // reference's order (i-, j-, k-, i+, j+, k+).
-template <typename R, int NS, int NT, bool CHEM>
-__global__ __launch_bounds__(256) void k_dplur(ZoneView<R> z, const ostc_param *__restrict__ p,
+template <typename R, int NS, int NT, bool CHEM, int TJ>
+__global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_param *__restrict__ p,
                                                const R *__restrict__ din, R *__restrict__ dout, int kc) {
   constexpr int NV = 5 + NS + NT;
-  __shared__ R sj[2][4][NV][64];  // [plus/minus][row][var][lane]
+  __shared__ R sj[2][TJ][NV][64];  // [plus/minus][row][var][lane]
   const int tx = threadIdx.x, ty = threadIdx.y;
   const uint3 bk = xcd_block();
-  const int i = (int)bk.x * 64 + tx, j = (int)bk.y * 4 + ty;
+  const int i = (int)bk.x * 64 + tx, j = (int)bk.y * TJ + ty;
   const int k0 = (int)bk.z * kc, k1 = min(k0 + kc, z.mz);
   const bool ok = i < z.mx && j < z.my;
   const int ii = min(i, z.mx - 1), jj = min(j, z.my - 1);  // clamped (loads of idle lanes stay in bounds)
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(256) void k_dplur(ZoneView<R> z, const ostc_param *
       jd_load<R, NS, NT>(z, p, din, q, e);
       jd_dir<R, NS, NT>(z, q, 1, e, sr[P + q], jp, nullptr);
     }
-    if (ty == 3 && j + 1 < z.my) {
+    if (ty == TJ - 1 && j + 1 < z.my) {
       const int64_t q = z.idx(ii, j + 1, k);
       JdCell<R, NS, NT> e;
       jd_load<R, NS, NT>(z, p, din, q, e);
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(256) void k_dplur(ZoneView<R> z, const ostc_param *
     if (ty > 0)
 #pragma unroll
       for (int l = 0; l < NV; ++l) jp[l] = sj[0][ty - 1][l][tx];
-    if (ty < 3)
+    if (ty < TJ - 1)
 #pragma unroll
       for (int l = 0; l < NV; ++l) jm[l] = sj[1][ty + 1][l][tx];
     const bool hi = i > 0, hj = j > 0, hk = k > 0;
@@ -1097,22 +1097,32 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
   }
   prof_mark(P_DQ0, st, false);
   if (last_err()) return OSTC_ERR_HIP;
-  // k-marching sweeps: (64 x 4) tiles, k split into chunks so that about
-  // 1024 workgroups (4 per CU) are in flight; at least 4 planes per chunk
-  const int tiles = ((z.mx + 63) / 64) * ((z.my + 3) / 4);
-  const int nkc = std::max(1, std::min((1024 + tiles - 1) / tiles, (z.mz + 3) / 4));
+  // k-marching sweeps: (64 x TJ) tiles, k split into chunks so that about
+  // `target` workgroups are in flight; at least 4 planes per chunk
+  const int tj = o.dplur_tj == 8 ? 8 : 4;
+  const int target = o.dplur_wg > 0 ? o.dplur_wg : 1024;
+  const int tiles = ((z.mx + 63) / 64) * ((z.my + tj - 1) / tj);
+  const int nkc = std::max(1, std::min((target + tiles - 1) / tiles, (z.mz + 3) / 4));
   const int kc = (z.mz + nkc - 1) / nkc;
-  const dim3 sgrid((unsigned)((z.mx + 63) / 64), (unsigned)((z.my + 3) / 4), (unsigned)((z.mz + kc - 1) / kc));
-  const dim3 sblock(64, 4, 1);
+  const dim3 sgrid((unsigned)((z.mx + 63) / 64), (unsigned)((z.my + tj - 1) / tj), (unsigned)((z.mz + kc - 1) / kc));
+  const dim3 sblock(64, tj, 1);
   R *in = z.dq0, *out = z.dqk;
   for (int s = 0; s < o.dplur_steps; ++s) {
     prof_mark(P_DPLUR, st, true);
     if (chem) {
-      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true>), sgrid, sblock, 0, st, z, dp,
-                                               (const R *)in, out, kc));
+      if (tj == 4)
+        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true, 4>), sgrid, sblock, 0, st, z, dp,
+                                                 (const R *)in, out, kc));
+      else
+        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true, 8>), sgrid, sblock, 0, st, z, dp,
+                                                 (const R *)in, out, kc));
     } else {
-      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false>), sgrid, sblock, 0, st, z, dp,
-                                               (const R *)in, out, kc));
+      if (tj == 4)
+        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false, 4>), sgrid, sblock, 0, st, z, dp,
+                                                 (const R *)in, out, kc));
+      else
+        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false, 8>), sgrid, sblock, 0, st, z, dp,
+                                                 (const R *)in, out, kc));
     }
     prof_mark(P_DPLUR, st, false);
     if (last_err()) return OSTC_ERR_HIP;

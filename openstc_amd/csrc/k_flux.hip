@@ -76,7 +76,7 @@ __device__ __forceinline__ void inviscid_face(const ZoneView<R> &z, const ostc_p
       mwl += pl[5 + l] * inv;
       mwr += pr[5 + l] * inv;
     }
-    const R tl = pl[4] / (pl[0] * (R)kRu * mwl), tr = pr[4] / (pr[0] * (R)kRu * mwr);
+    const R tl = fdiv<R>(pl[4], pl[0] * (R)kRu * mwl), tr = fdiv<R>(pr[4], pr[0] * (R)kRu * mwr);
     R hl[NS], hr[NS], cpl[NS], cpr[NS];
     enthalpy_cp<R>(tl, hl, cpl, p, NS);
     enthalpy_cp<R>(tr, hr, cpr, p, NS);
@@ -93,11 +93,11 @@ __device__ __forceinline__ void inviscid_face(const ZoneView<R> &z, const ostc_p
     }
     El = pl[0] * el - pl[4];
     Er = pr[0] * er - pr[4];
-    gam_l = cpL / cvL;
-    gam_r = cpR / cvR;
+    gam_l = fdiv<R>(cpL, cvL);
+    gam_r = fdiv<R>(cpR, cvR);
   } else {
-    El = el * pl[0] + pl[4] / (R)(kGammaAir - 1);
-    Er = er * pr[0] + pr[4] / (R)(kGammaAir - 1);
+    El = el * pl[0] + fdiv<R>(pl[4], (R)(kGammaAir - 1));
+    Er = er * pr[0] + fdiv<R>(pr[4], (R)(kGammaAir - 1));
   }
   const int64_t idr = idl + st;
   const R jl = z.jac[idl], jr = z.jac[idr];
@@ -106,15 +106,17 @@ __device__ __forceinline__ void inviscid_face(const ZoneView<R> &z, const ostc_p
   const R k2 = (R)0.5 * (jl * z.metric[mo + z.plane + idl] + jr * z.metric[mo + z.plane + idr]);
   const R k3 = (R)0.5 * (jl * z.metric[mo + 2 * z.plane + idl] + jr * z.metric[mo + 2 * z.plane + idr]);
   const R gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
-  const R ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / gk;
-  const R ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / gk;
-  const R c = (R)0.5 * (sqrt(gam_l * pl[4] / pl[0]) + sqrt(gam_r * pr[4] / pr[0]));
-  const R mach_l = ul / c, mach_r = ur / c;
+  const R igk = fdiv<R>((R)1, gk);
+  const R ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) * igk;
+  const R ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) * igk;
+  const R c = (R)0.5 * (sqrt(fdiv<R>(gam_l * pl[4], pl[0])) + sqrt(fdiv<R>(gam_r * pr[4], pr[0])));
+  const R ic = fdiv<R>((R)1, c);
+  const R mach_l = ul * ic, mach_r = ur * ic;
   R mlp, mrn, plp, prn;
   const R alpha = (R)(3 / 16.0);
   if (fabs(mach_l) > (R)1) {
     mlp = (R)0.5 * (mach_l + fabs(mach_l));
-    plp = mlp / mach_l;
+    plp = fdiv<R>(mlp, mach_l);
   } else {
     const R a = (mach_l + (R)1) * (mach_l + (R)1) * (R)0.25;
     const R b = (mach_l * mach_l - (R)1) * (mach_l * mach_l - (R)1);
@@ -123,7 +125,7 @@ __device__ __forceinline__ void inviscid_face(const ZoneView<R> &z, const ostc_p
   }
   if (fabs(mach_r) > (R)1) {
     mrn = (R)0.5 * (mach_r - fabs(mach_r));
-    prn = mrn / mach_r;
+    prn = fdiv<R>(mrn, mach_r);
   } else {
     const R a = (mach_r - (R)1) * (mach_r - (R)1) * (R)0.25;
     const R b = (mach_r * mach_r - (R)1) * (mach_r * mach_r - (R)1);
@@ -141,7 +143,7 @@ __device__ __forceinline__ void inviscid_face(const ZoneView<R> &z, const ostc_p
   fc[1] = coeff * (up ? pl[1] : pr[1]) + p_coeff * k1;
   fc[2] = coeff * (up ? pl[2] : pr[2]) + p_coeff * k2;
   fc[3] = coeff * (up ? pl[3] : pr[3]) + p_coeff * k3;
-  fc[4] = up ? coeff * (El + pl[4]) / pl[0] : coeff * (Er + pr[4]) / pr[0];
+  fc[4] = up ? fdiv<R>(coeff * (El + pl[4]), pl[0]) : fdiv<R>(coeff * (Er + pr[4]), pr[0]);
 #pragma unroll
   for (int l = 5; l < NV; ++l) fc[l] = coeff * (up ? pl[l] : pr[l]);
 }
@@ -192,9 +194,9 @@ __device__ __forceinline__ void viscous_face(const ZoneView<R> &z, const ostc_pa
   R mut = 0;
   if constexpr (NT > 0) mut = (R)0.5 * (z.mut[id0] + z.mut[id1]);
   const R vis = mul + mut;
-  R tau_xx = vis * ((R)4 * u_x - (R)2 * v_y - (R)2 * w_z) / (R)3;
-  R tau_yy = vis * ((R)4 * v_y - (R)2 * u_x - (R)2 * w_z) / (R)3;
-  R tau_zz = vis * ((R)4 * w_z - (R)2 * u_x - (R)2 * v_y) / (R)3;
+  R tau_xx = fdiv<R>(vis * ((R)4 * u_x - (R)2 * v_y - (R)2 * w_z), (R)3);
+  R tau_yy = fdiv<R>(vis * ((R)4 * v_y - (R)2 * u_x - (R)2 * w_z), (R)3);
+  R tau_zz = fdiv<R>(vis * ((R)4 * w_z - (R)2 * u_x - (R)2 * v_y), (R)3);
   const R tau_xy = vis * (u_y + v_x), tau_xz = vis * (u_z + w_x), tau_yz = vis * (v_z + w_y);
   const R *svp = z.sv;
   if constexpr (NT > 0) {
@@ -224,7 +226,7 @@ __device__ __forceinline__ void viscous_face(const ZoneView<R> &z, const ostc_pa
   fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + cond * (kx * t_x + ky * t_y + kz * t_z);
   if constexpr (NS > 0) {
     R turb_d = 0;
-    if constexpr (NT > 0) turb_d = mut / (R)p->Sct;
+    if constexpr (NT > 0) turb_d = fdiv<R>(mut, (R)p->Sct);
     R h[NS], dif[NS], yx[NS], yy[NS], yz[NS];
     const R tm = (R)0.5 * (z.bv[5 * P + id0] + z.bv[5 * P + id1]);
     enthalpy<R>(tm, h, p, NS);
@@ -265,12 +267,12 @@ __device__ __forceinline__ void viscous_face(const ZoneView<R> &z, const ostc_pa
     if (wd > (R)1e-25) {
       const R km = (R)0.5 * (svp[NS * P + id0] + svp[NS * P + id1]);
       const R om = (R)0.5 * (svp[(NS + 1) * P + id0] + svp[(NS + 1) * P + id1]);
-      const R param1 = sqrt(km) / ((R)0.09 * om * wd);
+      const R param1 = fdiv<R>(sqrt(km), (R)0.09 * om * wd);
       const R rhom = (R)0.5 * (z.bv[id0] + z.bv[id1]);
       const R d2 = wd * wd;
-      const R param2 = (R)500 * mul / (rhom * d2 * om);
-      const R cd = dmax((R)1e-20, (R)2 * rhom * (R)sst::sigma_omega2 / om * (k_x * o_x + k_y * o_y + k_z * o_z));
-      const R param3 = (R)4 * rhom * (R)sst::sigma_omega2 * km / (cd * d2);
+      const R param2 = fdiv<R>((R)500 * mul, rhom * d2 * om);
+      const R cd = dmax((R)1e-20, fdiv<R>((R)2 * rhom * (R)sst::sigma_omega2, om) * (k_x * o_x + k_y * o_y + k_z * o_z));
+      const R param3 = fdiv<R>((R)4 * rhom * (R)sst::sigma_omega2 * km, cd * d2);
       const R arg1 = dmin(dmax(param1, param2), param3);
       f1 = tanh(arg1 * arg1 * arg1 * arg1);
     }

@@ -86,6 +86,18 @@ __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
 // Block-id remapping so that each XCD (blocks are dealt round-robin to the 8
 // XCDs) works on one contiguous slab of the domain: stencil halos between
 // neighbouring tiles then hit the same L2.
+// a/b: IEEE division in FP64; in the FP32 bulk path a times the hardware
+// reciprocal (v_rcp_f32, 1 ulp), which replaces the ~10-instruction
+// correctly rounded division sequence in the flux kernels.
+template <typename R>
+__device__ __forceinline__ R fdiv(R a, R b) {
+  return a / b;
+}
+template <>
+__device__ __forceinline__ float fdiv<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+
 __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
   constexpr unsigned kXcd = 8;
   const unsigned per = nblocks / kXcd;

@@ -35,6 +35,7 @@ struct CellOpts {
   int mix_model;   // OSTC_AIR / OSTC_MIXTURE / OSTC_FR
   int chem;        // chem_src_method
   int rans_model, turb_implicit, implicit_method, dplur_steps;
+  int dplur_tj, dplur_wg;  // DPLUR sweep tile height (4 or 8) and workgroup target (launch tuning)
 };
 template <typename R>
 int cv_from_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
