@@ -129,6 +129,8 @@ def load():
             "orc_cp": (None, [D, _DP, Pp]),
             "orc_gibbs_div_rt": (None, [D, Pp, _DP]),
             "orc_sutherland": (D, [D]),
+            "orc_transport_point": (None, [Pp, D, D, _DP, _DP, _DP, _DP, _DP]),
+            "orc_mixture_viscosity": (D, [Pp, D, D, _DP]),
             "orc_compute_cv_from_bv": (None, [Z, Pp]),
             "orc_update_physical_properties": (None, [Z, Pp]),
             "orc_initialize_mut": (None, [Z, Pp]),
@@ -139,6 +141,7 @@ def load():
             "orc_viscous_flux": (None, [Z, Pp]),
             "orc_local_time_step": (None, [Z, Pp]),
             "orc_implicit_treatment": (None, [Z, Pp]),
+            "orc_jac_times_dq_point": (None, [Z, Pp, I, I, I, I, D, _DP]),
             "orc_update_cv_and_bv": (I, [Z, Pp]),
             "orc_limit_flow": (I, [Z, Pp]),
             "orc_apply_bc": (None, [Z, Pp, C.POINTER(OrcBC)]),

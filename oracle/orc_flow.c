@@ -1412,3 +1412,10 @@ int orc_omp_threads(void) {
   return 1;
 #endif
 }
+
+/* Test access to the DPLUR convective-Jacobian product of one cell (the
+ * product A_dir(Q)*dq + pm_sr*dq with dq read from z->dq). */
+void orc_jac_times_dq_point(const orc_zone *z, const orc_param *p, int i, int j, int k, int dir, double pm_sr,
+                            double *out) {
+  jac_times_dq(z, p, i, j, k, dir, pm_sr, out);
+}

@@ -41,9 +41,6 @@ static inline long IX(const orc_zone *z, int i, int j, int k) {
 static inline double dmax(double a, double b) { return a > b ? a : b; }
 static inline double dmin(double a, double b) { return a < b ? a : b; }
 
-void orc_transport_point(const orc_param *p, double t, double mw_total, const double *y, const double *cp,
-                         double *mu_out, double *lambda_out, double *rhoD);
-double orc_mixture_viscosity(const orc_param *p, double t, double mw_total, const double *y);
 void orc_finite_rate_chemistry(orc_zone *z, int i, int j, int k, const orc_param *p);
 void orc_epi_solve_dq(orc_zone *z, int i, int j, int k, double diag_or_one, const orc_param *p);
 void orc_epi_dqk(orc_zone *z, int i, int j, int k, double diag, const double *dq_total, const orc_param *p);

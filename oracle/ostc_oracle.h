@@ -127,6 +127,11 @@ void orc_enthalpy_and_cp(double t, double *h, double *cp, const orc_param *p);
 void orc_cp(double t, double *cp, const orc_param *p);
 void orc_gibbs_div_rt(double t, const orc_param *p, double *g);
 double orc_sutherland(double t);
+/* Mixture viscosity, conductivity and rho*D per species at one point (cp per
+ * species in J/(kg K)); viscosity alone (device form of the same rule). */
+void orc_transport_point(const orc_param *p, double t, double mw_total, const double *y, const double *cp,
+                         double *mu_out, double *lambda_out, double *rhoD);
+double orc_mixture_viscosity(const orc_param *p, double t, double mw_total, const double *y);
 
 /* ---- per-stage kernels (one call = the whole block) ---- */
 void orc_compute_cv_from_bv(orc_zone *z, const orc_param *p);
@@ -141,6 +146,10 @@ void orc_ausm_face(const orc_param *p, const double *pv, const double *ml, const
 void orc_viscous_flux(orc_zone *z, const orc_param *p);
 void orc_local_time_step(orc_zone *z, const orc_param *p);
 void orc_implicit_treatment(orc_zone *z, const orc_param *p);
+/* DPLUR convective-Jacobian product of cell (i,j,k) in direction dir
+ * (src/DPLURHPP.cuh:78-125) with dq read from z->dq (tests) */
+void orc_jac_times_dq_point(const orc_zone *z, const orc_param *p, int i, int j, int k, int dir, double pm_sr,
+                            double *out);
 int orc_update_cv_and_bv(orc_zone *z, const orc_param *p);
 int orc_limit_flow(orc_zone *z, const orc_param *p);
 void orc_apply_bc(orc_zone *z, const orc_param *p, const orc_bc *bc);
