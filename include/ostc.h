@@ -192,6 +192,66 @@ int ostc_read_chemkin(const char *mechanism, const char *thermo, const char *tra
                       ostc_param *param, ostc_mechanism_info *info);
 const char *ostc_chemkin_error(void);
 
+/* ---- case directory (host, CPU) ----
+ * A directory laid out as the reference's working directory: input_files/
+ * setup/{0_global_control,...,9_transport_property}.txt, input_files/grid/
+ * "grid   0.grd", input_files/boundary_condition/"boundary   0.txt" and
+ * "inner   0.txt", and the CHEMKIN files the setup names.  ostc_case_read
+ * stands in for the reference's Parameter (src/Parameter.cpp:8-168),
+ * Species/Reaction (src/ChemData.cpp:11-614), Mesh (src/Mesh.cpp:546-735),
+ * DParameter's limits (src/DParameter.cu:78-100), the boundary programs of
+ * DBoundCond::initialize_bc_on_GPU (src/BoundCond.cu:73-232) with the
+ * inflow states of Inflow::Inflow (src/BoundCond.cpp:6-106), and
+ * initialize_from_start (src/Initialize.cu:4-43, src/Field.cu:21-67).
+ * Unsupported settings (2-D blocks, parallel faces, restart, unsteady, WENO,
+ * SA) are rejected with OSTC_ERR_ARG and a message (ostc_case_error). */
+#define OSTC_MAX_BLOCK 16
+typedef struct ostc_case ostc_case;
+typedef struct ostc_case_info {
+  int n_block, ngg;
+  int total_step, output_screen, output_file, n_wall_point;
+  double convergence_criteria, grid_scale;
+  int mx[OSTC_MAX_BLOCK], my[OSTC_MAX_BLOCK], mz[OSTC_MAX_BLOCK];
+  int n_inner[OSTC_MAX_BLOCK], n_neg_jac[OSTC_MAX_BLOCK];
+} ostc_case_info;
+int ostc_case_read(const char *case_dir, ostc_case **out);
+int ostc_case_destroy(ostc_case *c);
+int ostc_case_get_info(const ostc_case *c, ostc_case_info *info);
+int ostc_case_get_param(const ostc_case *c, ostc_param *out);
+int ostc_case_get_mechanism(const ostc_case *c, ostc_mechanism_info *out);
+int ostc_case_get_bc(const ostc_case *c, int block, ostc_bc *out);
+/* registered inner faces of one block; *n_faces = their number */
+int ostc_case_get_inner(const ostc_case *c, int block, ostc_inner_face *out, int max_faces, int *n_faces);
+/* ngg-padded fields of a block in the zone layout: X, Y, Z, JAC, METRIC
+ * (read) and the initial BV, SV (read, or overwrite when write != 0 -- the
+ * way to start from fields the setup files cannot express). */
+int ostc_case_field(ostc_case *c, int block, int field, double *host, int n_planes, int write);
+/* wall points (x,y,z per point) used for the wall distance (src/Driver.cu:
+ * 128-149); the count is ostc_case_info.n_wall_point */
+int ostc_case_wall_points(const ostc_case *c, double *xyz, int max_points);
+const char *ostc_case_error(void);
+
+/* ---- time-loop driver: Driver<mix,turb> (src/Driver.cu:14-289) ----
+ * create = the constructor (one zone per block, geometry, initial fields and
+ * boundary programs uploaded; precision OSTC_FP32 or OSTC_FP64); initialize =
+ * initialize_computation (src/Driver.cu:43-104) incl. the wall distance;
+ * step = one iteration of steady_simulation with compute_residual on output
+ * steps (res[4] normalised by the step-1 values, err_max their max,
+ * src/Driver.cu:291-380); simulate = steady_simulation up to total_step or
+ * convergence, with io != 0 printing the reference's screen lines and
+ * writing history.dat and output/message/residual_scale.txt under the case
+ * directory.  The driver keeps no pointer to the case. */
+typedef struct ostc_driver ostc_driver;
+int ostc_driver_create(const ostc_case *c, int precision, ostc_driver **out);
+int ostc_driver_initialize(ostc_driver *d);
+int ostc_driver_step(ostc_driver *d, double res[4], int *res_computed, double *err_max);
+int ostc_driver_simulate(ostc_driver *d, int io, int *steps_done, double *err_max);
+int ostc_driver_get_zone(ostc_driver *d, int block, ostc_zone **out);
+int ostc_driver_get_ctx(ostc_driver *d, ostc_ctx **out);
+int ostc_driver_get_step(const ostc_driver *d, int *step);
+int ostc_driver_destroy(ostc_driver *d);
+const char *ostc_driver_error(void);
+
 /* ---- setup stages ---- */
 /* compute_cv_from_bv<M,T> (src/FieldOperation.cuh:37-73) */
 int ostc_compute_cv_from_bv(ostc_ctx *ctx, ostc_zone *zone, void *stream);

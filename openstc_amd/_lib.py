@@ -115,6 +115,17 @@ class SurDesc(C.Structure):
                 ("y_tol", C.c_float), ("elem_tol", C.c_float), ("elem_abs", C.c_float), ("dy_max", C.c_float)]
 
 
+MAX_BLOCK = 16
+
+
+class CaseInfo(C.Structure):
+    """struct ostc_case_info."""
+    _fields_ = [("n_block", C.c_int), ("ngg", C.c_int), ("total_step", C.c_int), ("output_screen", C.c_int),
+                ("output_file", C.c_int), ("n_wall_point", C.c_int), ("convergence_criteria", C.c_double),
+                ("grid_scale", C.c_double), ("mx", C.c_int * MAX_BLOCK), ("my", C.c_int * MAX_BLOCK),
+                ("mz", C.c_int * MAX_BLOCK), ("n_inner", C.c_int * MAX_BLOCK), ("n_neg_jac", C.c_int * MAX_BLOCK)]
+
+
 class ZoneDesc(C.Structure):
     _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("ngg", C.c_int),
                 ("precision", C.c_int), ("block_id", C.c_int)]
@@ -185,6 +196,25 @@ SIGNATURES = {
     "ostc_read_chemkin": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(Param),
                                     C.POINTER(MechanismInfo)]),
     "ostc_chemkin_error": (C.c_char_p, []),
+    "ostc_case_read": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "ostc_case_destroy": (C.c_int, [_P]),
+    "ostc_case_get_info": (C.c_int, [_P, C.POINTER(CaseInfo)]),
+    "ostc_case_get_param": (C.c_int, [_P, C.POINTER(Param)]),
+    "ostc_case_get_mechanism": (C.c_int, [_P, C.POINTER(MechanismInfo)]),
+    "ostc_case_get_bc": (C.c_int, [_P, C.c_int, C.POINTER(BC)]),
+    "ostc_case_get_inner": (C.c_int, [_P, C.c_int, C.POINTER(InnerFace), C.c_int, _I]),
+    "ostc_case_field": (C.c_int, [_P, C.c_int, C.c_int, _D, C.c_int, C.c_int]),
+    "ostc_case_wall_points": (C.c_int, [_P, _D, C.c_int]),
+    "ostc_case_error": (C.c_char_p, []),
+    "ostc_driver_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "ostc_driver_initialize": (C.c_int, [_P]),
+    "ostc_driver_step": (C.c_int, [_P, _D, _I, _D]),
+    "ostc_driver_simulate": (C.c_int, [_P, C.c_int, _I, _D]),
+    "ostc_driver_get_zone": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "ostc_driver_get_ctx": (C.c_int, [_P, C.POINTER(_P)]),
+    "ostc_driver_get_step": (C.c_int, [_P, _I]),
+    "ostc_driver_destroy": (C.c_int, [_P]),
+    "ostc_driver_error": (C.c_char_p, []),
     "ostc_device_sync": (C.c_int, []),
     "ostc_last_error": (C.c_char_p, []),
     "ostc_build_info": (C.c_char_p, []),

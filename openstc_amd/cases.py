@@ -90,6 +90,8 @@ class Block:
     metric: np.ndarray = None
     bv0: np.ndarray = None
     sv0: np.ndarray = None
+    rows: list = field(default_factory=list)       # boundary-file records (i1 i2 j1 j2 k1 k2 label, 0-based)
+    inner_raw: list = field(default_factory=list)  # inner-file records (1-based ranges + target block)
 
 
 @dataclass
@@ -104,6 +106,20 @@ class Case:
     wall_points: np.ndarray = None
     seed: int = 0
     meta: dict = field(default_factory=dict)
+
+
+def setup_spec(steps, output_screen, cfl, implicit, dplur_steps, reconstruction, viscous, rans=False,
+               species=False, reaction=False, chem_method=0, bcs=(), default_init="freestream",
+               convergence=1e-30):
+    """The case in the terms of the reference's setup files (written by
+    case_files.write_case).  bcs: (struct name, {type, label, values...}) in
+    the order of the boundary_conditions array.  The convergence criterion
+    defaults to 1e-30 so that runs take their full step count."""
+    return dict(total_step=steps, output_file=steps, output_screen=output_screen, cfl=cfl,
+                implicit_method=1 if implicit else 0, DPLUR_inner_step=dplur_steps, reconstruction=reconstruction,
+                viscous_order=2 if viscous else 0, turbulence=rans, species=species, reaction=reaction,
+                chemSrcMethod=chem_method, bcs=list(bcs), default_init=default_init,
+                convergence_criteria=convergence)
 
 
 def register_boundary(row, ngg=2):
@@ -274,6 +290,7 @@ def config1(mx=64, my=64, mz=8, steps=100, precision=L.FP64, seed=1) -> Case:
             (0, mx - 1, 0, 0, 0, mz - 1, 3), (0, mx - 1, my - 1, my - 1, 0, mz - 1, 3),
             (0, mx - 1, 0, my - 1, 0, 0, 3), (0, mx - 1, 0, my - 1, mz - 1, mz - 1, 3)]
     blk.bc = bc_program(rows, [(5, "inflow")], [fs], [], ngg)
+    blk.rows = rows
     finish_block(blk, ngg)
     bv, sv = uniform_field(blk, fs, 0, ngg)
     perturb_interior(bv[0], blk, seed, 0, ngg, rel=0.01)
@@ -284,8 +301,11 @@ def config1(mx=64, my=64, mz=8, steps=100, precision=L.FP64, seed=1) -> Case:
     # T from the equation of state of the perturbed state
     bv[5] = bv[4] * MW_AIR / (bv[0] * RU)
     blk.bv0, blk.sv0 = bv, sv
+    setup = setup_spec(steps, steps, 0.5, False, 0, 2, False,
+                       bcs=[("freestream", dict(type="inflow", label=5, mach=2.0, pressure=101325.0,
+                                                temperature=300.0, u=1.0, v=0.0, w=0.0))])
     return Case("config1_air_inviscid_explicit", param, [blk], ngg, precision, steps, steps, None, seed,
-                dict(mach=2.0, p=101325.0, T=300.0))
+                dict(mach=2.0, p=101325.0, T=300.0, setup=setup))
 
 
 def config1_split(mx=64, my=64, mz=8, split=None, steps=100, precision=L.FP64, seed=1) -> Case:
@@ -312,6 +332,7 @@ def config1_split(mx=64, my=64, mz=8, split=None, steps=100, precision=L.FP64, s
         else:
             rows.insert(0, (n - 1, n - 1, 0, my - 1, 0, mz - 1, 6))
         blk.bc = bc_program(rows, [(5, "inflow")], [fs], [], ngg)
+        blk.rows = rows
         blocks.append(blk)
     n0, n1 = blocks[0].mx, blocks[1].mx
     raw = [  # i1 i2 j1 j2 k1 k2 | target ranges | target block (1-based)
@@ -320,6 +341,7 @@ def config1_split(mx=64, my=64, mz=8, split=None, steps=100, precision=L.FP64, s
     ]
     for blk, r in zip(blocks, raw):
         blk.inner = [register_inner_face(r, ngg)]
+        blk.inner_raw = [r]
     for blk, geo in zip(blocks, mesh_geometry(blocks, ngg)):
         blk.xg, blk.yg, blk.zg, blk.jac, blk.metric, _ = geo
     # the perturbed single-block field, restricted to each block (ghosts too)
@@ -328,7 +350,7 @@ def config1_split(mx=64, my=64, mz=8, split=None, steps=100, precision=L.FP64, s
         blk.bv0 = full.blocks[0].bv0[:, :, :, lo:hi].copy()
         blk.sv0 = full.blocks[0].sv0[:, :, :, lo:hi].copy()
     return Case("config1_split_2blocks", full.param, blocks, ngg, precision, steps, steps, None, seed,
-                dict(split=split))
+                dict(split=split, setup=full.meta["setup"]))
 
 
 def _tanh_stretch(n, height, first):
@@ -380,6 +402,7 @@ def config2(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, 
             (0, mx - 1, 0, 0, 0, mz - 1, 2), (0, mx - 1, my - 1, my - 1, 0, mz - 1, 6),
             (0, mx - 1, 0, my - 1, 0, 0, 3), (0, mx - 1, 0, my - 1, mz - 1, mz - 1, 3)]
     blk.bc = bc_program(rows, [(5, "inflow"), (2, "wall")], [fs], [(False, 300.0)], ngg)
+    blk.rows = rows
     finish_block(blk, ngg)
     bv, sv = uniform_field(blk, fs, 2, ngg)
     if perturb:  # +-1% seeded perturbation (parity tests: no cancellation-only residuals)
@@ -393,8 +416,13 @@ def config2(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, 
     blk.bv0, blk.sv0 = bv, sv
     # wall points: the trimmed wall face (src/Driver.cu:128-149)
     wall = np.stack([x[:, 0, :].ravel(), y[:, 0, :].ravel(), z[:, 0, :].ravel()], axis=1)
+    setup = setup_spec(steps, 100, cfl, True, dplur_steps, 2, True, rans=True,
+                       bcs=[("freestream", dict(type="inflow", label=5, mach=mach, pressure=p_inf, temperature=t_inf,
+                                                u=1.0, v=0.0, w=0.0, turbulence_intensity=0.01,
+                                                turb_viscosity_ratio=1.0)),
+                            ("wall", dict(type="wall", label=2, thermal_type="adiabatic"))])
     return Case("config2_air_sst_dplur", param, [blk], ngg, precision, steps, 100, wall, seed,
-                dict(mach=mach, p=p_inf, T=t_inf, rho=rho, u=u_inf, y_first=y_first, beta=beta))
+                dict(mach=mach, p=p_inf, T=t_inf, rho=rho, u=u_inf, y_first=y_first, beta=beta, setup=setup))
 
 
 # ---------------------------------------------------------------- mixtures
@@ -430,7 +458,11 @@ def nasa_cp(p: L.Param, t: float) -> np.ndarray:
         tt = max(t, p.t_low[i])
         c = p.low_temp_coeff if tt < p.t_mid[i] else p.high_temp_coeff
         c = [c[i * 7 + q] for q in range(5)]
-        out[i] = (c[0] + c[1] * tt + c[2] * tt ** 2 + c[3] * tt ** 3 + c[4] * tt ** 4) * RU / p.mw[i]
+        t2 = tt * tt
+        t3 = t2 * tt
+        t4 = t3 * tt
+        out[i] = c[0] + c[1] * tt + c[2] * t2 + c[3] * t3 + c[4] * t4
+        out[i] *= RU / p.mw[i]
     return out
 
 
@@ -438,7 +470,10 @@ def mixture_viscosity(p: L.Param, t: float, y: np.ndarray) -> float:
     """Wilke mixture viscosity (host compute_viscosity, src/Transport.cu:11-38)."""
     ns = p.n_spec
     mw = np.array(p.mw[:ns])
-    mwt = 1.0 / np.sum(y / mw)
+    mwt = 0.0
+    for i in range(ns):
+        mwt += y[i] / mw[i]
+    mwt = 1.0 / mwt
     x = y * mwt / mw
     vis = np.empty(ns)
     for i in range(ns):
@@ -464,11 +499,16 @@ def mixture_inflow(p: L.Param, y: np.ndarray, mach, pressure, temperature, direc
     """Inflow::Inflow for a mixture given (Y, Ma, p, T) (src/BoundCond.cpp:6-158)."""
     ns = p.n_spec
     mwk = np.array(p.mw[:ns])
-    mw = 1.0 / np.sum(y / mwk)
+    mw = 0.0
+    for i in range(ns):
+        mw += y[i] / mwk[i]
+    mw = 1.0 / mw
     viscosity = mixture_viscosity(p, temperature, y)
     cpi = nasa_cp(p, temperature)
-    cp = np.sum(y * cpi)
-    cv = np.sum(y * (cpi - RU / mwk))
+    cp = cv = 0.0
+    for i in range(ns):
+        cp += y[i] * cpi[i]
+        cv += y[i] * (cpi[i] - RU / mwk[i])
     gamma = cp / cv
     c = math.sqrt(gamma * RU / mw * temperature)
     velocity = mach * c
@@ -550,6 +590,7 @@ def config3(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=4, cf
             (0, mx - 1, 0, 0, 0, mz - 1, 6), (0, mx - 1, my - 1, my - 1, 0, mz - 1, 6),
             (0, mx - 1, 0, my - 1, 0, 0, 3), (0, mx - 1, 0, my - 1, mz - 1, mz - 1, 3)]
     blk.bc = bc_program(rows, [(5, "inflow"), (9, "inflow")], [air, fuel], [], ngg)
+    blk.rows = rows
     finish_block(blk, ngg)
     # mixing-layer initial field on the padded block (ghosts get the profile too)
     shape = (mz + 2 * ngg, my + 2 * ngg, mx + 2 * ngg)
@@ -576,5 +617,11 @@ def config3(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=4, cf
     perturb_interior(bv[0], blk, seed, 0, ngg, rel=0.02)
     bv[5] = bv[4] * mw / (RU * bv[0])
     blk.bv0, blk.sv0 = bv, sv
+    setup = setup_spec(steps, 100, cfl, True, dplur_steps, 2, True, species=True, reaction=True,
+                       chem_method=chem_method,
+                       bcs=[("freestream", dict(type="inflow", label=5, mach=2.0, pressure=101325.0, temperature=1500.0,
+                                                u=1.0, v=0.0, w=0.0, O2=0.233, N2=0.767)),
+                            ("fuel", dict(type="inflow", label=9, mach=1.2, pressure=101325.0, temperature=1000.0,
+                                          u=1.0, v=0.0, w=0.0, H2=1.0))])
     return Case("config3_h2air_fr_epi_dplur", param, [blk], ngg, precision, steps, 100, None, seed,
-                dict(species=names, air=air.__dict__, fuel=fuel.__dict__, mech_info=info))
+                dict(species=names, air=air.__dict__, fuel=fuel.__dict__, mech_info=info, setup=setup))

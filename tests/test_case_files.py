@@ -1,0 +1,269 @@
+"""Case directories in the reference's format, read by libostc's C++ reader
+(Parameter src/Parameter.cpp:8-168, Mesh src/Mesh.cpp:546-735, inflow states
+src/BoundCond.cpp:6-106, boundary programs src/BoundCond.cu:73-232,
+initialize_from_start src/Initialize.cu:4-43), and the C++ time-loop driver
+(Driver src/Driver.cu:14-380).
+
+CPU: the synthetic configurations written with case_files.write_case read
+back bit-exactly equal to the Python-built cases (model constants, boundary
+programs, geometry, inner faces, wall points, initial field); the
+reference's own sample setup files (tests/golden/reference_setup, copied
+from its input_files/setup) parse to the values they state, with inflow
+states equal to an independent Python evaluation; grammar edge cases.
+GPU: the C++ driver reproduces the Python-driven loop bit-for-bit, and the
+command-line runner completes a case."""
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from openstc_amd import _lib as L
+from openstc_amd import case_files, cases
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SAMPLE = os.path.join(HERE, "golden", "reference_setup")
+
+
+def struct_diffs(a, b, name=""):
+    out = []
+    for f, _ in a._fields_:
+        va, vb = getattr(a, f), getattr(b, f)
+        if hasattr(va, "_fields_"):
+            out += struct_diffs(va, vb, f"{name}{f}.")
+        elif hasattr(va, "_length_"):
+            la, lb = list(va), list(vb)
+            if la and hasattr(la[0], "_fields_"):
+                for q, (x, y) in enumerate(zip(la, lb)):
+                    out += struct_diffs(x, y, f"{name}{f}[{q}].")
+            else:
+                la = [bytes(x) if hasattr(x, "_length_") else x for x in la]
+                lb = [bytes(x) if hasattr(x, "_length_") else x for x in lb]
+                if la != lb:
+                    out.append(name + f)
+        elif va != vb:
+            out.append(name + f)
+    return out
+
+
+BUILDERS = {
+    "config1": lambda: cases.config1(mx=20, my=16, mz=8, steps=3),
+    "config2": lambda: cases.config2(mx=24, my=20, mz=8, steps=3),
+    "config3": lambda: cases.config3(mx=16, my=12, mz=8, steps=3),
+    "config1_split": lambda: cases.config1_split(mx=20, my=16, mz=8, steps=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BUILDERS))
+def test_written_case_reads_back_bit_exact(tmp_path, name):
+    case = BUILDERS[name]()
+    d = case_files.write_case(case, str(tmp_path / name))
+    cd = case_files.CaseDir(d)
+    assert cd.info.n_block == len(case.blocks)
+    assert cd.info.total_step == case.meta["setup"]["total_step"]
+    assert struct_diffs(case.param, cd.param()) == []
+    for b, blk in enumerate(case.blocks):
+        assert struct_diffs(blk.bc, cd.bc(b)) == [], b
+        for fld, arr, n in (("X", blk.xg, 1), ("Y", blk.yg, 1), ("Z", blk.zg, 1), ("JAC", blk.jac, 1),
+                            ("METRIC", blk.metric, 9)):
+            assert np.array_equal(cd.field(b, fld, n).reshape(arr.shape), arr), (b, fld)
+        inner = cd.inner(b)
+        assert len(inner) == len(blk.inner)
+        for x, y in zip(blk.inner, inner):
+            assert struct_diffs(x, y) == []
+        # initialize_from_start: the default inflow ("freestream", inflow slot
+        # 0 in every configuration) everywhere, ghosts included
+        bv = cd.field(b, "BV", 6)
+        st = blk.bc.inflow[0]
+        ref = [st.density, st.u, st.v, st.w, st.pressure, st.temperature]
+        for l in range(6):
+            assert np.all(bv[l] == ref[l]), l
+        nsc = case.param.n_scalar
+        if nsc:
+            sv = cd.field(b, "SV", nsc)
+            for l in range(nsc):
+                assert np.all(sv[l] == st.sv[l]), l
+    if case.wall_points is not None:
+        assert np.array_equal(cd.wall_points(), case.wall_points)
+
+
+def sample_dir(tmp_path, mx=20, my=16, mz=8):
+    """The reference's sample setup (9-species FR + SST, DPLUR 3, NND-2) on a
+    3-D box with the sample's boundary labels: wall 2 at j=0, freestream 5 and
+    fuel 9 on i=0, outflow 6 elsewhere."""
+    d = tmp_path / "sample"
+    inp = d / "input_files"
+    (inp / "setup").mkdir(parents=True)
+    for fn in os.listdir(SAMPLE):
+        if fn.endswith(".txt") and fn[0].isdigit():
+            shutil.copyfile(os.path.join(SAMPLE, fn), inp / "setup" / fn)
+    grid = (inp / "setup" / "1_grid_information.txt")
+    grid.write_text(grid.read_text().replace("int dimension = 2", "int dimension = 3"))
+    (inp / "grid").mkdir()
+    (inp / "boundary_condition").mkdir()
+    (inp / "chemistry").mkdir()
+    for fn in ("combustion_mechanism.inp", "therm.dat", "tran.dat"):
+        shutil.copyfile(os.path.join(cases.DATA, fn), inp / "chemistry" / fn)
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    blk = cases.Block(mx, my, mz, i * 1e-3 * 1.0, j * 1e-3 * 0.5, k * 1e-3 * 0.7)
+    js = my // 2
+    blk.rows = [(0, mx - 1, 0, 0, 0, mz - 1, 2), (0, 0, js, my - 1, 0, mz - 1, 5), (0, 0, 0, js, 0, mz - 1, 9),
+                (mx - 1, mx - 1, 0, my - 1, 0, mz - 1, 6), (0, mx - 1, my - 1, my - 1, 0, mz - 1, 6),
+                (0, mx - 1, 0, my - 1, 0, 0, 6), (0, mx - 1, 0, my - 1, mz - 1, mz - 1, 6)]
+    case_files.write_grid(str(inp / "grid" / "grid   0.grd"), [blk])
+    case_files.write_boundary(str(inp / "boundary_condition" / "boundary   0.txt"), [blk])
+    return str(d), blk
+
+
+def test_reference_sample_setup(tmp_path):
+    d, blk = sample_dir(tmp_path)
+    cd = case_files.CaseDir(d)
+    p = cd.param()
+    info = cd.info
+    # 0_global_control / 2_scheme / 3_species_reactions / 4_laminar_turbulent / 9_transport_property
+    assert (info.total_step, info.output_file, info.output_screen) == (10000, 5000, 1000)
+    assert info.convergence_criteria == 1e-5 and info.grid_scale == 0.001
+    assert (p.mix_model, p.turb_method, p.rans_model, p.turb_implicit) == (L.FR, L.RANS, 2, 1)
+    assert (p.inviscid_scheme, p.reconstruction, p.limiter, p.viscous_scheme) == (3, 3, 0, 2)
+    assert (p.implicit_method, p.dplur_inner_step, p.chem_src_method) == (1, 3, 1)
+    assert (p.n_spec, p.n_reac, p.n_scalar, p.n_var) == (9, 19, 11, 16)
+    assert (p.cfl, p.Pr, p.Sc, p.Prt, p.Sct) == (0.5, 0.72, 0.5, 0.9, 0.9)
+    # gridScale 0.001 scales the coordinates
+    assert np.array_equal(cd.field(0, "X", 1)[0, 2:-2, 2:-2, 2:-2], blk.x * 0.001)
+    # 5_boundary_condition: freestream (label 5), fuel (label 9), isothermal wall 294.4 K
+    mech, minfo = cases.load_mechanism(True)
+    names = cases.species_names(minfo)
+    y_fs = np.zeros(9)
+    for n, v in (("O2", 0.258), ("N2", 0.486), ("H2O", 0.256)):
+        y_fs[names.index(n)] = v
+    y_fu = np.zeros(9)
+    y_fu[names.index("H2")] = 1.0
+    fs = cases.mixture_inflow(mech, y_fs, 2.44, 105431.0, 1270.0, rans=True, intensity=0.01, mut_ratio=1.0)
+    fu = cases.mixture_inflow(mech, y_fu, 1.0, 105431.0, 254.0, rans=True, intensity=0.01, mut_ratio=1.0)
+    bc = cd.bc(0)
+    for slot, st in ((0, fs), (1, fu)):
+        iv = bc.inflow[slot]
+        got = [iv.density, iv.u, iv.v, iv.w, iv.pressure, iv.temperature, iv.mut] + list(iv.sv[:11])
+        ref = [st.density, st.u, st.v, st.w, st.pressure, st.temperature, st.mut] + list(st.sv)
+        assert got == ref, slot
+    assert bc.wall[0].isothermal == 1 and bc.wall[0].temperature == 294.4
+    # application order: outflow faces, inflow 5 then 9, then the wall (trimmed)
+    kinds = [(bc.face[q].kind, bc.face[q].slot) for q in range(bc.n_face)]
+    assert kinds == [(6, 0)] * 4 + [(5, 0), (5, 1), (2, 0)]
+    w = bc.face[6]
+    assert list(w.range_start) == [0, 0, 0] and list(w.range_end) == [blk.mx - 1, 0, blk.mz - 1]
+    # DParameter limits from the freestream struct
+    assert p.limit_ll[0] == 1e-6 * fs.density and p.limit_ul[4] == 1e+3 * fs.pressure
+    assert list(p.sv_inf[:11]) == list(fs.sv)
+    assert cd.info.n_wall_point == blk.mx * blk.mz
+    # 8_initialization: groups_init = 1 -> the freestream everywhere
+    bv = cd.field(0, "BV", 6)
+    assert np.all(bv[0] == fs.density) and np.all(bv[5] == fs.temperature)
+
+
+def write_setup(tmp_path, edit):
+    case = cases.config1(mx=12, my=10, mz=6, steps=3)
+    d = case_files.write_case(case, str(tmp_path / "c"))
+    for fn, fun in edit.items():
+        path = os.path.join(d, "input_files", "setup", fn)
+        with open(path) as f:
+            text = f.read()
+        with open(path, "w") as f:
+            f.write(fun(text))
+    return d, case
+
+
+def test_setup_grammar_rules(tmp_path):
+    # first definition wins; a BOM-prefixed comment line is ignored; "//"
+    # continues an array on the next line; the value on a string array's
+    # opening line is dropped (the reference reads two tokens there)
+    d, case = write_setup(tmp_path, {
+        "2_scheme.txt": lambda t: "﻿// header\nreal    cfl = 0.25\n" + t,
+        "6_post_process.txt": lambda t: "array int post_process { 0 //\n 3 }\n",
+        "5_boundary_condition.txt": lambda t: t.replace(
+            "array   string  boundary_conditions {\n    freestream\n}",
+            "array string boundary_conditions { bogus\n freestream\n}"),
+    })
+    p = case_files.CaseDir(d).param()
+    assert p.cfl == 0.25
+    # init_cond patch: a box taking the state of another struct
+    d, case = write_setup(tmp_path / "b", {
+        "8_initialization.txt": lambda t: t.replace("int groups_init = 1", "int groups_init = 2") +
+        "struct init_cond_0 {\n string label = slow\n real x0 = -1\n real x1 = 0.5\n real y0 = -1\n"
+        " real y1 = 10\n real z0 = -1\n real z1 = 10\n}\n",
+        "5_boundary_condition.txt": lambda t: t + "struct slow {\n string type = inflow\n int label = 7\n"
+        " real mach = 0.5\n real pressure = 90000\n real temperature = 250\n}\n",
+    })
+    cd = case_files.CaseDir(d)
+    bv = cd.field(0, "BV", 6)
+    x = cd.field(0, "X", 1)[0]
+    slow = cases.air_inflow(0.5, 90000.0, 250.0)
+    fast = cases.air_inflow(2.0, 101325.0, 300.0)
+    inside = x <= 0.5
+    assert inside.any() and (~inside).any()
+    assert np.all(bv[0][inside] == slow.density) and np.all(bv[1][inside] == slow.u)
+    assert np.all(bv[0][~inside] == fast.density)
+
+
+@pytest.mark.parametrize("edit,msg", [
+    ({"1_grid_information.txt": lambda t: t.replace("int dimension = 3", "int dimension = 2")}, "dimension"),
+    ({"0_global_control.txt": lambda t: t.replace("bool parallel = 0", "bool parallel = 1")}, "parallel"),
+    ({"0_global_control.txt": lambda t: t.replace("int initial = 0", "int initial = 1")}, "initial"),
+    ({"2_scheme.txt": lambda t: t.replace("real    cfl", "real    cfl_typo")}, "cfl"),
+    ({"5_boundary_condition.txt": lambda t: t.replace("struct  freestream", "struct  other")}, "freestream"),
+])
+def test_unsupported_or_incomplete_setup_is_rejected(tmp_path, edit, msg):
+    d, _ = write_setup(tmp_path, edit)
+    with pytest.raises(L.OstcError, match=msg):
+        case_files.CaseDir(d)
+
+
+def inject(cd, case):
+    for b, blk in enumerate(case.blocks):
+        cd.set_field(b, "BV", blk.bv0)
+        if case.param.n_scalar:
+            cd.set_field(b, "SV", blk.sv0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,precision,steps", [("config1", L.FP64, 5), ("config2", L.FP32, 3),
+                                                   ("config3", L.FP32, 2), ("config1_split", L.FP64, 3)])
+def test_driver_matches_python_loop(tmp_path, name, precision, steps):
+    from openstc_amd.runtime import Simulation
+    print(f"[gpu-driver] {name} precision {precision} steps {steps}", flush=True)
+    case = BUILDERS[name]()
+    d = case_files.write_case(case, str(tmp_path / name))
+    cd = case_files.CaseDir(d)
+    inject(cd, case)
+    drv = case_files.Driver(cd, precision)
+    drv.initialize()
+    sim = Simulation(case, precision)
+    sim.initialize_computation()
+    for s in range(steps):
+        r = drv.step()
+        e = sim.step(output_screen=case.meta["setup"]["output_screen"])
+        if r is not None or e is not None:
+            assert r[0] == e, (s, r, e)
+    sim.sync()
+    for b in range(len(case.blocks)):
+        got = drv.download(b, "BV", 6)
+        ref = sim.zones[b].download("BV")
+        assert np.array_equal(got, ref), b
+    drv.close()
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_command_line_runner(tmp_path):
+    case = cases.config1(mx=24, my=16, mz=8, steps=4)
+    d = case_files.write_case(case, str(tmp_path / "cli"))
+    exe = os.path.join(os.path.dirname(L.LIB_PATH), "ostc_run")
+    print("[gpu-driver] ostc_run", d, flush=True)
+    r = subprocess.run([exe, d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Mcell-iter/s" in r.stdout and "converged to" in r.stdout
+    hist = open(os.path.join(d, "history.dat")).read().split("\n")
+    assert hist[0] == "step\terror_max" and hist[1].startswith("1\t")
+    assert os.path.exists(os.path.join(d, "output", "message", "residual_scale.txt"))
