@@ -312,8 +312,10 @@ int ostc_zone_create(ostc_ctx *ctx, const ostc_zone_desc *d, ostc_zone **out) {
     // limit_flow repair list, sized for every interior cell
     const size_t ncell = (size_t)d->mx * d->my * d->mz;
     const size_t nval = 6 + nsc + nv;
-    if (e == hipSuccess) e = hipMalloc(&z->lf_count, sizeof(int));
-    if (e == hipSuccess) e = hipMemset(z->lf_count, 0, sizeof(int));
+    // three list counters: slots 0/1 alternate between time steps, slot 2 is
+    // the stand-alone limit_flow's
+    if (e == hipSuccess) e = hipMalloc(&z->lf_count, 3 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(z->lf_count, 0, 3 * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&z->lf_idx, ncell * sizeof(int64_t));
     if (e == hipSuccess) e = hipMalloc(&z->lf_vals, ncell * nval * z->esize);
   }
@@ -588,20 +590,20 @@ int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_fac
         if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
         if (r) return r;
         prof_mark(P_UPDATE, st, true);
-        r = update_cv_bv(z->vd, c->dev, o, res, st);
+        r = update_cv_bv(z->vd, c->dev, o, res, st, step & 1);
       } else {
         float *res = nullptr;
         r = implicit_treatment(z->vf, c->dev, o, st, &res);
         if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
         if (r) return r;
         prof_mark(P_UPDATE, st, true);
-        r = update_cv_bv(z->vf, c->dev, o, res, st);
+        r = update_cv_bv(z->vf, c->dev, o, res, st, step & 1);
       }
       prof_mark(P_UPDATE, st, false);
     }
     if (r) return r;
     prof_mark(P_LIMIT, st, true);
-    r = ZONE_CALL(z, limit_flow(z->vf, c->dev, o, st), limit_flow(z->vd, c->dev, o, st));
+    r = ZONE_CALL(z, limit_flow(z->vf, c->dev, o, st, step & 1), limit_flow(z->vd, c->dev, o, st, step & 1));
     prof_mark(P_LIMIT, st, false);
     if (r) return r;
     prof_mark(P_BC, st, true);

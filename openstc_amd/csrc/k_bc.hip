@@ -12,137 +12,168 @@ namespace {
 struct FaceLaunch {
   int rs[3], re[3];
   int face, direction;
+  int g0;  // first layer a launch handles: 0 = the face point, 1 = the first ghost layer
 };
 
-// (a, b) = the two tangential directions of `face`
-__device__ __forceinline__ bool face_point(const FaceLaunch &f, int p[3]) {
+// Thread -> (face point, layer g): g = 0 is the face point itself, 1..ngg the
+// ghost layers outside it.  Every (point, layer) pair is independent, so the
+// layers run in parallel and each thread issues all its loads before its
+// stores.  On an i-face four adjacent lanes take the layers of one row (the
+// layers are adjacent in memory there); on j/k faces i varies fastest.
+__device__ __forceinline__ bool face_layer(const FaceLaunch &f, int ngg, int p[3], int &g) {
   const int a = f.face == 0 ? 1 : 0, b = f.face == 2 ? 1 : 2;
+  int ia, il;
+  if (f.face == 0) {
+    il = (int)(threadIdx.x & 3);
+    ia = (int)(blockIdx.x * 16 + (threadIdx.x >> 2));
+  } else {
+    il = (int)blockIdx.z;
+    ia = (int)(blockIdx.x * 64 + threadIdx.x);
+  }
+  const int ib = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  g = f.g0 + il;
   p[f.face] = f.rs[f.face];
-  p[a] = f.rs[a] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  p[b] = f.rs[b] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  return p[a] <= f.re[a] && p[b] <= f.re[b];
+  p[a] = f.rs[a] + ia;
+  p[b] = f.rs[b] + ib;
+  return g <= ngg && p[a] <= f.re[a] && p[b] <= f.re[b];
 }
 
-// compute_total_energy (src/FieldOperation.cuh:15-35) for one point
+// compute_total_energy (src/FieldOperation.cuh:15-35) of a point whose bv and
+// species cv are in registers
 template <typename R, int NS>
-__device__ __forceinline__ void total_energy(const ZoneView<R> &z, const ostc_param *__restrict__ p, int64_t id) {
-  const int64_t P = z.plane;
-  const R u = z.bv[P + id], v = z.bv[2 * P + id], w = z.bv[3 * P + id];
+__device__ __forceinline__ R total_energy_reg(const ostc_param *__restrict__ p, R rho, R u, R v, R w, R pr, R t,
+                                              const R *cvs, R &vel) {
   const R vel2 = u * u + v * v + w * w;
-  R e = (R)0.5 * z.bv[id] * vel2;
+  R e = (R)0.5 * rho * vel2;
   if constexpr (NS > 0) {
     R h[NS];
-    enthalpy<R>(z.bv[5 * P + id], h, p, NS);
+    enthalpy<R>(t, h, p, NS);
 #pragma unroll
-    for (int l = 0; l < NS; ++l) e += h[l] * z.cv[(5 + l) * P + id];
-    e -= z.bv[4 * P + id];
+    for (int l = 0; l < NS; ++l) e += h[l] * cvs[l];
+    e -= pr;
   } else {
-    e += z.bv[4 * P + id] / (R)(kGammaAir - 1);
+    e += pr / (R)(kGammaAir - 1);
   }
-  z.cv[4 * P + id] = e;
-  z.vel[id] = sqrt(vel2);
+  vel = sqrt(vel2);
+  return e;
 }
 
-// apply_outflow (src/BoundCond.cuh:127-154)
+// apply_outflow (src/BoundCond.cuh:127-154): ghost layer g copies the face point
 template <typename R, int NSC, bool RANS>
-__global__ void k_outflow(ZoneView<R> z, FaceLaunch f) {
-  int pt[3];
-  if (!face_point(f, pt)) return;
+__global__ __launch_bounds__(256) void k_outflow(ZoneView<R> z, FaceLaunch f) {
+  int pt[3], g;
+  if (!face_layer(f, z.g, pt, g)) return;
   int dir[3] = {0, 0, 0};
   dir[f.face] = f.direction;
   const int64_t P = z.plane, id = z.idx(pt[0], pt[1], pt[2]);
-  for (int g = 1; g <= z.g; ++g) {
-    const int64_t gi = z.idx(pt[0] + g * dir[0], pt[1] + g * dir[1], pt[2] + g * dir[2]);
+  const int64_t gi = z.idx(pt[0] + g * dir[0], pt[1] + g * dir[1], pt[2] + g * dir[2]);
+  R bv[6], sv[NSC > 0 ? NSC : 1], mut = 0;
 #pragma unroll
-    for (int l = 0; l < 6; ++l) z.bv[l * P + gi] = z.bv[l * P + id];
+  for (int l = 0; l < 6; ++l) bv[l] = z.bv[l * P + id];
 #pragma unroll
-    for (int l = 0; l < NSC; ++l) z.sv[l * P + gi] = z.sv[l * P + id];
-    if (RANS) z.mut[gi] = z.mut[id];
-  }
+  for (int l = 0; l < NSC; ++l) sv[l] = z.sv[l * P + id];
+  if (RANS) mut = z.mut[id];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) z.bv[l * P + gi] = bv[l];
+#pragma unroll
+  for (int l = 0; l < NSC; ++l) z.sv[l * P + gi] = sv[l];
+  if (RANS) z.mut[gi] = mut;
 }
 
-// apply_inflow (src/BoundCond.cuh:156-194)
+// apply_inflow (src/BoundCond.cuh:156-194): ghost layers take the inflow state
 template <typename R, int NSC, bool RANS>
-__global__ void k_inflow(ZoneView<R> z, FaceLaunch f, const ostc_inflow *__restrict__ in) {
-  int pt[3];
-  if (!face_point(f, pt)) return;
+__global__ __launch_bounds__(256) void k_inflow(ZoneView<R> z, FaceLaunch f, const ostc_inflow *__restrict__ in) {
+  int pt[3], g;
+  if (!face_layer(f, z.g, pt, g)) return;
   int dir[3] = {0, 0, 0};
   dir[f.face] = f.direction;
   const int64_t P = z.plane;
-  for (int g = 1; g <= z.g; ++g) {
-    const int64_t gi = z.idx(pt[0] + g * dir[0], pt[1] + g * dir[1], pt[2] + g * dir[2]);
-    z.bv[gi] = (R)in->density;
-    z.bv[P + gi] = (R)in->u;
-    z.bv[2 * P + gi] = (R)in->v;
-    z.bv[3 * P + gi] = (R)in->w;
-    z.bv[4 * P + gi] = (R)in->pressure;
-    z.bv[5 * P + gi] = (R)in->temperature;
+  const int64_t gi = z.idx(pt[0] + g * dir[0], pt[1] + g * dir[1], pt[2] + g * dir[2]);
+  z.bv[gi] = (R)in->density;
+  z.bv[P + gi] = (R)in->u;
+  z.bv[2 * P + gi] = (R)in->v;
+  z.bv[3 * P + gi] = (R)in->w;
+  z.bv[4 * P + gi] = (R)in->pressure;
+  z.bv[5 * P + gi] = (R)in->temperature;
 #pragma unroll
-    for (int l = 0; l < NSC; ++l) z.sv[l * P + gi] = (R)in->sv[l];
-    if (RANS) z.mut[gi] = (R)in->mut;
-  }
+  for (int l = 0; l < NSC; ++l) z.sv[l * P + gi] = (R)in->sv[l];
+  if (RANS) z.mut[gi] = (R)in->mut;
 }
 
-// apply_symmetry (src/BoundCond.cuh:45-125)
+// apply_symmetry (src/BoundCond.cuh:45-125): layer 0 = the face point (the
+// interior neighbour's state with its normal velocity removed), layer g = the
+// mirror of interior point -g with the normal velocity reversed
 template <typename R, int NSC, bool RANS>
-__global__ void k_symmetry(ZoneView<R> z, FaceLaunch f) {
-  int pt[3];
-  if (!face_point(f, pt)) return;
+__global__ __launch_bounds__(256) void k_symmetry(ZoneView<R> z, FaceLaunch f) {
+  int pt[3], g;
+  if (!face_layer(f, z.g, pt, g)) return;
   int dir[3] = {0, 0, 0};
   dir[f.face] = f.direction;
   const int64_t P = z.plane, id = z.idx(pt[0], pt[1], pt[2]);
-  const int64_t in = z.idx(pt[0] - dir[0], pt[1] - dir[1], pt[2] - dir[2]);
   R kx = z.metric[(f.face * 3) * P + id], ky = z.metric[(f.face * 3 + 1) * P + id],
     kz = z.metric[(f.face * 3 + 2) * P + id];
   const R km = sqrt(kx * kx + ky * ky + kz * kz);
   kx /= km;
   ky /= km;
   kz /= km;
-  const R u1 = z.bv[P + in], v1 = z.bv[2 * P + in], w1 = z.bv[3 * P + in];
-  R uk = kx * u1 + ky * v1 + kz * w1;
-  const R ut = u1 - kx * uk, vt = v1 - ky * uk, wt = w1 - kz * uk;
-  z.bv[P + id] = ut;
-  z.bv[2 * P + id] = vt;
-  z.bv[3 * P + id] = wt;
-  z.vel[id] = sqrt(ut * ut + vt * vt + wt * wt);
-  const R rho = z.bv[in];
-  z.bv[id] = rho;
-  z.bv[4 * P + id] = z.bv[4 * P + in];
-  z.bv[5 * P + id] = z.bv[5 * P + in];
+  if (g == 0) {
+    const int64_t in = z.idx(pt[0] - dir[0], pt[1] - dir[1], pt[2] - dir[2]);
+    const R u1 = z.bv[P + in], v1 = z.bv[2 * P + in], w1 = z.bv[3 * P + in];
+    const R rho = z.bv[in], pr = z.bv[4 * P + in], tt = z.bv[5 * P + in], e1 = z.cv[4 * P + in];
+    R sv[NSC > 0 ? NSC : 1];
 #pragma unroll
-  for (int l = 0; l < NSC; ++l) z.sv[l * P + id] = z.sv[l * P + in];
-  z.cv[id] = rho;
-  z.cv[P + id] = rho * ut;
-  z.cv[2 * P + id] = rho * vt;
-  z.cv[3 * P + id] = rho * wt;
-  z.cv[4 * P + id] = z.cv[4 * P + in] - (R)0.5 * z.bv[in] * (u1 * u1 + v1 * v1 + w1 * w1) +
-                     (R)0.5 * rho * (ut * ut + vt * vt + wt * wt);
+    for (int l = 0; l < NSC; ++l) sv[l] = z.sv[l * P + in];
+    const R uk = kx * u1 + ky * v1 + kz * w1;
+    const R ut = u1 - kx * uk, vt = v1 - ky * uk, wt = w1 - kz * uk;
+    z.bv[P + id] = ut;
+    z.bv[2 * P + id] = vt;
+    z.bv[3 * P + id] = wt;
+    z.vel[id] = sqrt(ut * ut + vt * vt + wt * wt);
+    z.bv[id] = rho;
+    z.bv[4 * P + id] = pr;
+    z.bv[5 * P + id] = tt;
 #pragma unroll
-  for (int l = 0; l < NSC; ++l) z.cv[(5 + l) * P + id] = rho * z.sv[l * P + id];
-  for (int g = 1; g <= z.g; ++g) {
-    const int64_t gi = z.idx(pt[0] + g * dir[0], pt[1] + g * dir[1], pt[2] + g * dir[2]);
-    const int64_t si = z.idx(pt[0] - g * dir[0], pt[1] - g * dir[1], pt[2] - g * dir[2]);
-    z.bv[gi] = z.bv[si];
-    const R u = z.bv[P + si], v = z.bv[2 * P + si], w = z.bv[3 * P + si];
-    uk = kx * u + ky * v + kz * w;
-    const R ug = u - (R)2 * uk * kx, vg = v - (R)2 * uk * ky, wg = w - (R)2 * uk * kz;
-    z.bv[P + gi] = ug;
-    z.bv[2 * P + gi] = vg;
-    z.bv[3 * P + gi] = wg;
-    z.vel[gi] = sqrt(ug * ug + vg * vg + wg * wg);
-    z.bv[4 * P + gi] = z.bv[4 * P + si];
-    z.bv[5 * P + gi] = z.bv[5 * P + si];
+    for (int l = 0; l < NSC; ++l) z.sv[l * P + id] = sv[l];
+    z.cv[id] = rho;
+    z.cv[P + id] = rho * ut;
+    z.cv[2 * P + id] = rho * vt;
+    z.cv[3 * P + id] = rho * wt;
+    z.cv[4 * P + id] = e1 - (R)0.5 * rho * (u1 * u1 + v1 * v1 + w1 * w1) + (R)0.5 * rho * (ut * ut + vt * vt + wt * wt);
 #pragma unroll
-    for (int l = 0; l < NSC; ++l) z.sv[l * P + gi] = z.sv[l * P + si];
-    if (RANS) z.mut[gi] = z.mut[si];
+    for (int l = 0; l < NSC; ++l) z.cv[(5 + l) * P + id] = rho * sv[l];
+    return;
   }
+  const int64_t gi = z.idx(pt[0] + g * dir[0], pt[1] + g * dir[1], pt[2] + g * dir[2]);
+  const int64_t si = z.idx(pt[0] - g * dir[0], pt[1] - g * dir[1], pt[2] - g * dir[2]);
+  const R rho = z.bv[si], u = z.bv[P + si], v = z.bv[2 * P + si], w = z.bv[3 * P + si];
+  const R pr = z.bv[4 * P + si], tt = z.bv[5 * P + si];
+  R sv[NSC > 0 ? NSC : 1], mut = 0;
+#pragma unroll
+  for (int l = 0; l < NSC; ++l) sv[l] = z.sv[l * P + si];
+  if (RANS) mut = z.mut[si];
+  const R uk = kx * u + ky * v + kz * w;
+  const R ug = u - (R)2 * uk * kx, vg = v - (R)2 * uk * ky, wg = w - (R)2 * uk * kz;
+  z.bv[gi] = rho;
+  z.bv[P + gi] = ug;
+  z.bv[2 * P + gi] = vg;
+  z.bv[3 * P + gi] = wg;
+  z.vel[gi] = sqrt(ug * ug + vg * vg + wg * wg);
+  z.bv[4 * P + gi] = pr;
+  z.bv[5 * P + gi] = tt;
+#pragma unroll
+  for (int l = 0; l < NSC; ++l) z.sv[l * P + gi] = sv[l];
+  if (RANS) z.mut[gi] = mut;
 }
 
-// apply_wall (src/BoundCond.cuh:196-319)
+// apply_wall (src/BoundCond.cuh:196-319): layer 0 = the wall point (no slip,
+// pressure of the interior neighbour, wall or interior temperature, SST omega
+// at the wall), layer g = the reflection of interior point -g; a ghost thread
+// recomputes the wall omega it copies
 template <typename R, int NS, bool RANS>
-__global__ void k_wall(ZoneView<R> z, FaceLaunch f, const ostc_param *__restrict__ p, int isothermal, double tw_in) {
-  int pt[3];
-  if (!face_point(f, pt)) return;
+__global__ __launch_bounds__(256) void k_wall(ZoneView<R> z, FaceLaunch f, const ostc_param *__restrict__ p,
+                                              int isothermal, double tw_in) {
+  int pt[3], g;
+  if (!face_layer(f, z.g, pt, g)) return;
   int dir[3] = {0, 0, 0};
   dir[f.face] = f.direction;
   const int64_t P = z.plane, id = z.idx(pt[0], pt[1], pt[2]);
@@ -151,82 +182,93 @@ __global__ void k_wall(ZoneView<R> z, FaceLaunch f, const ostc_param *__restrict
   if (!isothermal) t_wall = z.bv[5 * P + in];
   const R pw = z.bv[4 * P + in];
   R mw = (R)kMwAir;
+  R yw[NS > 0 ? NS : 1];
   if constexpr (NS > 0) {
     mw = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      const R y = z.sv[l * P + in];
-      z.sv[l * P + id] = y;
-      mw += y * (R)(1.0 / p->mw[l]);
+      yw[l] = z.sv[l * P + in];
+      mw += yw[l] * (R)(1.0 / p->mw[l]);
     }
     mw = (R)1 / mw;
   }
   const R rho_w = pw * mw / (t_wall * (R)kRu);
-  z.bv[id] = rho_w;
-  z.bv[P + id] = 0;
-  z.bv[2 * P + id] = 0;
-  z.bv[3 * P + id] = 0;
-  z.bv[4 * P + id] = pw;
-  z.bv[5 * P + id] = t_wall;
-  z.cv[id] = rho_w;
-  z.cv[P + id] = 0;
-  z.cv[2 * P + id] = 0;
-  z.cv[3 * P + id] = 0;
-#pragma unroll
-  for (int l = 0; l < NS; ++l) z.cv[(5 + l) * P + id] = z.sv[l * P + id] * rho_w;
-  total_energy<R, NS>(z, p, id);
+  R om = 0;
   if constexpr (RANS) {
     R mu_w;
     if constexpr (NS > 0) {
-      R y[NS];
-#pragma unroll
-      for (int l = 0; l < NS; ++l) y[l] = z.sv[l * P + id];
-      mixture_transport<R>(p, NS, t_wall, mw, y, (const R *)nullptr, &mu_w, (R *)nullptr, (R *)nullptr);
+      mixture_transport<R>(p, NS, t_wall, mw, yw, (const R *)nullptr, &mu_w, (R *)nullptr, (R *)nullptr);
     } else {
       mu_w = sutherland<R>(t_wall);
     }
     const R dy = z.wall_dist[in];
-    z.sv[NS * P + id] = 0;
-    z.cv[(NS + 5) * P + id] = 0;
-    R om;
     if (dy > (R)1e-25)
       om = (R)60 * mu_w / (rho_w * (R)sst::beta_1 * dy * dy);
     else
       om = z.sv[(NS + 1) * P + in];
-    z.sv[(NS + 1) * P + id] = om;
-    z.cv[(NS + 6) * P + id] = rho_w * om;
   }
-  for (int g = 1; g <= z.g; ++g) {
-    const int64_t si = z.idx(pt[0] - g * dir[0], pt[1] - g * dir[1], pt[2] - g * dir[2]);
-    const int64_t gi = z.idx(pt[0] + g * dir[0], pt[1] + g * dir[1], pt[2] + g * dir[2]);
-    const R ui = z.bv[P + si], vi = z.bv[2 * P + si], wi = z.bv[3 * P + si];
-    const R pi = z.bv[4 * P + si], ti = z.bv[5 * P + si];
-    R tg = ti;
-    if (isothermal) {
-      tg = (R)2 * t_wall - ti;
-      if (tg <= (R)0.1 * t_wall) tg = t_wall;
-    }
-    if constexpr (NS > 0) {
-      mw = 0;
+  if (g == 0) {
+    R cvs[NS > 0 ? NS : 1];
 #pragma unroll
-      for (int l = 0; l < NS; ++l) {
-        const R y = z.sv[l * P + si];
-        z.sv[l * P + gi] = y;
-        mw += y * (R)(1.0 / p->mw[l]);
-      }
-      mw = (R)1 / mw;
-    }
-    z.bv[gi] = pi * mw / (tg * (R)kRu);
-    z.bv[P + gi] = -ui;
-    z.bv[2 * P + gi] = -vi;
-    z.bv[3 * P + gi] = -wi;
-    z.bv[4 * P + gi] = pi;
-    z.bv[5 * P + gi] = tg;
+    for (int l = 0; l < NS; ++l) cvs[l] = yw[l] * rho_w;
+    R vel;
+    const R e = total_energy_reg<R, NS>(p, rho_w, (R)0, (R)0, (R)0, pw, t_wall, cvs, vel);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.sv[l * P + id] = yw[l];
+    z.bv[id] = rho_w;
+    z.bv[P + id] = 0;
+    z.bv[2 * P + id] = 0;
+    z.bv[3 * P + id] = 0;
+    z.bv[4 * P + id] = pw;
+    z.bv[5 * P + id] = t_wall;
+    z.cv[id] = rho_w;
+    z.cv[P + id] = 0;
+    z.cv[2 * P + id] = 0;
+    z.cv[3 * P + id] = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.cv[(5 + l) * P + id] = cvs[l];
+    z.cv[4 * P + id] = e;
+    z.vel[id] = vel;
     if constexpr (RANS) {
-      z.sv[NS * P + gi] = 0;
-      z.sv[(NS + 1) * P + gi] = z.sv[(NS + 1) * P + id];
-      z.mut[gi] = 0;
+      z.sv[NS * P + id] = 0;
+      z.cv[(NS + 5) * P + id] = 0;
+      z.sv[(NS + 1) * P + id] = om;
+      z.cv[(NS + 6) * P + id] = rho_w * om;
     }
+    return;
+  }
+  const int64_t si = z.idx(pt[0] - g * dir[0], pt[1] - g * dir[1], pt[2] - g * dir[2]);
+  const int64_t gi = z.idx(pt[0] + g * dir[0], pt[1] + g * dir[1], pt[2] + g * dir[2]);
+  const R ui = z.bv[P + si], vi = z.bv[2 * P + si], wi = z.bv[3 * P + si];
+  const R pi = z.bv[4 * P + si], ti = z.bv[5 * P + si];
+  R ys[NS > 0 ? NS : 1];
+  R mg = (R)kMwAir;
+  if constexpr (NS > 0) {
+    mg = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      ys[l] = z.sv[l * P + si];
+      mg += ys[l] * (R)(1.0 / p->mw[l]);
+    }
+    mg = (R)1 / mg;
+  }
+  R tg = ti;
+  if (isothermal) {
+    tg = (R)2 * t_wall - ti;
+    if (tg <= (R)0.1 * t_wall) tg = t_wall;
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) z.sv[l * P + gi] = ys[l];
+  z.bv[gi] = pi * mg / (tg * (R)kRu);
+  z.bv[P + gi] = -ui;
+  z.bv[2 * P + gi] = -vi;
+  z.bv[3 * P + gi] = -wi;
+  z.bv[4 * P + gi] = pi;
+  z.bv[5 * P + gi] = tg;
+  if constexpr (RANS) {
+    z.sv[NS * P + gi] = 0;
+    z.sv[(NS + 1) * P + gi] = om;
+    z.mut[gi] = 0;
   }
 }
 
@@ -314,11 +356,14 @@ __global__ void k_inner(ZoneView<R> z, ZoneView<R> t, const ostc_param *__restri
   }
 }
 
-dim3 face_grid(const FaceLaunch &f, dim3 &block) {
+// launch shape of face_layer: layers g0..ngg of every face point
+dim3 face_grid(const FaceLaunch &f, int ngg, dim3 &block) {
   const int a = f.face == 0 ? 1 : 0, b = f.face == 2 ? 1 : 2;
   const int na = f.re[a] - f.rs[a] + 1, nb = f.re[b] - f.rs[b] + 1;
+  const int nl = ngg + 1 - f.g0;
   block = dim3(64, 4, 1);
-  return dim3((na + 63) / 64, (nb + 3) / 4, 1);
+  if (f.face == 0) return dim3((na + 15) / 16, (nb + 3) / 4, 1);  // 4 lanes (layers) per point
+  return dim3((na + 63) / 64, (nb + 3) / 4, nl);
 }
 
 }  // namespace
@@ -335,8 +380,10 @@ int apply_bc(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, cons
     }
     f.face = bf.face;
     f.direction = bf.direction;
+    f.g0 = (bf.kind == 5 || bf.kind == 6) ? 1 : 0;  // inflow/outflow write only the ghost layers
+    if (z.g > 3) return OSTC_ERR_ARG;               // four lanes per point on i-faces
     dim3 block;
-    const dim3 grid = face_grid(f, block);
+    const dim3 grid = face_grid(f, z.g, block);
     const int nsc = o.ns + o.nt;
     const bool rans = o.nt > 0;
 #define OSTC_BC_SC(KERNEL, ...)                                                              \

@@ -661,12 +661,33 @@ __device__ __forceinline__ int newton_t(const ostc_param *__restrict__ p, const 
   return iter;
 }
 
-// update_cv_and_bv (src/FieldOperation.cuh:139-183)
 template <typename R, int NS, int NT>
+__device__ __forceinline__ bool limit_bad_flow(const ostc_param *__restrict__ p, const R *var) {
+  bool bad = false;
+#pragma unroll
+  for (int l = 0; l < 5; ++l)
+    if (isnan(var[l]) || var[l] < (R)p->limit_ll[l] || var[l] > (R)p->limit_ul[l]) bad = true;
+  return bad;
+}
+
+template <typename R>
+__device__ __forceinline__ bool limit_bad_turb(const ostc_param *__restrict__ p, R t0, R t1) {
+  return p->rans_model == 2 && (isnan(t0) || isnan(t1) || t0 < (R)0 || t1 < (R)0);
+}
+
+// update_cv_and_bv (src/FieldOperation.cuh:139-183).  With CHECK the cell is
+// also classified as limit_flow's first step does (src/TimeAdvanceFunc.cu:
+// 34-224) on the values just written, and a cell to repair is appended to
+// the zone's repair list for k_limit_repair.
+template <typename R, int NS, int NT, bool CHECK>
 __global__ __launch_bounds__(256) void k_update(ZoneView<R> z, const ostc_param *__restrict__ p,
-                                                const R *__restrict__ dqf) {
+                                                const R *__restrict__ dqf, int slot) {
   constexpr int NV = 5 + NS + NT;
   const auto c = cell_index(z, 0);
+  // the other list slot belongs to the previous time step, whose repairs are done
+  if (CHECK && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0 &&
+      threadIdx.z == 0)
+    z.lf_count[slot ^ 1] = 0;
   if (!c.ok) return;
   const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
   const R dtj = z.dt[id] / z.jac[id];
@@ -700,12 +721,25 @@ __global__ __launch_bounds__(256) void k_update(ZoneView<R> z, const ostc_param 
     double t = (double)z.bv[5 * P + id];
     const int it = newton_t<NS>(p, y, gas_const, e, t);
     if (it > 1000) atomicAdd(&z.counters[0], 1ull);
-    z.bv[5 * P + id] = (R)t;
-    z.bv[4 * P + id] = (R)((double)rho * t * gas_const);
+    const R tr = (R)t, pr = (R)((double)rho * t * gas_const);
+    z.bv[5 * P + id] = tr;
+    z.bv[4 * P + id] = pr;
+    if constexpr (CHECK) {
+      const R var[5] = {rho, u, v, w, pr};
+      bool bad = limit_bad_flow<R, NS, NT>(p, var);
+      if constexpr (NT > 0) bad = bad || limit_bad_turb<R>(p, cv[5 + NS] * rinv, cv[6 + NS] * rinv);
+      if (bad) z.lf_idx[atomicAdd(&z.lf_count[slot], 1)] = id;
+    }
   } else {
     const R pr = (R)(kGammaAir - 1) * (cv[4] - (R)0.5 * rho * vel2);
     z.bv[4 * P + id] = pr;
     z.bv[5 * P + id] = pr * (R)kMwAir * rinv / (R)kRu;
+    if constexpr (CHECK) {
+      const R var[5] = {rho, u, v, w, pr};
+      bool bad = limit_bad_flow<R, NS, NT>(p, var);
+      if constexpr (NT > 0) bad = bad || limit_bad_turb<R>(p, cv[5 + NS] * rinv, cv[6 + NS] * rinv);
+      if (bad) z.lf_idx[atomicAdd(&z.lf_count[slot], 1)] = id;
+    }
   }
   z.vel[id] = sqrt(vel2);
 }
@@ -720,26 +754,24 @@ __global__ __launch_bounds__(256) void k_update(ZoneView<R> z, const ostc_param 
 // mz is the reference's and is kept.
 //   pass 1: classify; for bad cells compute the final bv/sv/cv into a list
 //   pass 2: write the list back
+// Repaired state of cell (i, j, k) into out[6 + NSC + NV] (bv, sv, cv);
+// false if the cell needs no repair.  Reads the field only.
 template <typename R, int NS, int NT>
-__global__ __launch_bounds__(256) void k_limit_check(ZoneView<R> z, const ostc_param *__restrict__ p) {
-  constexpr int NSC = NS + NT, NV = 5 + NSC, NVAL = 6 + NSC + NV;
-  const auto c = cell_index(z, 0);
-  if (!c.ok) return;
+__device__ bool limit_fix_cell(const ZoneView<R> &z, const ostc_param *__restrict__ p, int ci, int cj, int ck,
+                               R *__restrict__ out) {
+  constexpr int NSC = NS + NT, NV = 5 + NSC;
+  struct {
+    int i, j, k;
+  } c{ci, cj, ck};
   const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
   const double *ll = p->limit_ll, *ul = p->limit_ul;
   R var[5];
-  bool bad = false;
 #pragma unroll
-  for (int l = 0; l < 5; ++l) {
-    var[l] = z.bv[l * P + id];
-    if (isnan(var[l]) || var[l] < (R)ll[l] || var[l] > (R)ul[l]) bad = true;
-  }
+  for (int l = 0; l < 5; ++l) var[l] = z.bv[l * P + id];
+  const bool bad = limit_bad_flow<R, NS, NT>(p, var);
   bool tbad = false;
-  if constexpr (NT > 0) {
-    const R t0 = z.sv[NS * P + id], t1 = z.sv[(NS + 1) * P + id];
-    tbad = p->rans_model == 2 && (isnan(t0) || isnan(t1) || t0 < (R)0 || t1 < (R)0);
-  }
-  if (!bad && !tbad) return;
+  if constexpr (NT > 0) tbad = limit_bad_turb<R>(p, z.sv[NS * P + id], z.sv[(NS + 1) * P + id]);
+  if (!bad && !tbad) return false;
   R bv[6], sv[NSC > 0 ? NSC : 1], cv[NV];
 #pragma unroll
   for (int l = 0; l < 6; ++l) bv[l] = z.bv[l * P + id];
@@ -856,21 +888,53 @@ __global__ __launch_bounds__(256) void k_limit_check(ZoneView<R> z, const ostc_p
       cv[NS + 6] = cv[0] * a1;
     }
   }
-  const int slot = atomicAdd(z.lf_count, 1);
-  z.lf_idx[slot] = id;
-  R *out = z.lf_vals + (int64_t)slot * NVAL;
 #pragma unroll
   for (int l = 0; l < 6; ++l) out[l] = bv[l];
 #pragma unroll
   for (int l = 0; l < NSC; ++l) out[6 + l] = sv[l];
 #pragma unroll
   for (int l = 0; l < NV; ++l) out[6 + NSC + l] = cv[l];
+  return true;
 }
 
+// pass 1 of the stand-alone limit_flow: classify every cell, repair into the list
 template <typename R, int NS, int NT>
-__global__ __launch_bounds__(256) void k_limit_apply(ZoneView<R> z) {
+__global__ __launch_bounds__(256) void k_limit_check(ZoneView<R> z, const ostc_param *__restrict__ p) {
   constexpr int NSC = NS + NT, NV = 5 + NSC, NVAL = 6 + NSC + NV;
-  const int n = *z.lf_count;
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  R vals[NVAL];
+  if (!limit_fix_cell<R, NS, NT>(z, p, c.i, c.j, c.k, vals)) return;
+  const int slot = atomicAdd(&z.lf_count[2], 1);
+  z.lf_idx[slot] = z.idx(c.i, c.j, c.k);
+  R *out = z.lf_vals + (int64_t)slot * NVAL;
+#pragma unroll
+  for (int l = 0; l < NVAL; ++l) out[l] = vals[l];
+}
+
+// pass 1 after a checking update: repair the listed cells (the update has
+// already classified every cell)
+template <typename R, int NS, int NT>
+__global__ __launch_bounds__(256) void k_limit_repair(ZoneView<R> z, const ostc_param *__restrict__ p, int slot) {
+  constexpr int NSC = NS + NT, NV = 5 + NSC, NVAL = 6 + NSC + NV;
+  const int n = z.lf_count[slot];
+  const int64_t nx = z.nx, ny = z.ny;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int64_t id = z.lf_idx[s];
+    const int i = (int)(id % nx) - z.g, j = (int)((id / nx) % ny) - z.g, k = (int)(id / (nx * ny)) - z.g;
+    R vals[NVAL];
+    limit_fix_cell<R, NS, NT>(z, p, i, j, k, vals);
+    R *out = z.lf_vals + (int64_t)s * NVAL;
+#pragma unroll
+    for (int l = 0; l < NVAL; ++l) out[l] = vals[l];
+  }
+}
+
+// pass 2: write the list of one slot back
+template <typename R, int NS, int NT>
+__global__ __launch_bounds__(256) void k_limit_apply(ZoneView<R> z, int slot) {
+  constexpr int NSC = NS + NT, NV = 5 + NSC, NVAL = 6 + NSC + NV;
+  const int n = z.lf_count[slot];
   const int64_t P = z.plane;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int64_t id = z.lf_idx[s];
@@ -1133,19 +1197,38 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
   return OSTC_OK;
 }
 
+// check_slot >= 0: also classify every cell for limit_flow into repair-list
+// slot check_slot (0 or 1, alternating between time steps; the update clears
+// the other slot), then call limit_flow with the same slot
 template <typename R>
-int update_cv_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, const R *dq_final, hipStream_t st) {
+int update_cv_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, const R *dq_final, hipStream_t st,
+                 int check_slot) {
   const auto g = cell_grid(z.mx, z.my, z.mz);
-  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_update<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp, dq_final));
+  if (check_slot >= 0)
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_update<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp, dq_final,
+                                             check_slot & 1));
+  else
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_update<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp, dq_final,
+                                             0));
   return last_err();
 }
 
+// limit_flow.  After a checking update (slot 0 or 1) only the listed cells
+// are repaired; otherwise (slot < 0) every cell is classified first, into
+// slot 2.
 template <typename R>
-int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st) {
-  const auto g = cell_grid(z.mx, z.my, z.mz);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, z.lf_count);
-  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_check<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp));
-  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_apply<R, NS_, NT_>), dim3(256), dim3(256), 0, st, z));
+int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, int slot) {
+  if (slot >= 0) {
+    slot &= 1;
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_repair<R, NS_, NT_>), dim3(256), dim3(256), 0, st, z, dp,
+                                             slot));
+  } else {
+    slot = 2;
+    const auto g = cell_grid(z.mx, z.my, z.mz);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, z.lf_count + 2);
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_check<R, NS_, NT_>), g.grid, g.block, 0, st, z, dp));
+  }
+  OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_limit_apply<R, NS_, NT_>), dim3(256), dim3(256), 0, st, z, slot));
   return last_err();
 }
 
@@ -1186,8 +1269,8 @@ int wall_distance(const ZoneView<R> &z, const double *wall_xyz, int n_wall, hipS
   template int implicit_treatment<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t,    \
                                      R **);                                                                     \
   template int update_cv_bv<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, const R *,            \
-                               hipStream_t);                                                                    \
-  template int limit_flow<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);           \
+                               hipStream_t, int);                                                               \
+  template int limit_flow<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t, int);      \
   template int store_last_step<R>(const ZoneView<R> &, hipStream_t);                                            \
   template int residual_partials<R>(const ZoneView<R> &, double *, hipStream_t, int *);                         \
   template int wall_distance<R>(const ZoneView<R> &, const double *, int, hipStream_t);

@@ -52,10 +52,14 @@ int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &
 // leaves the final increment in *result (one of z.dq/z.dq0/z.dqk)
 template <typename R>
 int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result);
+// check_slot >= 0: also classify every cell for limit_flow into repair-list
+// slot (check_slot & 1); then call limit_flow with the same slot.  Time steps
+// alternate the slot.
 template <typename R>
-int update_cv_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, const R *dq_final, hipStream_t st);
+int update_cv_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, const R *dq_final, hipStream_t st,
+                 int check_slot = -1);
 template <typename R>
-int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
+int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, int slot = -1);
 template <typename R>
 int store_last_step(const ZoneView<R> &z, hipStream_t st);
 // per-zone partial sums -> partial[nblocks*4]; returns nblocks
