@@ -55,6 +55,8 @@ def parse():
                     help="solver iteration at which surrogate and plain chemistry are compared")
     ap.add_argument("--chem-reps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-scale", action="store_true", help="skip the four-block 25 M-cell leg")
+    ap.add_argument("--scale-steps", type=int, default=5)
     ap.add_argument("--profile-json", default=None, help="also write per-stage timings here")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="instead: run the streaming-copy calibration kernel on N MiB and exit")
@@ -231,6 +233,91 @@ def chemistry_leg(steps: int, warmup: int, eval_iter: int, reps: int):
     return out
 
 
+def scale_leg(steps: int, warmup: int, reps: int, dims=(256, 192, 128), nblocks: int = 4):
+    """Full path at scale: the four-block SST + finite-rate H2-jet workload
+    (4 x 256x192x128 = 25.2 M cells, FP32 bulk + FP64 chemistry/Newton, EPI,
+    DPLUR x4, on-device inner-face halos).  Whole-iteration throughput, then
+    the chemistry step of every cell by the plain FP64 point-implicit
+    integrator vs the surrogate with fallback on the solver's state."""
+    import numpy as np
+
+    from openstc_amd import cases, surrogate as S
+    from openstc_amd import _lib as L
+    from openstc_amd.runtime import Simulation
+
+    mx, my, mz = dims
+    t0 = time.perf_counter()
+    case = cases.config5(mx=mx, my=my, mz=mz, nblocks=nblocks)
+    cells = sum(b.mx * b.my * b.mz for b in case.blocks)
+    print(f"[bench] {case.name}: {cells} cells, setup", file=sys.stderr, flush=True)
+    sim = Simulation(case)
+    for blk in case.blocks:  # host copies are no longer needed
+        blk.bv0 = blk.sv0 = blk.xg = blk.yg = blk.zg = blk.jac = blk.metric = None
+    sim.initialize_computation()
+    setup_s = time.perf_counter() - t0
+    lib = sim.ctx.lib
+    for _ in range(warmup):
+        sim.step(output_screen=case.output_screen)
+    sim.sync()
+    print(f"[bench] {case.name}: timing {steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(output_screen=case.output_screen)
+    sim.sync()
+    el = time.perf_counter() - t0
+    out = {"workload": case.name, "blocks": nblocks, "block": list(dims), "cells": cells, "steps": steps,
+           "warmup": warmup, "value": round(cells * steps / el / 1e6, 3), "unit": "Mcell-iter/s",
+           "ms_per_step": round(el / steps * 1e3, 3), "setup_s": round(setup_s, 1),
+           "model": "9-species/19-reaction H2-air finite-rate (EPI) + k-omega SST, DPLUR x4, CFL %g, "
+                    "FP32 bulk + FP64 chemistry/Newton" % case.param.cfl}
+    if not os.path.exists(S.DEFAULT_WEIGHTS):
+        out["surrogate"] = None
+        sim.close()
+        return out
+    sur = S.Surrogate.load(sim.ctx)
+    ncell = [b.mx * b.my * b.mz for b in case.blocks]
+    nmax = max(ncell)
+    dy_p = S.DeviceArray(nbytes=nmax * 9 * 8)
+    dy_s = S.DeviceArray(nbytes=nmax * 9 * 8)
+
+    def run(z, mode, buf, st):
+        L.check(lib.ostc_chem_zone_step(sim.ctx.h, z.h, sur.h, mode, buf.p, st, None), "zone chemistry")
+
+    times = {0: 0.0, 1: 0.0}
+    sq_err = sq_ref = 0.0
+    fallbacks = 0
+    for z, n in zip(sim.zones, ncell):
+        run(z, 0, dy_p, None)
+        run(z, 1, dy_s, None)
+        for mode, buf in ((0, dy_p), (1, dy_s)):
+            L.check(lib.ostc_device_sync(), "sync")
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run(z, mode, buf, None)
+            L.check(lib.ostc_device_sync(), "sync")
+            times[mode] += (time.perf_counter() - t0) / reps
+        stats = (C.c_longlong * 4)()
+        run(z, 1, dy_s, stats)
+        ref = dy_p.get(np.float64, (n, 9))
+        got = dy_s.get(np.float64, (n, 9))
+        sq_err += float(np.sum((got - ref) ** 2))
+        sq_ref += float(np.sum(ref ** 2))
+        fallbacks += int(stats[1] + stats[2] + stats[3])
+    out["surrogate"] = {
+        "plain_ms": round(times[0] * 1e3, 3), "surrogate_ms": round(times[1] * 1e3, 3),
+        "speedup": round(times[0] / times[1], 3),
+        "rel_l2": round(float(np.sqrt(sq_err / max(sq_ref, 1e-300))), 6),
+        "fallback_rate": round(fallbacks / cells, 5),
+        "note": "surrogate trained on the 192x96x96 mixing-layer states; this workload's vitiated-air / jet "
+                "states are partly outside that range and fall back to the plain integrator",
+    }
+    for b in (dy_p, dy_s):
+        b.free()
+    sur.close()
+    sim.close()
+    return out
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -329,6 +416,8 @@ def main():
     sim.close()
     if not a.no_chem:
         out["chemistry"] = chemistry_leg(a.chem_steps, 2, a.chem_eval_iter, a.chem_reps)
+    if not a.no_scale:
+        out["full_path_at_scale"] = scale_leg(a.scale_steps, 2, a.chem_reps)
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(build, a.cpu_steps)

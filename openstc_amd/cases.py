@@ -625,3 +625,118 @@ def config3(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=4, cf
                                           u=1.0, v=0.0, w=0.0, H2=1.0))])
     return Case("config3_h2air_fr_epi_dplur", param, [blk], ngg, precision, steps, 100, None, seed,
                 dict(species=names, air=air.__dict__, fuel=fuel.__dict__, mech_info=info, setup=setup))
+
+
+def config5(mx=256, my=192, mz=128, nblocks=4, steps=200, precision=L.FP32, dplur_steps=4, cfl=1.0, chem_method=1,
+            lx=0.4, ly=0.1, lz=0.05, seed=5) -> Case:
+    """Full path at scale: SST RANS + 9-species finite-rate chemistry on
+    `nblocks` blocks of mx x my x mz joined along i by inner faces (shared node
+    planes, the reference's inner{rank}.txt records).  Vitiated-air crossflow
+    (O2 0.258 / N2 0.486 / H2O 0.256 by mass, 1270 K, Ma 2.0, p = 1 atm)
+    enters at block 1's i = 0; an H2 jet (254 K, Ma 2.5, normal to the wall)
+    enters through a slot in the j = 0 wall of block 1; isothermal wall
+    (300 K) elsewhere at j = 0, outflow at j = max and the last block's
+    i = max, symmetry in k.  Initial field: the crossflow with a seeded
+    analytic H2 plume above the slot (tanh-blended ellipsoid, composition,
+    temperature and velocity blended, +-2 % perturbation).  EPI chemistry,
+    DPLUR 4 sweeps."""
+    ngg = 2
+    mech, info = load_mechanism(True)
+    names = species_names(info)
+    ns = mech.n_spec
+    y_air = np.zeros(ns)
+    for n, v in (("O2", 0.258), ("N2", 0.486), ("H2O", 0.256)):
+        y_air[names.index(n)] = v
+    y_fuel = np.zeros(ns)
+    y_fuel[names.index("H2")] = 1.0
+    p_inf, t_air, t_jet = 101325.0, 1270.0, 254.0
+    air = mixture_inflow(mech, y_air, 2.0, p_inf, t_air, rans=True, intensity=0.01, mut_ratio=1.0)
+    jet = mixture_inflow(mech, y_fuel, 2.5, p_inf, t_jet, direction=(0.0, 1.0, 0.0), rans=True, intensity=0.01,
+                         mut_ratio=1.0)
+    param = mixture_param(mech, rans=True, implicit=True, dplur_steps=dplur_steps, cfl=cfl, chem_method=chem_method,
+                          freestream=air)
+    nx_tot = nblocks * (mx - 1) + 1
+    dx, dy, dz = lx / (nx_tot - 1), ly / (my - 1), lz / (mz - 1)
+    # jet slot on block 1's wall: i in [i0, i1], k in [k0, k1]
+    i0, i1 = mx // 4, mx // 4 + max(1, mx // 16)
+    k0, k1 = mz // 2 - max(1, mz // 16), mz // 2 + max(1, mz // 16)
+    xj, zj = (0.5 * (i0 + i1)) * dx, (0.5 * (k0 + k1)) * dz
+    blocks = []
+    from .runtime import mesh_geometry, register_inner_face
+    for b in range(nblocks):
+        k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+        x = (i + b * (mx - 1)) * dx
+        blk = Block(mx, my, mz, x, j * dy, k * dz)
+        rows = []
+        if b == 0:
+            rows.append((0, 0, 0, my - 1, 0, mz - 1, 5))
+            rows += [(0, i0 - 1, 0, 0, 0, mz - 1, 2), (i1 + 1, mx - 1, 0, 0, 0, mz - 1, 2),
+                     (i0, i1, 0, 0, 0, k0 - 1, 2), (i0, i1, 0, 0, k1 + 1, mz - 1, 2),
+                     (i0, i1, 0, 0, k0, k1, 9)]
+        else:
+            rows.append((0, mx - 1, 0, 0, 0, mz - 1, 2))
+        if b == nblocks - 1:
+            rows.append((mx - 1, mx - 1, 0, my - 1, 0, mz - 1, 6))
+        rows += [(0, mx - 1, my - 1, my - 1, 0, mz - 1, 6),
+                 (0, mx - 1, 0, my - 1, 0, 0, 3), (0, mx - 1, 0, my - 1, mz - 1, mz - 1, 3)]
+        blk.rows = rows
+        blk.bc = bc_program(rows, [(5, "inflow"), (9, "inflow"), (2, "wall")], [air, jet], [(True, 300.0)], ngg)
+        # inner faces: i = max of block b <-> i = 1 of block b+1 (1-based records)
+        raw = []
+        if b > 0:
+            raw.append([1, 1, -1, -my, 1, mz, mx, mx, -1, -my, 1, mz, b])
+        if b < nblocks - 1:
+            raw.append([mx, mx, -1, -my, 1, mz, 1, 1, -1, -my, 1, mz, b + 2])
+        blk.inner_raw = raw
+        blk.inner = [register_inner_face(r, ngg) for r in raw]
+        blocks.append(blk)
+    for blk, geo in zip(blocks, mesh_geometry(blocks, ngg)):
+        blk.xg, blk.yg, blk.zg, blk.jac, blk.metric, _ = geo
+    mwk = np.array(mech.mw[:ns])
+    for b, blk in enumerate(blocks):
+        # seeded analytic plume: tanh-blended ellipsoid above the slot
+        r = np.sqrt(((blk.xg - xj) / (6 * (i1 - i0 + 1) * dx)) ** 2 + (blk.yg / (0.25 * ly)) ** 2 +
+                    ((blk.zg - zj) / (2 * (k1 - k0 + 1) * dz)) ** 2)
+        phi = 0.5 * (1.0 - np.tanh((r - 1.0) / 0.15))
+        shape = blk.xg.shape
+        sv = np.empty((ns + 2,) + shape)
+        for l in range(ns):
+            sv[l] = phi * y_fuel[l] + (1 - phi) * y_air[l]
+        sv[ns] = air.sv[ns]
+        sv[ns + 1] = air.sv[ns + 1]
+        bv = np.empty((6,) + shape)
+        bv[1] = (1 - phi) * air.u
+        bv[2] = phi * 0.3 * jet.v
+        bv[3] = 0.0
+        bv[4] = p_inf
+        bv[5] = phi * t_jet + (1 - phi) * t_air
+        s = seed * 16 + b
+        perturb_interior(bv[1], blk, s, 1, ngg, rel=0.02)
+        perturb_interior(bv[2], blk, s, 2, ngg, absval=0.02 * air.u)
+        perturb_interior(bv[3], blk, s, 3, ngg, absval=0.02 * air.u)
+        perturb_interior(bv[4], blk, s, 4, ngg, rel=0.02)
+        mw = 1.0 / np.einsum("l...,l->...", sv[:ns], 1.0 / mwk)
+        bv[0] = bv[4] * mw / (RU * bv[5])
+        perturb_interior(bv[0], blk, s, 0, ngg, rel=0.02)
+        bv[5] = bv[4] * mw / (RU * bv[0])
+        blk.bv0, blk.sv0 = bv, sv
+    # wall points: the wall faces in condition order, blocks in order, faces in file order
+    pts = []
+    for blk in blocks:
+        for r in blk.rows:
+            if r[6] != 2:
+                continue
+            sl = (slice(r[4], r[5] + 1), slice(r[2], r[3] + 1), slice(r[0], r[1] + 1))
+            pts.append(np.stack([blk.x[sl].ravel(), blk.y[sl].ravel(), blk.z[sl].ravel()], axis=1))
+    wall = np.concatenate(pts, axis=0)
+    setup = setup_spec(steps, 100, cfl, True, dplur_steps, 2, True, rans=True, species=True, reaction=True,
+                       chem_method=chem_method,
+                       bcs=[("freestream", dict(type="inflow", label=5, mach=2.0, pressure=p_inf, temperature=t_air,
+                                                u=1.0, v=0.0, w=0.0, O2=0.258, N2=0.486, H2O=0.256,
+                                                turbulence_intensity=0.01, turb_viscosity_ratio=1.0)),
+                            ("jet", dict(type="inflow", label=9, mach=2.5, pressure=p_inf, temperature=t_jet,
+                                         u=0.0, v=1.0, w=0.0, H2=1.0, turbulence_intensity=0.01,
+                                         turb_viscosity_ratio=1.0)),
+                            ("wall", dict(type="wall", label=2, thermal_type="isothermal", temperature=300.0))])
+    return Case(f"config5_h2jet_vitiated_fr_sst_{nblocks}x{mx}x{my}x{mz}", param, blocks, ngg, precision, steps, 100,
+                wall, seed, dict(species=names, air=air.__dict__, jet=jet.__dict__, mech_info=info, setup=setup))

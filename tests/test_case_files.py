@@ -53,6 +53,7 @@ BUILDERS = {
     "config2": lambda: cases.config2(mx=24, my=20, mz=8, steps=3),
     "config3": lambda: cases.config3(mx=16, my=12, mz=8, steps=3),
     "config1_split": lambda: cases.config1_split(mx=20, my=16, mz=8, steps=3),
+    "config5": lambda: cases.config5(mx=16, my=12, mz=10, nblocks=4, steps=3),
 }
 
 
@@ -229,7 +230,8 @@ def inject(cd, case):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,precision,steps", [("config1", L.FP64, 5), ("config2", L.FP32, 3),
-                                                   ("config3", L.FP32, 2), ("config1_split", L.FP64, 3)])
+                                                   ("config3", L.FP32, 2), ("config1_split", L.FP64, 3),
+                                                   ("config5", L.FP32, 2)])
 def test_driver_matches_python_loop(tmp_path, name, precision, steps):
     from openstc_amd.runtime import Simulation
     print(f"[gpu-driver] {name} precision {precision} steps {steps}", flush=True)
