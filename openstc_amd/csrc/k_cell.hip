@@ -237,16 +237,16 @@ __global__ __launch_bounds__(256) void k_source(ZoneView<R> z, const ostc_param 
     const int meth = p->chem_src_method;
     if (meth == 1) {
       double jac[NS * NS];
-      chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, t, y, omega, jac, nullptr);
+      chem_source<NS, kMaxReac, 1>(p, NS, p->n_reac, rho, t, y, omega, jac);
 #pragma unroll
       for (int q = 0; q < NS * NS; ++q) z.chem_jac[q * P + id] = jac[q];
     } else if (meth == 2) {
       double jd[NS];
-      chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, t, y, omega, nullptr, jd);
+      chem_source<NS, kMaxReac, 2>(p, NS, p->n_reac, rho, t, y, omega, jd);
 #pragma unroll
       for (int q = 0; q < NS; ++q) z.chem_jac[q * P + id] = jd[q];
     } else {
-      chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, t, y, omega, nullptr, nullptr);
+      chem_source<NS, kMaxReac, 0>(p, NS, p->n_reac, rho, t, y, omega, nullptr);
     }
 #pragma unroll
     for (int l = 0; l < NS; ++l) dq[5 + l] = (R)((double)J * omega[l]);

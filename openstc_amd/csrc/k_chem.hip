@@ -34,7 +34,10 @@ __global__ __launch_bounds__(256) void k_source_batch(const ostc_param *__restri
   double yy[NS], om[NS], a[NS * NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) yy[l] = y[c * NS + l];
-  chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho[c], t[c], yy, om, jac ? a : nullptr, nullptr);
+  if (jac)
+    chem_source<NS, kMaxReac, 1>(p, NS, p->n_reac, rho[c], t[c], yy, om, a);
+  else
+    chem_source<NS, kMaxReac, 0>(p, NS, p->n_reac, rho[c], t[c], yy, om, nullptr);
 #pragma unroll
   for (int l = 0; l < NS; ++l) omega[c * NS + l] = om[l];
   if (jac)

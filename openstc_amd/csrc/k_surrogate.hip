@@ -178,19 +178,30 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
   const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63, g = ln >> 4, cn = ln & 15;
   const long n = src.n;
   const long stride = (long)gridDim.x * kWaves * 64;
+  // The inputs of a pass are loaded once (features, then the checks) and the
+  // next pass's inputs are requested before this pass's layers run, so the
+  // global-memory latency overlaps the matrix work.
+  double nT = 0, nP = 0, nDT = 0, nY[kNs];
+  {
+    const long c0 = ((long)blockIdx.x * kWaves + wave) * 64 + ln;
+    if (c0 < n) src.load(c0, nT, nP, nY, nDT);
+  }
   for (long base = ((long)blockIdx.x * kWaves + wave) * 64; base < n; base += stride) {
     float *stg = sStage[wave];
+    double T = nT, P = nP, DT = nDT, Y[kNs];
+#pragma unroll
+    for (int k = 0; k < kNs; ++k) Y[k] = nY[k];
+    {
+      const long cn_ = base + stride + ln;
+      if (cn_ < n) src.load(cn_, nT, nP, nY, nDT);
+    }
     // ---- features, one lane per cell, staged through LDS
     {
       const long c = base + ln;
       float f[kSurIn];
 #pragma unroll
       for (int q = 0; q < kSurIn; ++q) f[q] = 0.0f;
-      if (c < n) {
-        double T, P, DT, Y[kNs];
-        src.load(c, T, P, Y, DT);
-        features(s, T, P, Y, DT, f);
-      }
+      if (c < n) features(s, T, P, Y, DT, f);
 #pragma unroll
       for (int q = 0; q < kSurIn; ++q) stg[ln * kStageLd + q] = f[q];
     }
@@ -323,8 +334,6 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
     const long c = base + ln;
     int why = 0;  // 0 accepted, 1 input range, 2 output range, 3 element balance
     if (c < n) {
-      double T, P, DT, Y[kNs];
-      src.load(c, T, P, Y, DT);
       float o[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) o[q] = stg[ln * kStageLd + q];
@@ -364,7 +373,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
 // Plain point-implicit step on the fallback list (grid-stride over a count
 // that lives on the device).
 template <typename Src>
-__global__ __launch_bounds__(256) void k_sur_fallback(const ostc_param *__restrict__ p, Src src,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_sur_fallback(
+    const ostc_param *__restrict__ p, Src src,
                                                       const int *__restrict__ count, const long *__restrict__ list,
                                                       double *__restrict__ dyo) {
   const int m = *count;
@@ -589,8 +599,11 @@ static int infer_impl(const SurState &st, const ostc_param *dp, const Src &src, 
   InferOut io{dy, st.fb_count, st.fb_list, st.stats, raw};
   hipLaunchKernelGGL((k_sur_forward<Src, false>), dim3(forward_grid(src.n, st.cus)), dim3(kWaves * 64), 0, s,
                      st.params, src, reinterpret_cast<const bf16x8 *>(st.packed), st.bias, tr, io);
-  const long fb_blocks = std::min<long>((src.n + 255) / 256, 4L * st.cus);
-  hipLaunchKernelGGL(k_sur_fallback<Src>, dim3((unsigned)std::max<long>(1, fb_blocks)), dim3(256), 0, s, dp, src,
+  // one wave per workgroup (the kernel's launch bound), enough of them to
+  // spread a typical fallback list over every CU; the kernel strides over
+  // the device-side count
+  const long fb_blocks = std::min<long>((src.n + 63) / 64, 16L * st.cus);
+  hipLaunchKernelGGL(k_sur_fallback<Src>, dim3((unsigned)std::max<long>(1, fb_blocks)), dim3(64), 0, s, dp, src,
                      st.fb_count, st.fb_list, dy);
   return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;
 }

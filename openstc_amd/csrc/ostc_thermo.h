@@ -229,10 +229,16 @@ __device__ __forceinline__ void reaction_rates(const ostc_param *__restrict__ p,
 // each reaction's q1/q2 is formed and immediately added to the production
 // sums and the Jacobian (same summation order over reactions as the
 // reference), so no per-reaction array is kept and every per-species array
-// has compile-time indices (registers, no scratch).
-template <int NS, int NR>
+// has compile-time indices (registers, no scratch).  JAC selects the
+// Jacobian at compile time: 0 none, 1 EPI (jac = NS*NS matrix), 2 DA
+// diagonal (jac = NS values).  (A run-time null test on a pointer to a local
+// array would keep that array in private memory: in the private address
+// space null is a valid address, so the comparison cannot be folded.)
+template <int NS, int NR, int JAC>
 __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, int ns, int nr, double rho, double t,
-                                            const double *y, double *omega, double *jac_epi, double *jac_da) {
+                                            const double *y, double *omega, double *jac) {
+  double *jac_epi = JAC == 1 ? jac : nullptr;
+  double *jac_da = JAC == 2 ? jac : nullptr;
   (void)ns;
   double c[NS], g[NS];
 #pragma unroll
@@ -245,7 +251,7 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
   double *zz = jac_epi;
 #pragma unroll
   for (int l = 0; l < NS; ++l) creation[l] = od[l] = 0;
-  if (jac_epi) {
+  if constexpr (JAC == 1) {
 #pragma unroll
     for (int q = 0; q < NS * NS; ++q) zz[q] = 0;
   }
@@ -263,7 +269,7 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
       creation[i] += q2 * nf[i] + q1 * nb[i];
       od[i] += q1 * nf[i] + q2 * nb[i];
     }
-    if (jac_epi) {
+    if constexpr (JAC == 1) {
 #pragma unroll
       for (int m = 0; m < NS; ++m) {
         const int dm = nb[m] - nf[m];
@@ -278,9 +284,9 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
   for (int i = 0; i < NS; ++i) {
     const double cr = creation[i] * (p->mw[i] * 1e+3), d = od[i] * (p->mw[i] * 1e+3);
     omega[i] = cr - d;
-    if (jac_da) jac_da[i] = y[i] > 1e-25 ? -d / (y[i] * rho) : 0.0;
+    if constexpr (JAC == 2) jac_da[i] = y[i] > 1e-25 ? -d / (y[i] * rho) : 0.0;
   }
-  if (jac_epi) {
+  if constexpr (JAC == 1) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const bool on = y[n] > 1e-25;
@@ -380,7 +386,7 @@ __device__ __forceinline__ void plain_chem_step(const ostc_param *__restrict__ p
   const double rho = pc * mw / (kRu * tc);
   double om[NS], a[NS * NS];
   int piv[NS];
-  chem_source<NS, kMaxReac>(p, NS, p->n_reac, rho, tc, yy, om, a, nullptr);
+  chem_source<NS, kMaxReac, 1>(p, NS, p->n_reac, rho, tc, yy, om, a);
 #pragma unroll
   for (int q = 0; q < NS * NS; ++q) a[q] = -dtc * a[q];
 #pragma unroll
