@@ -35,9 +35,16 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kNs = 9;
-constexpr int kWaves = 8;              // waves per workgroup of the fused kernel
+constexpr int kWaves = 12;             // waves per workgroup of the fused kernel (3 per SIMD)
 constexpr int kTiles = 4;              // 16-cell slabs per wave pass (64 cells)
-constexpr int kStageLd = 17;           // padded LDS row of the output staging
+constexpr int kStageLd = 9;            // padded LDS staging row: 8 values per cell
+
+// LDS hand-off between the lanes of one wave
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -178,53 +185,53 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
   const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63, g = ln >> 4, cn = ln & 15;
   const long n = src.n;
   const long stride = (long)gridDim.x * kWaves * 64;
-  // The inputs of a pass are loaded once (features, then the checks) and the
-  // next pass's inputs are requested before this pass's layers run, so the
-  // global-memory latency overlaps the matrix work.
-  double nT = 0, nP = 0, nDT = 0, nY[kNs];
-  {
-    const long c0 = ((long)blockIdx.x * kWaves + wave) * 64 + ln;
-    if (c0 < n) src.load(c0, nT, nP, nY, nDT);
-  }
   for (long base = ((long)blockIdx.x * kWaves + wave) * 64; base < n; base += stride) {
     float *stg = sStage[wave];
-    double T = nT, P = nP, DT = nDT, Y[kNs];
-#pragma unroll
-    for (int k = 0; k < kNs; ++k) Y[k] = nY[k];
-    {
-      const long cn_ = base + stride + ln;
-      if (cn_ < n) src.load(cn_, nT, nP, nY, nDT);
-    }
-    // ---- features, one lane per cell, staged through LDS
+    // ---- features, one lane per cell, staged through LDS in two halves
+    // (rows hold 8 values: features 0..7, then 8..11)
+    float f[kSurIn];
     {
       const long c = base + ln;
-      float f[kSurIn];
 #pragma unroll
       for (int q = 0; q < kSurIn; ++q) f[q] = 0.0f;
-      if (c < n) features(s, T, P, Y, DT, f);
+      if (c < n) {
+        double T, P, DT, Y[kNs];
+        src.load(c, T, P, Y, DT);
+        features(s, T, P, Y, DT, f);
+      }
 #pragma unroll
-      for (int q = 0; q < kSurIn; ++q) stg[ln * kStageLd + q] = f[q];
+      for (int q = 0; q < 8; ++q) stg[ln * kStageLd + q] = f[q];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
     // ---- layer 0 operand: slots 8g..8g+7 of the 32-slot chunk = features
+    // (g = 0: features 0..7, g = 1: features 8..11 then zeros, g > 1: zeros)
     bf16x8 b0[kTiles];
 #pragma unroll
     for (int t = 0; t < kTiles; ++t) {
       const float *row = stg + (16 * t + cn) * kStageLd;
-      float f8[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int q = 8 * g + j;
-        f8[j] = q < kSurIn ? row[q < kSurIn ? q : 0] : 0.0f;
-      }
       uint4 u;
-      u.x = pack_bf16(f8[0], f8[1]);
-      u.y = pack_bf16(f8[2], f8[3]);
-      u.z = pack_bf16(f8[4], f8[5]);
-      u.w = pack_bf16(f8[6], f8[7]);
+      u.x = g == 0 ? pack_bf16(row[0], row[1]) : 0u;
+      u.y = g == 0 ? pack_bf16(row[2], row[3]) : 0u;
+      u.z = g == 0 ? pack_bf16(row[4], row[5]) : 0u;
+      u.w = g == 0 ? pack_bf16(row[6], row[7]) : 0u;
       b0[t] = __builtin_bit_cast(bf16x8, u);
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < kSurIn - 8; ++q) stg[ln * kStageLd + q] = f[8 + q];
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+      const float *row = stg + (16 * t + cn) * kStageLd;
+      if (g == 1) {
+        uint4 u;
+        u.x = pack_bf16(row[0], row[1]);
+        u.y = pack_bf16(row[2], row[3]);
+        u.z = 0u;
+        u.w = 0u;
+        b0[t] = __builtin_bit_cast(bf16x8, u);
+      }
+
       const long c = base + 16 * t + cn;
       if (TRAIN && c < n && g < 2) {
         float4 *dst = reinterpret_cast<float4 *>(tr.a0 + c * 16 + 8 * g);
@@ -312,31 +319,37 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
       for (int t = 0; t < kTiles; ++t)
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[ki][t], ki == 0 ? bo : acc[t], 0, 0, 0);
     }
+    if (TRAIN) {
 #pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-      const long c = base + 16 * t + cn;
-      float o[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) o[i] = acc[t][i];
-      if (TRAIN) {
-        if (c < n) *reinterpret_cast<float4 *>(tr.yout + c * 16 + 4 * g) = make_float4(o[0], o[1], o[2], o[3]);
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) stg[(16 * t + cn) * kStageLd + 4 * g + i] = o[i];
+      for (int t = 0; t < kTiles; ++t) {
+        const long c = base + 16 * t + cn;
+        if (c < n)
+          *reinterpret_cast<float4 *>(tr.yout + c * 16 + 4 * g) =
+              make_float4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
       }
+      continue;
     }
-    if (TRAIN) continue;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // outputs to one lane per cell, in two halves: rows 0..7 (g < 2), 8..15
+    float o[16];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      wave_sync();
+#pragma unroll
+      for (int t = 0; t < kTiles; ++t)
+        if ((g >> 1) == half)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) stg[(16 * t + cn) * kStageLd + 4 * (g & 1) + i] = acc[t][i];
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < 8; ++q) o[8 * half + q] = stg[ln * kStageLd + q];
+    }
 
     // ---- decode + checks: one lane per cell
     const long c = base + ln;
     int why = 0;  // 0 accepted, 1 input range, 2 output range, 3 element balance
     if (c < n) {
-      float o[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) o[q] = stg[ln * kStageLd + q];
+      double T, P, DT, Y[kNs];
+      src.load(c, T, P, Y, DT);
       if (io.raw) {
 #pragma unroll
         for (int q = 0; q < 16; q += 4)
