@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE on this workload (FETCH_SIZE scaled x2.0 and
 # WRITE_SIZE x1.0, the factors a 512 MiB copy kernel gives).  Keyed by
 # (workload, stage); the summary with the commands is
-# profiles/kernels_rans_256x128x64_and_fr_192x96x96.md.  Re-measured whenever
+# profiles/kernels_rans_256x128x64_fr_192x96x96_fourblock_4x256x192x128.md.  Re-measured whenever
 # the kernel changes; a stage without an entry reports null.
 MEASURED_TRAFFIC = {
     ("config2_air_sst_dplur", "dplur_sweep"): 496.6e6,
