@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--chem-eval-iter", type=int, default=200,
                     help="solver iteration at which surrogate and plain chemistry are compared")
     ap.add_argument("--chem-reps", type=int, default=5)
+    ap.add_argument("--surrogate", default=None, help="surrogate weights file (default: the packaged model)")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-scale", action="store_true", help="skip the four-block 25 M-cell leg")
     ap.add_argument("--scale-steps", type=int, default=5)
@@ -153,7 +154,7 @@ def cpu_baseline(case_builder, steps: int):
 SUR_FLOP_PER_CELL = 2 * (12 * 128 + 3 * 128 * 128 + 128 * 10)  # algorithmic MLP forward FLOP per cell
 
 
-def chemistry_leg(steps: int, warmup: int, eval_iter: int, reps: int):
+def chemistry_leg(steps: int, warmup: int, eval_iter: int, reps: int, weights: str | None = None):
     """Finite-rate workload (192x96x96 mixing layer, FP32 bulk + FP64
     chemistry, DPLUR x4, CFL 0.2): whole-iteration throughput, then, on the
     solver's own state at iteration `eval_iter`, the chemistry step of every
@@ -187,11 +188,12 @@ def chemistry_leg(steps: int, warmup: int, eval_iter: int, reps: int):
         sim.step(output_screen=case.output_screen)
         done += 1
     sim.sync()
-    if not os.path.exists(S.DEFAULT_WEIGHTS):
+    weights = weights or S.DEFAULT_WEIGHTS
+    if not os.path.exists(weights):
         out["surrogate"] = None
         sim.close()
         return out
-    sur = S.Surrogate.load(sim.ctx)
+    sur = S.Surrogate.load(sim.ctx, weights)
     z = sim.zones[0]
     dy_p = S.DeviceArray(nbytes=cells * 9 * 8)
     dy_s = S.DeviceArray(nbytes=cells * 9 * 8)
@@ -233,7 +235,8 @@ def chemistry_leg(steps: int, warmup: int, eval_iter: int, reps: int):
     return out
 
 
-def scale_leg(steps: int, warmup: int, reps: int, dims=(256, 192, 128), nblocks: int = 4):
+def scale_leg(steps: int, warmup: int, reps: int, dims=(256, 192, 128), nblocks: int = 4,
+              weights: str | None = None):
     """Full path at scale: the four-block SST + finite-rate H2-jet workload
     (4 x 256x192x128 = 25.2 M cells, FP32 bulk + FP64 chemistry/Newton, EPI,
     DPLUR x4, on-device inner-face halos).  Whole-iteration throughput, then
@@ -270,11 +273,12 @@ def scale_leg(steps: int, warmup: int, reps: int, dims=(256, 192, 128), nblocks:
            "ms_per_step": round(el / steps * 1e3, 3), "setup_s": round(setup_s, 1),
            "model": "9-species/19-reaction H2-air finite-rate (EPI) + k-omega SST, DPLUR x4, CFL %g, "
                     "FP32 bulk + FP64 chemistry/Newton" % case.param.cfl}
-    if not os.path.exists(S.DEFAULT_WEIGHTS):
+    weights = weights or S.DEFAULT_WEIGHTS
+    if not os.path.exists(weights):
         out["surrogate"] = None
         sim.close()
         return out
-    sur = S.Surrogate.load(sim.ctx)
+    sur = S.Surrogate.load(sim.ctx, weights)
     ncell = [b.mx * b.my * b.mz for b in case.blocks]
     nmax = max(ncell)
     dy_p = S.DeviceArray(nbytes=nmax * 9 * 8)
@@ -415,9 +419,9 @@ def main():
     }
     sim.close()
     if not a.no_chem:
-        out["chemistry"] = chemistry_leg(a.chem_steps, 2, a.chem_eval_iter, a.chem_reps)
+        out["chemistry"] = chemistry_leg(a.chem_steps, 2, a.chem_eval_iter, a.chem_reps, a.surrogate)
     if not a.no_scale:
-        out["full_path_at_scale"] = scale_leg(a.scale_steps, 2, a.chem_reps)
+        out["full_path_at_scale"] = scale_leg(a.scale_steps, 2, a.chem_reps, weights=a.surrogate)
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(build, a.cpu_steps)

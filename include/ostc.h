@@ -335,7 +335,7 @@ int ostc_chem_zone_step(ostc_ctx *ctx, ostc_zone *zone, ostc_surrogate *sur, int
                         long long stats[4], void *stream);
 
 /* ---- learned chemistry surrogate (no counterpart in the reference) ----
- * MLP 12-128-128-128-128-n_out, GELU (tanh form), on the features
+ * MLP 12-128-128-128-128-n_out on the features
  * (1000/T, ln(p/1 atm), BoxCox_0.1(Y_k) for the 9 species, ln dt),
  * standardised with in_mean/in_std; its standardised outputs (out_mean,
  * out_std) encode the plain step's dY: kind 0 (n_out 10): log10 of
@@ -347,12 +347,16 @@ int ostc_chem_zone_step(ostc_ctx *ctx, ostc_zone *zone, ostc_surrogate *sur, int
  * exceeds dy_max (the output range the network is trusted with), or when an
  * element balance |sum_k elem_w[j][k] dY_k| exceeds
  * elem_tol * sum_jk |elem_w[j][k] dY_k| + elem_abs (elem_w = atoms/W_k).
+ * Hidden activation: activation 0 = GELU (tanh form), 1 = hard-swish
+ * z*clamp(z/6 + 1/2, 0, 1) (no transcendental: the cheaper inference).
  * Master weights are FP32, per layer W[out][in] row-major then b[out]. */
+#define OSTC_SUR_GELU 0
+#define OSTC_SUR_HSWISH 1
 #define OSTC_SUR_IN 12
 #define OSTC_SUR_MAX_OUT 16
 #define OSTC_SUR_MAX_ELEM 4
 typedef struct ostc_surrogate_desc {
-  int n_out, kind, n_elem, reserved;
+  int n_out, kind, n_elem, activation;
   float in_mean[OSTC_SUR_IN], in_std[OSTC_SUR_IN];
   float out_mean[OSTC_SUR_MAX_OUT], out_std[OSTC_SUR_MAX_OUT];
   float loss_w[OSTC_SUR_MAX_OUT];             /* weights of the training loss */

@@ -106,7 +106,7 @@ SUR_IN, SUR_MAX_OUT, SUR_MAX_ELEM = 12, 16, 4
 
 class SurDesc(C.Structure):
     """struct ostc_surrogate_desc."""
-    _fields_ = [("n_out", C.c_int), ("kind", C.c_int), ("n_elem", C.c_int), ("reserved", C.c_int),
+    _fields_ = [("n_out", C.c_int), ("kind", C.c_int), ("n_elem", C.c_int), ("activation", C.c_int),
                 ("in_mean", C.c_float * SUR_IN), ("in_std", C.c_float * SUR_IN),
                 ("out_mean", C.c_float * SUR_MAX_OUT), ("out_std", C.c_float * SUR_MAX_OUT),
                 ("loss_w", C.c_float * SUR_MAX_OUT), ("elem_w", C.c_float * (SUR_MAX_ELEM * 9)),

@@ -724,6 +724,8 @@ int sur_check_desc(const ostc_surrogate_desc &d) {
   if (d.kind == 1 && d.n_out != 9) return fail(OSTC_ERR_ARG, "surrogate kind 1 has 9 outputs");
   if (d.kind < 0 || d.kind > 1) return fail(OSTC_ERR_ARG, "surrogate kind must be 0 or 1");
   if (d.n_elem < 0 || d.n_elem > OSTC_SUR_MAX_ELEM) return fail(OSTC_ERR_ARG, "n_elem out of range");
+  if (d.activation != OSTC_SUR_GELU && d.activation != OSTC_SUR_HSWISH)
+    return fail(OSTC_ERR_ARG, "surrogate activation must be 0 (GELU) or 1 (hard-swish)");
   if (!(d.dy_max > 0)) return fail(OSTC_ERR_ARG, "dy_max must be positive");
   for (int f = 0; f < OSTC_SUR_IN; ++f)
     if (!(d.in_std[f] > 0)) return fail(OSTC_ERR_ARG, "in_std must be positive");
@@ -735,6 +737,7 @@ ostc::SurParams sur_params(const ostc_surrogate_desc &d) {
   p.n_out = d.n_out;
   p.kind = d.kind;
   p.n_elem = d.n_elem;
+  p.act = d.activation;
   for (int f = 0; f < OSTC_SUR_IN; ++f) {
     p.in_mean[f] = d.in_mean[f];
     p.in_rstd[f] = (float)(1.0 / (double)d.in_std[f]);
@@ -755,23 +758,31 @@ ostc::SurParams sur_params(const ostc_surrogate_desc &d) {
 int sur_ensure_fallback(ostc_surrogate *s, long n) {
   if (n <= s->st.fb_cap) return OSTC_OK;
   HIP_TRY(hipDeviceSynchronize());
-  if (s->st.fb_list) {
+  for (void *buf : {(void *)s->st.fb_list, (void *)s->st.out12}) {
+    if (!buf) continue;
     for (auto &q : s->owned)
-      if (q == s->st.fb_list) q = nullptr;
-    HIP_TRY(hipFree(s->st.fb_list));
-    s->st.fb_list = nullptr;
+      if (q == buf) q = nullptr;
+    HIP_TRY(hipFree(buf));
   }
-  HIP_TRY(sur_alloc(s, &s->st.fb_list, (size_t)n));
+  s->st.fb_list = nullptr;
+  s->st.out12 = nullptr;
+  s->st.fb_cap = 0;
+  HIP_TRY(sur_alloc(s, &s->st.fb_list, (size_t)ostc::kFbLists * ostc::fb_list_cap(n)));
+  HIP_TRY(sur_alloc(s, &s->st.out12, (size_t)n * 12));
   s->st.fb_cap = n;
   return OSTC_OK;
 }
 
 int sur_read_stats(ostc_surrogate *s, long long stats[4], hipStream_t st) {
   if (!stats) return OSTC_OK;
-  unsigned long long h[4];
+  unsigned long long h[ostc::kFbLists * ostc::kStatStride];
   HIP_TRY(hipMemcpyAsync(h, s->st.stats, sizeof(h), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  for (int q = 0; q < 4; ++q) stats[q] = (long long)h[q];
+  for (int q = 0; q < 4; ++q) {
+    unsigned long long v = 0;
+    for (int l = 0; l < ostc::kFbLists; ++l) v += h[l * ostc::kStatStride + q];
+    stats[q] = (long long)v;
+  }
   return OSTC_OK;
 }
 
@@ -814,8 +825,8 @@ int ostc_surrogate_create(ostc_ctx *ctx, const ostc_surrogate_desc *desc, const 
   S.packed = pk;
   A(&S.bias, (size_t)ostc::kSurBias);
   A(&S.loss_w, 16);
-  A(&S.fb_count, 1);
-  A(&S.stats, 4);
+  A(&S.fb_count, (size_t)ostc::kFbLists * ostc::kFbStride);
+  A(&S.stats, (size_t)ostc::kFbLists * ostc::kStatStride);
   if (max_batch > 0) {
     const size_t B = (size_t)max_batch;
     A(&S.a0, B * 16);

@@ -1,6 +1,10 @@
 // Learned chemistry surrogate (no counterpart in the reference): a
-// 12-128-128-128-128-n_out MLP with GELU that maps a chemistry state
-// (T, p, Y_k, dt) to the plain point-implicit step's dY_k.
+// 12-128-128-128-128-n_out MLP that maps a chemistry state (T, p, Y_k, dt) to
+// the plain point-implicit step's dY_k.  Hidden activation: GELU (tanh form)
+// or hard-swish z*clamp(z/6 + 1/2, 0, 1), a compile-time parameter of the
+// kernels chosen by the model's descriptor; hard-swish is two VALU operations
+// per unit against GELU's four plus two transcendentals, which dominate the
+// non-MFMA work of the forward pass.
 //
 // Inference: BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.
 // The network is evaluated in the transposed form Y^T = W X^T: a weight tile
@@ -11,9 +15,13 @@
 // inside each 32-wide k chunk is permuted consistently in the packed weights,
 // so activations never leave registers between layers.
 //
-// Cells that fail the input-range, output-range or element-balance checks are
-// appended to a list and re-done by the FP64 plain integrator in a second
-// kernel (no serialisation inside a wavefront).
+// The forward kernel writes the standardised outputs straight from the MFMA
+// accumulators; a second, high-occupancy kernel (one thread per cell) decodes
+// them, runs the checks and stores dY.  In the fused form the decode's global
+// loads, stores and list atomics stalled the low-occupancy MLP waves for as
+// long as the MLP itself.  Cells that fail the input-range, output-range or
+// element-balance checks are appended to a list and re-done by the FP64 plain
+// integrator in a third kernel (no serialisation inside a wavefront).
 //
 // Training: the same forward kernel (saving pre-activations and BF16-rounded
 // activations), a weighted-MSE gradient kernel, FP32-MFMA
@@ -64,6 +72,31 @@ __device__ __forceinline__ float gelu_grad_f(float z) {
   return s + z * s * (1.0f - s) * kGeluC * (1.0f + 3.0f * kGeluA * z * z);
 }
 
+// hard-swish: the gate clamp(z/6 + 1/2, 0, 1) is one FMA with the clamp
+// output modifier
+constexpr float kSixth = 1.0f / 6.0f;
+__device__ __forceinline__ float hswish_f(float z) {
+  return z * __builtin_amdgcn_fmed3f(fmaf(z, kSixth, 0.5f), 0.0f, 1.0f);
+}
+__device__ __forceinline__ float hswish_grad_f(float z) {
+  return z <= -3.0f ? 0.0f : (z >= 3.0f ? 1.0f : fmaf(z, 2.0f * kSixth, 0.5f));
+}
+
+template <int ACT>
+__device__ __forceinline__ float act_f(float z) {
+  if constexpr (ACT == OSTC_SUR_HSWISH)
+    return hswish_f(z);
+  else
+    return gelu_f(z);
+}
+template <int ACT>
+__device__ __forceinline__ float act_grad_f(float z) {
+  if constexpr (ACT == OSTC_SUR_HSWISH)
+    return hswish_grad_f(z);
+  else
+    return gelu_grad_f(z);
+}
+
 // two FP32 values -> packed BF16 pair (round to nearest even)
 __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
   typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -74,18 +107,37 @@ __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
 __device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
 
 // ---- sources of chemistry states ----------------------------------------
+// Each source reads one cell's state as stored (Raw: FP32 or FP64) and
+// widens it to FP64 for the features and the checks.
+template <typename Raw>
+__device__ __forceinline__ void unpack_raw(const Raw &r, double &T, double &P, double *Y, double &DT) {
+  T = (double)r.t;
+  P = (double)r.p;
+  DT = (double)r.dt;
+#pragma unroll
+  for (int k = 0; k < kNs; ++k) Y[k] = (double)r.y[k];
+}
+
 struct BatchSrc {
   const double *t, *p, *y, *dt;
   const int *idx;  // optional gather (training)
   long n;
+  struct Raw {
+    double t, p, dt, y[kNs];
+  };
+  __device__ void load_raw(long c, Raw &r) const {
+    const long q = row(c);
+    r.t = t[q];
+    r.p = p[q];
+    r.dt = dt[q];
+#pragma unroll
+    for (int k = 0; k < kNs; ++k) r.y[k] = y[q * kNs + k];
+  }
   __device__ long row(long c) const { return idx ? (long)idx[c] : c; }
   __device__ void load(long c, double &T, double &P, double *Y, double &DT) const {
-    const long r = row(c);
-    T = t[r];
-    P = p[r];
-    DT = dt[r];
-#pragma unroll
-    for (int k = 0; k < kNs; ++k) Y[k] = y[r * kNs + k];
+    Raw r;
+    load_raw(c, r);
+    unpack_raw(r, T, P, Y, DT);
   }
 };
 
@@ -94,18 +146,26 @@ struct ZoneSrc {
   const R *bv, *sv, *dt;
   int mx, my, g, nx, ny;
   long plane, n;
+  struct Raw {
+    R t, p, dt, y[kNs];
+  };
+  __device__ void load_raw(long c, Raw &r) const {
+    const long q = at(c);
+    r.t = bv[5 * plane + q];
+    r.p = bv[4 * plane + q];
+    r.dt = dt[q];
+#pragma unroll
+    for (int k = 0; k < kNs; ++k) r.y[k] = sv[k * plane + q];
+  }
   __device__ long at(long c) const {
     const long i = c % mx, jk = c / mx;
     const long j = jk % my, k = jk / my;
     return ((k + g) * ny + (j + g)) * (long)nx + (i + g);
   }
   __device__ void load(long c, double &T, double &P, double *Y, double &DT) const {
-    const long q = at(c);
-    T = (double)bv[5 * plane + q];
-    P = (double)bv[4 * plane + q];
-    DT = (double)dt[q];
-#pragma unroll
-    for (int k = 0; k < kNs; ++k) Y[k] = (double)sv[k * plane + q];
+    Raw r;
+    load_raw(c, r);
+    unpack_raw(r, T, P, Y, DT);
   }
 };
 
@@ -136,21 +196,24 @@ struct TrainOut {
 };
 
 struct InferOut {
+  float *out12;                  // [n][12] standardised outputs (forward -> decode)
   double *dy;                    // [n][9]
-  int *fb_count;                 // fallback list length
-  long *fb_list;                 // fallback cells
-  unsigned long long *stats;     // [4]
+  int *fb_count;                 // sub-list lengths (stride kFbStride)
+  long *fb_list;                 // sub-lists of fallback cells, fb_cap apart
+  long fb_cap;                   // capacity of one sub-list
+  unsigned long long *stats;     // statistics rows (stride kStatStride)
   float *raw;                    // optional [n][16] standardised outputs
 };
 
-// GELU of two accumulator quads (units 4g..4g+3 of two 16-unit tiles) packed
-// into the next layer's B operand
-__device__ __forceinline__ bf16x8 gelu_pack(const f32x4 &a0, const f32x4 &a1) {
+// activation of two accumulator quads (units 4g..4g+3 of two 16-unit tiles)
+// packed into the next layer's B operand
+template <int ACT>
+__device__ __forceinline__ bf16x8 act_pack(const f32x4 &a0, const f32x4 &a1) {
   uint4 u;
-  u.x = pack_bf16(gelu_f(a0[0]), gelu_f(a0[1]));
-  u.y = pack_bf16(gelu_f(a0[2]), gelu_f(a0[3]));
-  u.z = pack_bf16(gelu_f(a1[0]), gelu_f(a1[1]));
-  u.w = pack_bf16(gelu_f(a1[2]), gelu_f(a1[3]));
+  u.x = pack_bf16(act_f<ACT>(a0[0]), act_f<ACT>(a0[1]));
+  u.y = pack_bf16(act_f<ACT>(a0[2]), act_f<ACT>(a0[3]));
+  u.z = pack_bf16(act_f<ACT>(a1[0]), act_f<ACT>(a1[1]));
+  u.w = pack_bf16(act_f<ACT>(a1[2]), act_f<ACT>(a1[3]));
   return __builtin_bit_cast(bf16x8, u);
 }
 
@@ -171,7 +234,7 @@ __device__ __forceinline__ int tile_l0(int mt) { return mt; }
 __device__ __forceinline__ int tile_h(int l, int mt, int kc) { return 8 + (l - 1) * 32 + mt * 4 + kc; }
 __device__ __forceinline__ int tile_out(int kc) { return 104 + kc; }
 
-template <typename Src, bool TRAIN>
+template <typename Src, bool TRAIN, int ACT>
 __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src src, const bf16x8 *__restrict__ wpk,
                                                             const float *__restrict__ bias, TrainOut tr,
                                                             InferOut io) {
@@ -242,7 +305,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
           dst[1] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);  // bias column for the weight gradient
       }
     }
-    // the staging rows are rewritten with the outputs below
+    // the staging rows are rewritten by the next pass (their reads above are
+    // consumed before the first MFMA)
     __builtin_amdgcn_wave_barrier();
 
     bf16x8 bc[4][kTiles];  // current layer input: [k chunk][slab]
@@ -260,7 +324,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
       }
 #pragma unroll
       for (int t = 0; t < kTiles; ++t) {
-        bc[kc][t] = gelu_pack(acc[0][t], acc[1][t]);
+        bc[kc][t] = act_pack<ACT>(acc[0][t], acc[1][t]);
         if (TRAIN) save_layer(tr, 0, n, base + 16 * t + cn, 32 * kc + 4 * g, acc[0][t], acc[1][t], bc[kc][t]);
       }
     }
@@ -268,6 +332,10 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
 #pragma unroll
     for (int l = 1; l <= 3; ++l) {
       bf16x8 bn[4][kTiles];
+      // the 32 A tiles of the layer in issue order q = (kc*4 + ki)*2 + h,
+      // each read from LDS one step ahead of its MFMAs
+      bf16x8 abuf[2];
+      abuf[0] = sW[tile_h(l, 0, 0) * 64 + ln];
 #pragma unroll
       for (int kc = 0; kc < 4; ++kc) {
         f32x4 acc[2][kTiles];
@@ -278,14 +346,19 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
         for (int ki = 0; ki < 4; ++ki)
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
-            const bf16x8 a = sW[tile_h(l, 2 * kc + h, ki) * 64 + ln];
+            const int q = (kc * 4 + ki) * 2 + h;
+            if (q + 1 < 32) {
+              const int qn = q + 1;
+              abuf[qn & 1] = sW[tile_h(l, 2 * (qn >> 3) + (qn & 1), (qn >> 1) & 3) * 64 + ln];
+            }
+            const bf16x8 a = abuf[q & 1];
 #pragma unroll
             for (int t = 0; t < kTiles; ++t)
               acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[ki][t], ki == 0 ? bz[h] : acc[h][t], 0, 0, 0);
           }
 #pragma unroll
         for (int t = 0; t < kTiles; ++t) {
-          bn[kc][t] = gelu_pack(acc[0][t], acc[1][t]);
+          bn[kc][t] = act_pack<ACT>(acc[0][t], acc[1][t]);
           if (TRAIN) save_layer(tr, l, n, base + 16 * t + cn, 32 * kc + 4 * g, acc[0][t], acc[1][t], bn[kc][t]);
         }
       }
@@ -329,70 +402,103 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
       }
       continue;
     }
-    // outputs to one lane per cell, in two halves: rows 0..7 (g < 2), 8..15
-    float o[16];
+    // standardised outputs in the accumulator layout: lane (g, cn) holds
+    // outputs 4g..4g+3 of cell 16t+cn (outputs 12..15 are padding)
+    if (g < 3) {
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      wave_sync();
-#pragma unroll
-      for (int t = 0; t < kTiles; ++t)
-        if ((g >> 1) == half)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) stg[(16 * t + cn) * kStageLd + 4 * (g & 1) + i] = acc[t][i];
-      wave_sync();
-#pragma unroll
-      for (int q = 0; q < 8; ++q) o[8 * half + q] = stg[ln * kStageLd + q];
-    }
-
-    // ---- decode + checks: one lane per cell
-    const long c = base + ln;
-    int why = 0;  // 0 accepted, 1 input range, 2 output range, 3 element balance
-    if (c < n) {
-      double T, P, DT, Y[kNs];
-      src.load(c, T, P, Y, DT);
-      if (io.raw) {
-#pragma unroll
-        for (int q = 0; q < 16; q += 4)
-          *reinterpret_cast<float4 *>(io.raw + c * 16 + q) = make_float4(o[q], o[q + 1], o[q + 2], o[q + 3]);
-      }
-      float dy[kNs];
-      sur_decode(s, o, dy);
-      why = sur_check(s, T, P, DT, Y, dy);
-      if (why == 0) {
-#pragma unroll
-        for (int k = 0; k < kNs; ++k) io.dy[c * kNs + k] = (double)dy[k];
-      }
-    }
-    const unsigned long long fail = __ballot(why != 0);
-    if (fail) {
-      const int first = __ffsll((long long)fail) - 1;
-      int at = 0;
-      if (ln == first) at = atomicAdd(io.fb_count, __popcll(fail));
-      at = __shfl(at, first);
-      if (why != 0) io.fb_list[at + __popcll(fail & ((1ull << ln) - 1ull))] = c;
-    }
-    if (io.stats) {
-      const unsigned long long valid = __ballot(c < n);
-      if (ln == 0) atomicAdd(&io.stats[0], (unsigned long long)__popcll(valid));
-#pragma unroll
-      for (int r = 1; r <= 3; ++r) {
-        const unsigned long long m = __ballot(why == r);
-        if (m && ln == 0) atomicAdd(&io.stats[r], (unsigned long long)__popcll(m));
+      for (int t = 0; t < kTiles; ++t) {
+        const long c = base + 16 * t + cn;
+        if (c < n)
+          *reinterpret_cast<float4 *>(io.out12 + c * 12 + 4 * g) =
+              make_float4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
       }
     }
   }
 }
 
-// Plain point-implicit step on the fallback list (grid-stride over a count
-// that lives on the device).
+// Decode + checks, one thread per cell: accepted cells store dY, the others
+// join the wave's fallback sub-list (one atomic per wave).
+template <typename Src>
+__global__ __launch_bounds__(256) void k_sur_decode(SurParams s, Src src, InferOut io) {
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  const int ln = threadIdx.x & 63;
+  int why = 0;  // 0 accepted, 1 input range, 2 output range, 3 element balance
+  if (c < src.n) {
+    float o[16];
+    const float4 *r4 = reinterpret_cast<const float4 *>(io.out12 + c * 12);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float4 v = r4[q];
+      o[4 * q] = v.x;
+      o[4 * q + 1] = v.y;
+      o[4 * q + 2] = v.z;
+      o[4 * q + 3] = v.w;
+    }
+#pragma unroll
+    for (int q = 12; q < 16; ++q) o[q] = 0.0f;
+    double T, P, DT, Y[kNs];
+    src.load(c, T, P, Y, DT);
+    if (io.raw) {
+#pragma unroll
+      for (int q = 0; q < 16; q += 4)
+        *reinterpret_cast<float4 *>(io.raw + c * 16 + q) = make_float4(o[q], o[q + 1], o[q + 2], o[q + 3]);
+    }
+    float dy[kNs];
+    sur_decode(s, o, dy);
+    why = sur_check(s, T, P, DT, Y, dy);
+    if (why == 0) {
+#pragma unroll
+      for (int k = 0; k < kNs; ++k) io.dy[c * kNs + k] = (double)dy[k];
+    }
+  }
+  const int list = (int)((c >> 6) % kFbLists);
+  const unsigned long long fail = __ballot(why != 0);
+  if (fail) {
+    const int first = __ffsll((long long)fail) - 1;
+    int at = 0;
+    if (ln == first) at = atomicAdd(io.fb_count + list * kFbStride, __popcll(fail));
+    at = __shfl(at, first);
+    if (why != 0) io.fb_list[list * io.fb_cap + at + __popcll(fail & ((1ull << ln) - 1ull))] = c;
+  }
+  if (io.stats) {
+    unsigned long long *row = io.stats + list * kStatStride;
+    const unsigned long long valid = __ballot(c < src.n);
+    if (ln == 0) atomicAdd(&row[0], (unsigned long long)__popcll(valid));
+#pragma unroll
+    for (int r = 1; r <= 3; ++r) {
+      const unsigned long long m = __ballot(why == r);
+      if (m && ln == 0) atomicAdd(&row[r], (unsigned long long)__popcll(m));
+    }
+  }
+}
+
+// Plain point-implicit step on the fallback sub-lists (grid-stride over their
+// total, which lives on the device; entry q is found by a binary search over
+// the sub-list offsets).
 template <typename Src>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_sur_fallback(
     const ostc_param *__restrict__ p, Src src,
                                                       const int *__restrict__ count, const long *__restrict__ list,
-                                                      double *__restrict__ dyo) {
-  const int m = *count;
+                                                      long cap, double *__restrict__ dyo) {
+  __shared__ int pre[kFbLists + 1];
+  for (int l = threadIdx.x; l < kFbLists; l += blockDim.x) pre[l + 1] = count[l * kFbStride];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    pre[0] = 0;
+    for (int l = 1; l <= kFbLists; ++l) pre[l] += pre[l - 1];
+  }
+  __syncthreads();
+  const int m = pre[kFbLists];
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x) {
-    const long c = list[q];
+    int lo = 0, hi = kFbLists;  // pre[lo] <= q < pre[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (pre[mid] <= q)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const long c = list[lo * cap + (q - pre[lo])];
     double T, P, DT, Y[kNs], dy[kNs];
     src.load(c, T, P, Y, DT);
     plain_chem_step<kNs>(p, T, P, Y, DT, dy);
@@ -413,9 +519,11 @@ __global__ __launch_bounds__(256) void k_zone_plain(const ostc_param *__restrict
   for (int k = 0; k < kNs; ++k) dyo[c * kNs + k] = dy[k];
 }
 
-__global__ void k_sur_reset(int *count, unsigned long long *stats) {
-  if (threadIdx.x == 0) *count = 0;
-  if (threadIdx.x < 4) stats[threadIdx.x] = 0;
+__global__ __launch_bounds__(kFbLists) void k_sur_reset(int *count, unsigned long long *stats) {
+  const int l = threadIdx.x;
+  count[l * kFbStride] = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) stats[l * kStatStride + r] = 0;
 }
 
 // ---- training kernels -----------------------------------------------------
@@ -471,14 +579,14 @@ struct GemmArgs {
   long K;          // multiple of 16
   int splits;      // split of K (WGRAD)
   // DGRAD: A(m,k) = wq[k*lda + m] (k < kmax), B(k,n) = g[n*ldb + k];
-  //        out[n*128 + m] = C * gelu'(z[n*128 + m])
+  //        out[n*128 + m] = C * act'(z[n*128 + m])
   // WGRAD: A(m,k) = g[k*lda + m], B(k,n) = act[k*ldb + n]; out[s][m*N + n]
   const float *a, *b, *z;
   int lda, ldb, kmax;
   float *out;
 };
 
-template <int MODE>
+template <int MODE, int ACT>
 __global__ __launch_bounds__(256) void k_sur_gemm(GemmArgs ga) {
   const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63, g = ln >> 4, cn = ln & 15;
   const long tiles_m = ga.M / 16, tiles_n = ga.N / 16;
@@ -511,7 +619,7 @@ __global__ __launch_bounds__(256) void k_sur_gemm(GemmArgs ga) {
   if (MODE == GEMM_DGRAD) {
     float r[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = acc[i] * gelu_grad_f(ga.z[nn * 128 + mr + i]);
+    for (int i = 0; i < 4; ++i) r[i] = acc[i] * act_grad_f<ACT>(ga.z[nn * 128 + mr + i]);
     *reinterpret_cast<float4 *>(ga.out + nn * 128 + mr) = make_float4(r[0], r[1], r[2], r[3]);
   } else {
     float *o = ga.out + sp * (long)ga.M * ga.N;
@@ -604,20 +712,34 @@ static unsigned forward_grid(long n, int cus) {
   return (unsigned)std::max<long>(1, std::min<long>(wgs, cus));
 }
 
+template <typename Src, bool TRAIN>
+static void launch_forward(const SurState &st, const Src &src, const TrainOut &tr, const InferOut &io,
+                           hipStream_t s) {
+  const dim3 grid(forward_grid(src.n, st.cus)), block(kWaves * 64);
+  const bf16x8 *wpk = reinterpret_cast<const bf16x8 *>(st.packed);
+  if (st.params.act == OSTC_SUR_HSWISH)
+    hipLaunchKernelGGL((k_sur_forward<Src, TRAIN, OSTC_SUR_HSWISH>), grid, block, 0, s, st.params, src, wpk,
+                       st.bias, tr, io);
+  else
+    hipLaunchKernelGGL((k_sur_forward<Src, TRAIN, OSTC_SUR_GELU>), grid, block, 0, s, st.params, src, wpk, st.bias,
+                       tr, io);
+}
+
 template <typename Src>
 static int infer_impl(const SurState &st, const ostc_param *dp, const Src &src, double *dy, float *raw,
                       hipStream_t s) {
-  hipLaunchKernelGGL(k_sur_reset, dim3(1), dim3(64), 0, s, st.fb_count, st.stats);
+  hipLaunchKernelGGL(k_sur_reset, dim3(1), dim3(kFbLists), 0, s, st.fb_count, st.stats);
   TrainOut tr{};
-  InferOut io{dy, st.fb_count, st.fb_list, st.stats, raw};
-  hipLaunchKernelGGL((k_sur_forward<Src, false>), dim3(forward_grid(src.n, st.cus)), dim3(kWaves * 64), 0, s,
-                     st.params, src, reinterpret_cast<const bf16x8 *>(st.packed), st.bias, tr, io);
+  InferOut io{st.out12, dy, st.fb_count, st.fb_list, fb_list_cap(st.fb_cap), st.stats, raw};
+  launch_forward<Src, false>(st, src, tr, io, s);
+  hipLaunchKernelGGL(k_sur_decode<Src>, dim3((unsigned)((src.n + 255) / 256)), dim3(256), 0, s, st.params, src,
+                     io);
   // one wave per workgroup (the kernel's launch bound), enough of them to
   // spread a typical fallback list over every CU; the kernel strides over
   // the device-side count
   const long fb_blocks = std::min<long>((src.n + 63) / 64, 16L * st.cus);
   hipLaunchKernelGGL(k_sur_fallback<Src>, dim3((unsigned)std::max<long>(1, fb_blocks)), dim3(64), 0, s, dp, src,
-                     st.fb_count, st.fb_list, dy);
+                     st.fb_count, st.fb_list, fb_list_cap(st.fb_cap), dy);
   return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;
 }
 
@@ -654,8 +776,7 @@ int sur_gradients(const SurState &st, long nb, const double *t, const double *p,
   BatchSrc src{t, p, y, dt, idx, nb};
   TrainOut tr{st.a0, st.z, st.a, st.yout};
   InferOut io{};
-  hipLaunchKernelGGL((k_sur_forward<BatchSrc, true>), dim3(forward_grid(nb, st.cus)), dim3(kWaves * 64), 0, s,
-                     st.params, src, reinterpret_cast<const bf16x8 *>(st.packed), st.bias, tr, io);
+  launch_forward<BatchSrc, true>(st, src, tr, io, s);
   const unsigned lblocks = (unsigned)((nb * 16 + 255) / 256);
   hipLaunchKernelGGL(k_sur_loss, dim3(lblocks), dim3(256), 0, s, st.yout, target, idx, st.loss_w,
                      st.params.n_out, nb, st.g5, st.loss_part);
@@ -680,12 +801,13 @@ int sur_gradients(const SurState &st, long nb, const double *t, const double *p,
     wg.b = act;
     wg.ldb = ldh;
     wg.out = st.part;
-    hipLaunchKernelGGL(k_sur_gemm<GEMM_WGRAD>, dim3(grid_for((long)(M / 16) * (N / 16) * wg.splits)), dim3(256), 0,
+    // (the weight gradient does not use the activation parameter)
+    hipLaunchKernelGGL((k_sur_gemm<GEMM_WGRAD, OSTC_SUR_GELU>), dim3(grid_for((long)(M / 16) * (N / 16) * wg.splits)), dim3(256), 0,
                        s, wg);
     hipLaunchKernelGGL(k_sur_reduce, dim3((unsigned)((mout * (kin + 1) + 255) / 256)), dim3(256), 0, s, st.part,
                        wg.splits, M, N, mout, kin, grad + sur_layer_offset(l));
     if (l == 0) break;
-    // input gradient through W (BF16-rounded) and GELU'
+    // input gradient through W (BF16-rounded) and the activation's derivative
     GemmArgs dg{};
     dg.M = 128;
     dg.N = (int)nb;
@@ -699,7 +821,11 @@ int sur_gradients(const SurState &st, long nb, const double *t, const double *p,
     dg.z = st.z + (long)(l - 1) * nb * 128;
     float *gnext = st.gbuf[l & 1];
     dg.out = gnext;
-    hipLaunchKernelGGL(k_sur_gemm<GEMM_DGRAD>, dim3(grid_for((long)(128 / 16) * (nb / 16))), dim3(256), 0, s, dg);
+    const dim3 dgrid(grid_for((long)(128 / 16) * (nb / 16)));
+    if (st.params.act == OSTC_SUR_HSWISH)
+      hipLaunchKernelGGL((k_sur_gemm<GEMM_DGRAD, OSTC_SUR_HSWISH>), dgrid, dim3(256), 0, s, dg);
+    else
+      hipLaunchKernelGGL((k_sur_gemm<GEMM_DGRAD, OSTC_SUR_GELU>), dgrid, dim3(256), 0, s, dg);
     gcur = gnext;
     ldg = 128;
   }

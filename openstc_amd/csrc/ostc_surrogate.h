@@ -23,6 +23,18 @@ constexpr float kInvAtm = (float)(1.0 / 101325.0);
 constexpr float kBoxCox = 0.1f;
 constexpr float kInvBoxCox = 10.0f;
 constexpr float kAdamB1 = 0.9f, kAdamB2 = 0.999f, kAdamEps = 1e-8f;
+// The fallback list is split into kFbLists sub-lists, each with its own
+// counter (and row of statistics) 128 bytes apart: wave w of the decode kernel
+// appends to sub-list w % kFbLists, so the list atomics spread over many
+// memory channels instead of serialising on one address.
+constexpr int kFbLists = 64;
+constexpr int kFbStride = 32;    // ints between two sub-list counters
+constexpr int kStatStride = 16;  // u64 between two statistics rows
+// capacity of one sub-list for n cells (its waves times 64 cells)
+__host__ __device__ inline long fb_list_cap(long n) {
+  const long waves = (n + 63) / 64;
+  return (waves + kFbLists - 1) / kFbLists * 64;
+}
 
 // parameter layout: per layer W[out][in] row-major, then b[out]
 __host__ __device__ inline int sur_in_dim(int l) { return l == 0 ? kSurIn : kSurHidden; }
@@ -35,7 +47,7 @@ __host__ __device__ inline int sur_n_params(int n_out) { return 1664 + 3 * 16512
 
 // device copy of ostc_surrogate_desc with reciprocal standard deviations
 struct SurParams {
-  int n_out, kind, n_elem;
+  int n_out, kind, n_elem, act;
   float in_mean[kSurIn], in_rstd[kSurIn];
   float out_mean[16], out_std[16];
   float elem_w[OSTC_SUR_MAX_ELEM * 9];
@@ -55,10 +67,11 @@ struct SurState {
   void *packed = nullptr;  // kSurTiles * 64 lanes * 8 BF16
   float *bias = nullptr;   // kSurBias
   float *loss_w = nullptr; // 16
-  int *fb_count = nullptr;
-  long *fb_list = nullptr;
+  int *fb_count = nullptr;                 // [kFbLists * kFbStride]
+  long *fb_list = nullptr;                 // [kFbLists][fb_list_cap(fb_cap)]
+  float *out12 = nullptr;  // [fb_cap][12] standardised outputs of the forward kernel
   long fb_cap = 0;
-  unsigned long long *stats = nullptr;  // 4
+  unsigned long long *stats = nullptr;  // [kFbLists * kStatStride], 4 used per row
   // training buffers (max_batch rows)
   float *a0 = nullptr, *z = nullptr, *a = nullptr, *yout = nullptr, *g5 = nullptr;
   float *gbuf[2] = {nullptr, nullptr};

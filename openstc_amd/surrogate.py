@@ -5,7 +5,7 @@ point-implicit chemistry step (``ostc_chem_plain_step``: (I - dt J) d(rhoY) =
 dt omega at fixed T, p) of the 9-species H2-air mechanism.  States are
 sampled from the solver's own trajectories of the finite-rate mixing-layer
 workload, labelled by that step on the GPU, and the network (12-128x4-n_out,
-GELU) is trained on the GPU by the library's forward/backward kernels with
+GELU or hard-swish hidden activation) is trained on the GPU by the library's forward/backward kernels with
 Adam on FP32 master weights.  Inference runs on BF16 MFMA with per-cell
 fallback to the plain step.
 
@@ -25,6 +25,8 @@ import numpy as np
 from . import _lib as L
 
 KIND_LOGMAG, KIND_ASINH = 0, 1
+ACT_GELU, ACT_HSWISH = 0, 1
+ACTIVATIONS = {"gelu": ACT_GELU, "hswish": ACT_HSWISH}
 N_SPEC = 9
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2air9.npz")
@@ -107,10 +109,11 @@ def element_weights(param: L.Param, info: L.MechanismInfo) -> np.ndarray:
 
 
 def make_desc(t, p, y, dt, dy, param: L.Param, info: L.MechanismInfo, kind=KIND_LOGMAG, floor=1e-12,
-              y_tol=1e-5, elem_tol=1e-2, elem_abs=1e-9, mag_weight=4.0) -> L.SurDesc:
+              y_tol=1e-5, elem_tol=1e-2, elem_abs=1e-9, mag_weight=4.0, activation=ACT_GELU) -> L.SurDesc:
     """Standardisation, fallback box and balance weights from a training set."""
     d = L.SurDesc()
     d.kind = kind
+    d.activation = activation
     d.n_out = 10 if kind == KIND_LOGMAG else 9
     f = raw_features(t, p, y, dt).astype(np.float64)
     for q in range(12):

@@ -40,14 +40,15 @@ def main(argv=None):
     ap.add_argument("--val-per-snapshot", type=int, default=20000)
     ap.add_argument("--cfl", type=float, default=0.2)
     ap.add_argument("--grid", type=int, nargs=3, default=[192, 96, 96])
-    ap.add_argument("--epochs", type=int, default=100)
-    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--keep-val", type=int, default=8192, help="held-out states stored with the weights")
     ap.add_argument("--uniform", action="store_true", help="sample training states uniformly, not by |dY| decade")
     ap.add_argument("--mag-weight", type=float, default=16.0)
     ap.add_argument("--target", type=float, default=8e-4, help="rel_l2 the output gate is calibrated to")
+    ap.add_argument("--activation", choices=sorted(S.ACTIVATIONS), default="hswish")
     a = ap.parse_args(argv)
     mx, my, mz = a.grid
     t0 = time.time()
@@ -64,7 +65,8 @@ def main(argv=None):
     t, p, y, dt, dy, it = tr
     vt, vp, vy, vdt, vdy, vit = va
     log(f"[train_surrogate] {len(t)} training / {len(vt)} held-out states labelled ({time.time() - t0:.1f}s)")
-    desc = S.make_desc(t, p, y, dt, dy, param, info, mag_weight=a.mag_weight)
+    desc = S.make_desc(t, p, y, dt, dy, param, info, mag_weight=a.mag_weight,
+                       activation=S.ACTIVATIONS[a.activation])
     target = S.standardised_targets(dy, desc)
     sur = S.Surrogate(ctx, desc, max_batch=a.batch, seed=a.seed)
     vdev = [S.DeviceArray(np.ascontiguousarray(v)) for v in (vt, vp, vy, vdt)]
@@ -94,6 +96,7 @@ def main(argv=None):
              info=np.array(json.dumps({"grid": a.grid, "cfl": a.cfl, "steps": a.steps, "every": a.every,
                                        "n_train": int(len(t)), "n_heldout": int(len(vt)), "epochs": a.epochs,
                                        "batch": a.batch, "lr": a.lr, "mag_weight": a.mag_weight,
+                                       "activation": a.activation,
                                        "heldout_ungated": ungated, "dy_max": dy_max, "target": a.target,
                                        "heldout_test_gated": final})))
     log(f"[train_surrogate] wrote {a.out} ({time.time() - t0:.1f}s)")

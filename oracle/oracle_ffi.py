@@ -90,7 +90,7 @@ class OrcZone(C.Structure):
 
 
 class OrcSurDesc(C.Structure):
-    _fields_ = [("n_out", C.c_int), ("kind", C.c_int), ("n_elem", C.c_int), ("reserved", C.c_int),
+    _fields_ = [("n_out", C.c_int), ("kind", C.c_int), ("n_elem", C.c_int), ("activation", C.c_int),
                 ("in_mean", C.c_float * 12), ("in_std", C.c_float * 12),
                 ("out_mean", C.c_float * 16), ("out_std", C.c_float * 16),
                 ("loss_w", C.c_float * 16), ("elem_w", C.c_float * 36),

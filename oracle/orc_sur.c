@@ -46,6 +46,20 @@ static double gelu_grad(double z) {
   const double s = 1.0 / (1.0 + exp(-u));
   return s + z * s * (1.0 - s) * GELU_C * (1.0 + 3.0 * GELU_A * z * z);
 }
+/* hard-swish z*clamp(z/6 + 1/2, 0, 1); derivative (2z + 3)/6 inside (-3, 3) */
+static double hswish(double z) {
+  const double c = z / 6.0 + 0.5;
+  return z * (c < 0.0 ? 0.0 : c > 1.0 ? 1.0 : c);
+}
+static double hswish_grad(double z) {
+  if (z <= -3.0) return 0.0;
+  if (z >= 3.0) return 1.0;
+  return (2.0 * z + 3.0) / 6.0;
+}
+static double act(const orc_sur_desc *d, double z) { return d->activation == 1 ? hswish(z) : gelu(z); }
+static double act_grad(const orc_sur_desc *d, double z) {
+  return d->activation == 1 ? hswish_grad(z) : gelu_grad(z);
+}
 
 void orc_sur_features(const orc_sur_desc *d, double t, double p, const double *y, double dt, float *f) {
   const float inv_atm = (float)(1.0 / 101325.0);
@@ -80,7 +94,7 @@ static void forward_one(const orc_sur_desc *d, const float *w, int quant, const 
       s += b[m];
       if (l < NL - 1) {
         z[l][m] = s;
-        const double a = gelu(s);
+        const double a = act(d, s);
         h[l + 1][m] = quant ? orc_bf16(a) : a;
       } else {
         out[m] = s;
@@ -134,7 +148,7 @@ double orc_sur_loss_grad(const orc_sur_desc *d, const float *w, int quant, long 
           const double wk = quant ? orc_bf16(W[m * ki + k]) : (double)W[m * ki + k];
           s += wk * g[m];
         }
-        gp[k] = s * gelu_grad(z[l - 1][k]);
+        gp[k] = s * act_grad(d, z[l - 1][k]);
       }
       memcpy(g, gp, sizeof(double) * ki);
     }

@@ -185,15 +185,17 @@ int orc_newton_temperature(const orc_param *p, const double *y, double e, double
 int orc_omp_threads(void);
 
 /* ---- learned chemistry surrogate (no reference counterpart) ----
- * FP64 restatement of the product's MLP 12-128-128-128-128-n_out (GELU, tanh
- * form): features, forward, weighted-MSE gradient, output decoding and the
+ * FP64 restatement of the product's MLP 12-128-128-128-128-n_out (hidden
+ * activation by the descriptor: 0 = GELU tanh form, 1 = hard-swish
+ * z*clamp(z/6 + 1/2, 0, 1)): features, forward, weighted-MSE gradient,
+ * output decoding and the
  * fallback checks.  With quant = 1 the BF16 rounding points of the product's
  * inference (features, weights, hidden activations; round to nearest even)
  * are emulated and the backward treats them as identity (straight-through);
  * with quant = 0 everything is plain FP64 (for finite-difference checks).
  * The descriptor has the layout of ostc_surrogate_desc. */
 typedef struct orc_sur_desc {
-  int n_out, kind, n_elem, reserved;
+  int n_out, kind, n_elem, activation;
   float in_mean[12], in_std[12];
   float out_mean[16], out_std[16];
   float loss_w[16];

@@ -47,21 +47,35 @@ def states(n, seed):
     return t, p, y, dt
 
 
-def make(env, n=4096, seed=1, max_batch=1024, wseed=3):
+def weights(n_out, wseed, activation):
+    w = S.init_weights(n_out, wseed)
+    if activation == S.ACT_HSWISH:
+        # hidden biases spread over (-3.5, 3.5): units on all three branches
+        # of hard-swish (0, quadratic, identity)
+        rng = np.random.default_rng(wseed + 100)
+        for l in range(4):
+            b0 = 1664 - 128 if l == 0 else 1664 + l * 16512 - 128
+            w[b0:b0 + 128] = rng.uniform(-3.5, 3.5, 128).astype(np.float32)
+    return w
+
+
+def make(env, n=4096, seed=1, max_batch=1024, wseed=3, activation=S.ACT_GELU):
     ctx, param, info = env
     t, p, y, dt = states(n, seed)
     dy = S.plain_step(ctx, t, p, y, dt)
-    desc = S.make_desc(t, p, y, dt, dy, param, info)
-    sur = S.Surrogate(ctx, desc, S.init_weights(desc.n_out, wseed), max_batch=max_batch)
+    desc = S.make_desc(t, p, y, dt, dy, param, info, activation=activation)
+    w = weights(desc.n_out, wseed, activation)
+    sur = S.Surrogate(ctx, desc, w, max_batch=max_batch)
     od = O.OrcSurDesc()
     C.memmove(C.addressof(od), C.addressof(desc), C.sizeof(od))
-    return sur, od, (t, p, y, dt, dy)
+    return sur, od, (t, p, y, dt, dy), w
 
 
-def test_forward_matches_oracle(env):
-    sur, od, (t, p, y, dt, dy) = make(env)
+@pytest.mark.parametrize("activation", [S.ACT_GELU, S.ACT_HSWISH])
+def test_forward_matches_oracle(env, activation):
+    sur, od, (t, p, y, dt, dy), w = make(env, activation=activation)
     n = 2000
-    say("forward n=%d" % n)
+    say("forward n=%d activation %d" % (n, activation))
     d = [S.DeviceArray(np.ascontiguousarray(a[:n])) for a in (t, p, y, dt)]
     raw = S.DeviceArray(nbytes=n * 16 * 4)
     dyd = S.DeviceArray(nbytes=n * 9 * 8)
@@ -69,20 +83,23 @@ def test_forward_matches_oracle(env):
     g = raw.get(np.float32, (n, 16))
     ref = np.zeros((n, 16))
     yy = np.ascontiguousarray(y[:n])
-    O.load().orc_sur_forward(C.byref(od), fptr(S.init_weights(sur.desc.n_out, 3)), 1, n, O.dp(t[:n].copy()),
+    O.load().orc_sur_forward(C.byref(od), fptr(w), 1, n, O.dp(t[:n].copy()),
                              O.dp(p[:n].copy()), O.dp(yy), O.dp(dt[:n].copy()), O.dp(ref))
     err = np.abs(g - ref)[:, :sur.desc.n_out]
     say("raw output max abs diff", err.max(), "median", np.median(err))
-    assert err.max() < 2e-3 and np.median(err) < 1e-5
+    # hidden units of the hard-swish case reach ~4: one BF16 ulp of such a
+    # unit is 4x that of the GELU case
+    assert err.max() < (2e-3 if activation == S.ACT_GELU else 1e-2) and np.median(err) < 1e-5
     for a in d + [raw, dyd]:
         a.free()
     sur.close()
 
 
-def test_gradients_match_oracle(env):
-    sur, od, (t, p, y, dt, dy) = make(env, max_batch=512)
+@pytest.mark.parametrize("activation", [S.ACT_GELU, S.ACT_HSWISH])
+def test_gradients_match_oracle(env, activation):
+    sur, od, (t, p, y, dt, dy), w = make(env, max_batch=512, activation=activation)
     nb = 256
-    say("gradients nb=%d" % nb)
+    say("gradients nb=%d activation %d" % (nb, activation))
     tgt = S.standardised_targets(dy, sur.desc)
     d = [S.DeviceArray(np.ascontiguousarray(a)) for a in (t, p, y, dt)]
     dtg = S.DeviceArray(tgt)
@@ -93,7 +110,7 @@ def test_gradients_match_oracle(env):
     gg = grad.get(np.float32, (sur.n_params,)).astype(np.float64)
     go = np.zeros(sur.n_params)
     yy = np.ascontiguousarray(y[idx])
-    lo = O.load().orc_sur_loss_grad(C.byref(od), fptr(S.init_weights(sur.desc.n_out, 3)), 1, nb,
+    lo = O.load().orc_sur_loss_grad(C.byref(od), fptr(w), 1, nb,
                                     O.dp(t[idx].copy()), O.dp(p[idx].copy()), O.dp(yy), O.dp(dt[idx].copy()),
                                     fptr(np.ascontiguousarray(tgt[idx])), O.dp(go))
     say("loss gpu %.8g oracle %.8g" % (loss, lo))
@@ -110,7 +127,7 @@ def test_gradients_match_oracle(env):
 
 
 def test_training_reduces_loss(env):
-    sur, od, (t, p, y, dt, dy) = make(env, n=4096, max_batch=1024)
+    sur, od, (t, p, y, dt, dy), _ = make(env, n=4096, max_batch=1024)
     tgt = S.standardised_targets(dy, sur.desc)
     say("training 12 epochs x 4 batches")
     hist = S.train(sur, t, p, y, dt, tgt, epochs=12, batch=1024, lr_max=2e-3, log=say)
@@ -120,7 +137,7 @@ def test_training_reduces_loss(env):
 
 def test_fallback_paths(env):
     ctx = env[0]
-    sur, od, (t, p, y, dt, dy) = make(env)
+    sur, od, (t, p, y, dt, dy), _ = make(env)
     n = 1024
     t, p, y, dt = (a[:n].copy() for a in (t, p, y, dt))
     t[:100] = 5000.0          # input outside the trained box -> reason 1

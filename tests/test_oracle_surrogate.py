@@ -4,7 +4,7 @@
   nearest even) on random and tie values.
 * Features: against an independent numpy FP32 evaluation.
 * Forward (quant = 0): against a torch float64 MLP with the same weights
-  (nn.functional.linear + GELU tanh form).
+  (nn.functional.linear + GELU tanh form, or nn.functional.hardswish).
 * Gradient (quant = 0): against central finite differences of the oracle's
   own loss.
 * Decode/checks: closed cases (balanced change accepted, out-of-box input,
@@ -40,11 +40,11 @@ def states(n, seed):
     return t, p, y, dt
 
 
-def make_desc(mech, n=2048):
+def make_desc(mech, n=2048, activation=S.ACT_GELU):
     param, info = mech
     t, p, y, dt = states(n, 1)
     dy = (np.random.default_rng(2).normal(size=(n, 9)) * 1e-5)
-    d = S.make_desc(t, p, y, dt, dy, param, info)
+    d = S.make_desc(t, p, y, dt, dy, param, info, activation=activation)
     od = O.OrcSurDesc()
     C.memmove(C.addressof(od), C.addressof(d), C.sizeof(od))
     return d, od
@@ -81,7 +81,7 @@ def test_features_match_numpy(mech):
     assert np.mean(got == ref) > 0.5
 
 
-def torch_forward(w, feat, n_out):
+def torch_forward(w, feat, n_out, activation=S.ACT_GELU):
     wt = torch.tensor(w.astype(np.float64))
     h = torch.tensor(feat.astype(np.float64))
     off = 0
@@ -93,14 +93,21 @@ def torch_forward(w, feat, n_out):
         off += fout * fin + fout
         h = torch.nn.functional.linear(h, W, b)
         if l < 4:
-            h = torch.nn.functional.gelu(h, approximate="tanh")
+            if activation == S.ACT_HSWISH:
+                h = torch.nn.functional.hardswish(h)
+            else:
+                h = torch.nn.functional.gelu(h, approximate="tanh")
     return h.numpy()
 
 
-def test_forward_matches_torch_fp64(mech):
+@pytest.mark.parametrize("activation", [S.ACT_GELU, S.ACT_HSWISH])
+def test_forward_matches_torch_fp64(mech, activation):
     lib = O.load()
-    d, od = make_desc(mech)
+    d, od = make_desc(mech, activation=activation)
     w = S.init_weights(d.n_out, seed=5)
+    if activation == S.ACT_HSWISH:
+        # pre-activations spread over both kinks (+-3) and the quadratic part
+        w *= 6.0
     n = 300
     t, p, y, dt = states(n, 4)
     y = np.ascontiguousarray(y)
@@ -109,15 +116,23 @@ def test_forward_matches_torch_fp64(mech):
     feat = np.empty((n, 12), np.float32)
     for c in range(n):
         lib.orc_sur_features(C.byref(od), t[c], p[c], O.dp(np.ascontiguousarray(y[c])), dt[c], fptr(feat[c]))
-    ref = torch_forward(w, feat, d.n_out)
+    ref = torch_forward(w, feat, d.n_out, activation)
     assert np.abs(out[:, :d.n_out] - ref).max() < 1e-12
     assert np.all(out[:, d.n_out:] == 0)
 
 
-def test_gradient_matches_finite_differences(mech):
+@pytest.mark.parametrize("activation,shift", [(S.ACT_GELU, 0.0), (S.ACT_HSWISH, 0.0), (S.ACT_HSWISH, 8.0),
+                                              (S.ACT_HSWISH, -8.0)])
+def test_gradient_matches_finite_differences(mech, activation, shift):
+    """shift moves every hidden bias so that hard-swish units sit on one
+    branch: 0 -> the quadratic part, +8 -> the identity (derivative 1),
+    -8 -> zero (derivative 0: the earlier layers' gradients vanish)."""
     lib = O.load()
-    d, od = make_desc(mech)
+    d, od = make_desc(mech, activation=activation)
     w = S.init_weights(d.n_out, seed=7)
+    for l in range(4):
+        b0 = 1664 - 128 if l == 0 else 1664 + l * 16512 - 128
+        w[b0:b0 + 128] += np.float32(shift)
     n = 24
     t, p, y, dt = states(n, 8)
     y = np.ascontiguousarray(y)
