@@ -510,14 +510,18 @@ __device__ __forceinline__ void jd_dir(const ZoneView<R> &z, int64_t q, int dir,
 // k planes: each cell's Jacobian products are formed once per direction and
 // handed to its neighbours -- along i by lane shuffles inside the wave (one
 // wave = one i row), along j through LDS, along k in registers from the
-// previous plane.  Tile-edge neighbours outside the tile are evaluated by
-// the edge lanes/rows themselves.  Off-diagonal sums are accumulated in the
-// reference's order (i-, j-, k-, i+, j+, k+).
+// previous plane.  Tile-edge neighbours outside the tile: the rows above and
+// below by the edge rows themselves (whole-wave loads); the cells left and
+// right of every row by 2*TJ lanes of the first wave in one pass, handed over
+// through LDS (single-lane halo loads per row would double the tile's vector
+// memory instructions, which bound this kernel).  Off-diagonal sums are
+// accumulated in the reference's order (i-, j-, k-, i+, j+, k+).
 template <typename R, int NS, int NT, bool CHEM, int TJ>
 __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_param *__restrict__ p,
                                                const R *__restrict__ din, R *__restrict__ dout, int kc) {
   constexpr int NV = 5 + NS + NT;
   __shared__ R sj[2][TJ][NV][64];  // [plus/minus][row][var][lane]
+  __shared__ R si[2][TJ][NV];      // [left P_i / right M_i][row][var]
   const int tx = threadIdx.x, ty = threadIdx.y;
   const uint3 bk = xcd_block();
   const int i = (int)bk.x * 64 + tx, j = (int)bk.y * TJ + ty;
@@ -565,17 +569,21 @@ __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_par
       ip[l] = __shfl_up(pi[l], 1);
       im[l] = __shfl_down(mi[l], 1);
     }
-    if (tx == 0 && i > 0 && i - 1 < z.mx) {
-      const int64_t q = z.idx(i - 1, jj, k);
-      JdCell<R, NS, NT> e;
-      jd_load<R, NS, NT>(z, p, din, q, e);
-      jd_dir<R, NS, NT>(z, q, 0, e, sr[q], ip, nullptr);
-    }
-    if (tx == 63 && i + 1 < z.mx) {
-      const int64_t q = z.idx(i + 1, jj, k);
-      JdCell<R, NS, NT> e;
-      jd_load<R, NS, NT>(z, p, din, q, e);
-      jd_dir<R, NS, NT>(z, q, 0, e, sr[q], nullptr, im);
+    if (ty == 0 && tx < 2 * TJ) {
+      // lane 2r: the cell left of row r's segment (its P_i); lane 2r+1: the
+      // cell right of it (its M_i)
+      const int r = tx >> 1, side = tx & 1;
+      const int ih = side ? (int)bk.x * 64 + 64 : (int)bk.x * 64 - 1;
+      const int jr = min((int)bk.y * TJ + r, z.my - 1);
+      if (ih >= 0 && ih < z.mx) {
+        const int64_t q = z.idx(ih, jr, k);
+        JdCell<R, NS, NT> e;
+        R hp[NV], hm[NV];
+        jd_load<R, NS, NT>(z, p, din, q, e);
+        jd_dir<R, NS, NT>(z, q, 0, e, sr[q], hp, hm);
+#pragma unroll
+        for (int l = 0; l < NV; ++l) si[side][r][l] = side ? hm[l] : hp[l];
+      }
     }
     // j: rows at the tile edge evaluate the neighbour row themselves
     R jp[NV], jm[NV];
@@ -592,6 +600,12 @@ __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_par
       jd_dir<R, NS, NT>(z, q, 1, e, sr[P + q], nullptr, jm);
     }
     __syncthreads();
+    if (tx == 0 && i > 0 && i - 1 < z.mx)
+#pragma unroll
+      for (int l = 0; l < NV; ++l) ip[l] = si[0][ty][l];
+    if (tx == 63 && i + 1 < z.mx)
+#pragma unroll
+      for (int l = 0; l < NV; ++l) im[l] = si[1][ty][l];
     if (ty > 0)
 #pragma unroll
       for (int l = 0; l < NV; ++l) jp[l] = sj[0][ty - 1][l][tx];
