@@ -273,7 +273,7 @@ def scale_leg(steps: int, warmup: int, reps: int, dims=(256, 192, 128), nblocks:
            "ms_per_step": round(el / steps * 1e3, 3), "setup_s": round(setup_s, 1),
            "model": "9-species/19-reaction H2-air finite-rate (EPI) + k-omega SST, DPLUR x4, CFL %g, "
                     "FP32 bulk + FP64 chemistry/Newton" % case.param.cfl}
-    weights = weights or S.DEFAULT_WEIGHTS
+    weights = weights or (S.JET_WEIGHTS if os.path.exists(S.JET_WEIGHTS) else S.DEFAULT_WEIGHTS)
     if not os.path.exists(weights):
         out["surrogate"] = None
         sim.close()
@@ -312,8 +312,11 @@ def scale_leg(steps: int, warmup: int, reps: int, dims=(256, 192, 128), nblocks:
         "speedup": round(times[0] / times[1], 3),
         "rel_l2": round(float(np.sqrt(sq_err / max(sq_ref, 1e-300))), 6),
         "fallback_rate": round(fallbacks / cells, 5),
-        "note": "surrogate trained on the 192x96x96 mixing-layer states; this workload's vitiated-air / jet "
-                "states are partly outside that range and fall back to the plain integrator",
+        "weights": os.path.basename(weights),
+        "note": ("model trained on this workload's states (other initial perturbations, its first 60 iterations)"
+                 if os.path.basename(weights) == os.path.basename(S.JET_WEIGHTS) else
+                 "model trained on the 192x96x96 mixing-layer states; this workload's vitiated-air / jet states "
+                 "are partly outside that range and fall back to the plain integrator"),
     }
     for b in (dy_p, dy_s):
         b.free()

@@ -29,7 +29,8 @@ ACT_GELU, ACT_HSWISH = 0, 1
 ACTIVATIONS = {"gelu": ACT_GELU, "hswish": ACT_HSWISH}
 N_SPEC = 9
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
-DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2air9.npz")
+DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2air9.npz")        # mixing-layer states
+JET_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2jet9.npz")            # four-block H2-jet states
 ATM = 101325.0
 
 
@@ -303,29 +304,38 @@ def stratified_pick(dy: np.ndarray, count: int, rng, lo=-12.0, hi=0.0, bins=12) 
 
 
 def harvest(case, n_steps: int, every: int, per_snapshot: int, seed: int = 0, log=print, stratify=True):
-    """Run the solver on the GPU and sample interior states every `every`
-    iterations together with their plain-step labels (the dt is the local time
-    step the iteration used).  Returns t, p, y, dt, dy, iteration."""
+    """Run the solver on the GPU and sample interior states of every block
+    every `every` iterations together with their plain-step labels (the dt is
+    the local time step the iteration used); `per_snapshot` states per block
+    and snapshot.  Returns t, p, y, dt, dy, iteration."""
     from .runtime import Simulation
     rng = np.random.default_rng(seed)
     sim = Simulation(case)
     sim.initialize_computation()
     out = [[], [], [], [], [], []]
     t0 = time.time()
+    stop = False
     for s in range(1, n_steps + 1):
         sim.step(output_screen=1 << 30)
-        if s % every == 0:
-            t, p, y, dt = interior_states(sim)
+        if s % every != 0:
+            continue
+        for b in range(len(sim.zones)):
+            t, p, y, dt = interior_states(sim, b)
             if not (np.isfinite(t).all() and np.isfinite(y).all()):
-                log(f"[harvest] non-finite state at iteration {s}; stopping")
+                log(f"[harvest] non-finite state at iteration {s}, block {b}; stopping")
+                stop = True
                 break
-            dy = zone_plain_dy(sim)
+            dy = zone_plain_dy(sim, b)
             m = min(per_snapshot, len(t))
             pick = stratified_pick(dy, m, rng) if stratify else rng.choice(len(t), size=m, replace=False)
             for lst, a in zip(out, (t, p, y, dt, dy, np.full(len(t), s))):
                 lst.append(a[pick])
-            log(f"[harvest] iteration {s}: T {t.min():.0f}..{t.max():.0f} K, dt {dt.min():.2e}..{dt.max():.2e} s, "
-                f"max|dY| {np.abs(dy).max():.2e}, {len(pick)} picked, {time.time() - t0:.1f}s")
+            log(f"[harvest] iteration {s} block {b}: T {t.min():.0f}..{t.max():.0f} K, "
+                f"dt {dt.min():.2e}..{dt.max():.2e} s, max|dY| {np.abs(dy).max():.2e}, {len(pick)} picked, "
+                f"{time.time() - t0:.1f}s")
+            del t, p, y, dt, dy
+        if stop:
+            break
     sim.close()
     return tuple(np.concatenate(v) for v in out)
 
