@@ -36,9 +36,6 @@ HBM_PEAK_GBS = 8000.0
 # the kernel changes; a stage without an entry reports null.
 MEASURED_TRAFFIC = {
     ("config2_air_sst_dplur", "dplur_sweep"): 496.6e6,
-    ("config2_air_sst_dplur", "flux_i"): 382.5e6,
-    ("config2_air_sst_dplur", "flux_j"): 425.8e6,
-    ("config2_air_sst_dplur", "flux_k"): 379.0e6,
 }
 
 
@@ -93,7 +90,7 @@ def alg_scalars(nv: int, rans: bool, dplur: bool):
     turb = 5 if rans else 0  # mut, lambda_t, wall distance (+ rho, k, omega already in bv/sv)
     flux = 6 + nsc + 9 + 1 + 2 + (turb - 2 if rans else 0) + 2 * nv  # bv, sv, metric, J, mu, lambda (+mut, lt, d), dq RMW
     return {
-        "flux_i": flux, "flux_j": flux, "flux_k": flux,
+        "flux_i": flux, "flux_j": flux, "flux_k": flux, "flux_ij": flux,
         "source": (4 + nsc + 9 + 1 + 1 + 1) + (2 + nv + 2) if rans else nv,
         "local_dt": 9 + 3 + 3 + (1 if rans else 0) + 4,
         "dplur_dq0": nv + 1 + 3 + (2 if rans else 0) + nv,

@@ -11,6 +11,8 @@
 //  * j/k-sweeps: tile = 64 cells (i) x L cells along the sweep; faces are
 //    enumerated i-fastest so every wavefront reads one contiguous i-row.
 // The tile's first face is shared with the previous tile (recomputed, 1/L).
+#include <cstdlib>
+
 #include "ostc_kernels.h"
 #include "ostc_thermo.h"
 
@@ -368,6 +370,490 @@ __global__ __launch_bounds__(256) void sweep_kernel(ZoneView<R> z, const ostc_pa
   }
 }
 
+// ---------------------------------------------------------------------------
+// Marching flux kernels for single-species air (NS = 0).  The per-face sweeps
+// above load every stencil value of every face from global memory (~110 load
+// instructions per j/k face); the vector-memory address unit, not HBM, bounds
+// them.  Here each wave owns a 64-lane segment of an i-row (cells
+// i0-2 .. i0+61, of which lanes 2..61 are written) and marches along j or k:
+// every cell is loaded once, i-neighbours come from DPP lane shifts, the
+// marched direction from a register window of rows, and only the central
+// differences across the third direction are loaded (two cells per value).
+//  * MARCH = 1: i-faces and j-faces, dq = (dq - d_i) - d_j
+//  * MARCH = 2: k-faces, dq -= d_k
+// Same face arithmetic as inviscid_face / viscous_face, with the cross
+// derivatives 0.25*(f(L+e) - f(L-e) + f(R+e) - f(R-e)) formed as
+// 0.25*(D_e f(L) + D_e f(R)), D_e f = f(+e) - f(-e).
+
+// value of lane l-1 / l+1 (DPP wave shift; the first / last lane keeps x)
+template <typename R>
+__device__ __forceinline__ R lane_prev(R x) {
+  if constexpr (sizeof(R) == 4) {
+    const int v = __builtin_bit_cast(int, x);
+    return __builtin_bit_cast(R, __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false));
+  } else {
+    const long long v = __builtin_bit_cast(long long, x);
+    const int lo = (int)v, hi = (int)(v >> 32);
+    const int a = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
+    const int b = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(R, (long long)(((unsigned long long)(unsigned)b << 32) | (unsigned)a));
+  }
+}
+template <typename R>
+__device__ __forceinline__ R lane_next(R x) {
+  if constexpr (sizeof(R) == 4) {
+    const int v = __builtin_bit_cast(int, x);
+    return __builtin_bit_cast(R, __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false));
+  } else {
+    const long long v = __builtin_bit_cast(long long, x);
+    const int lo = (int)v, hi = (int)(v >> 32);
+    const int a = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
+    const int b = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(R, (long long)(((unsigned long long)(unsigned)b << 32) | (unsigned)a));
+  }
+}
+
+// what a face needs of each of its two cells
+template <typename R, int NT>
+struct FaceCell {
+  R pv[5 + NT];  // rho u v w p [k omega]
+  R t;
+  R m[9];        // metric (xi_x xi_y xi_z eta_x ... zeta_z)
+  R jac, mul, mut, lam, lamt, wd;
+};
+
+template <typename R, int NT>
+__device__ __forceinline__ void face_cell_load(const ZoneView<R> &z, int64_t q, FaceCell<R, NT> &c, bool visc) {
+  const int64_t P = z.plane;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) c.pv[l] = z.bv[l * P + q];
+#pragma unroll
+  for (int l = 0; l < NT; ++l) c.pv[5 + l] = z.sv[l * P + q];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) c.m[r] = z.metric[r * P + q];
+  c.jac = z.jac[q];
+  c.t = c.mul = c.mut = c.lam = c.lamt = c.wd = 0;
+  if (visc) {
+    c.t = z.bv[5 * P + q];
+    c.mul = z.mul[q];
+    c.lam = z.lambda[q];
+    if constexpr (NT > 0) {
+      c.mut = z.mut[q];
+      c.lamt = z.lambda_t[q];
+      c.wd = z.wall_dist[q];
+    }
+  }
+}
+
+// gradient variables: u v w T [k omega]
+template <typename R, int NT>
+__device__ __forceinline__ R grad_var(const FaceCell<R, NT> &c, int g) {
+  return g < 3 ? c.pv[1 + g] : (g == 3 ? c.t : c.pv[5 + (g - 4)]);
+}
+
+template <typename R, int NT>
+__device__ __forceinline__ void face_cell_shift(const FaceCell<R, NT> &a, FaceCell<R, NT> &b, bool next) {
+#pragma unroll
+  for (int l = 0; l < 5 + NT; ++l) b.pv[l] = next ? lane_next(a.pv[l]) : lane_prev(a.pv[l]);
+#pragma unroll
+  for (int r = 0; r < 9; ++r) b.m[r] = next ? lane_next(a.m[r]) : lane_prev(a.m[r]);
+  b.t = next ? lane_next(a.t) : lane_prev(a.t);
+  b.jac = next ? lane_next(a.jac) : lane_prev(a.jac);
+  b.mul = next ? lane_next(a.mul) : lane_prev(a.mul);
+  b.mut = next ? lane_next(a.mut) : lane_prev(a.mut);
+  b.lam = next ? lane_next(a.lam) : lane_prev(a.lam);
+  b.lamt = next ? lane_next(a.lamt) : lane_prev(a.lamt);
+  b.wd = next ? lane_next(a.wd) : lane_prev(a.wd);
+}
+
+// minmod of minmod() above in select form (same values bit for bit:
+// sgn(0) = +1, equal signs -> +-min(|a|,|b|), else +0)
+template <typename R>
+__device__ __forceinline__ R minmod_sel(R a, R b) {
+  const bool na = a < (R)0, nb = b < (R)0;
+  const R m = dmin(fabs(a), fabs(b));
+  return na == nb ? (na ? -m : m) : (R)0;
+}
+
+// AUSM+ (src/InviscidScheme.cuh:70-149) on values: pv = points L-1, L, R, R+1.
+// minmod is symmetric, so MUSCL's minmod(d_i, d_i1) and minmod(d_i1, d_i)
+// are one value (formed once).
+template <typename R, int NT, int DIR>
+__device__ __forceinline__ void inviscid_face_v(const ostc_param *__restrict__ p, const R (&pv)[4][5 + NT],
+                                                const FaceCell<R, NT> &cl, const FaceCell<R, NT> &cr,
+                                                R *__restrict__ fc) {
+  constexpr int NV = 5 + NT;
+  R pl[NV], pr[NV];
+  const int rec = p->reconstruction;
+  if (rec == 2) {
+    const R kappa = (R)(1.0 / 3.0);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      const R d_i = pv[1][l] - pv[0][l], d_i1 = pv[2][l] - pv[1][l], d_i2 = pv[3][l] - pv[2][l];
+      const R ml = minmod_sel(d_i, d_i1), mr = minmod_sel(d_i1, d_i2);
+      pl[l] = pv[1][l] + (R)0.25 * (((R)1 - kappa) * ml + ((R)1 + kappa) * ml);
+      pr[l] = pv[2][l] - (R)0.25 * (((R)1 - kappa) * mr + ((R)1 + kappa) * mr);
+    }
+  } else if (rec == 3) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      const R d_i = pv[1][l] - pv[0][l], d_i1 = pv[2][l] - pv[1][l], d_i2 = pv[3][l] - pv[2][l];
+      pl[l] = pv[1][l] + (R)0.5 * minmod_sel(d_i, d_i1);
+      pr[l] = pv[2][l] - (R)0.5 * minmod_sel(d_i1, d_i2);
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      pl[l] = pv[1][l];
+      pr[l] = pv[2][l];
+    }
+  }
+  const R el = (R)0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+  const R er = (R)0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+  const R El = el * pl[0] + fdiv<R>(pl[4], (R)(kGammaAir - 1));
+  const R Er = er * pr[0] + fdiv<R>(pr[4], (R)(kGammaAir - 1));
+  const R gam_l = (R)kGammaAir, gam_r = (R)kGammaAir;
+  const R jl = cl.jac, jr = cr.jac;
+  const R k1 = (R)0.5 * (jl * cl.m[DIR * 3] + jr * cr.m[DIR * 3]);
+  const R k2 = (R)0.5 * (jl * cl.m[DIR * 3 + 1] + jr * cr.m[DIR * 3 + 1]);
+  const R k3 = (R)0.5 * (jl * cl.m[DIR * 3 + 2] + jr * cr.m[DIR * 3 + 2]);
+  const R gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+  const R igk = fdiv<R>((R)1, gk);
+  const R ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) * igk;
+  const R ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) * igk;
+  const R c = (R)0.5 * (sqrt(fdiv<R>(gam_l * pl[4], pl[0])) + sqrt(fdiv<R>(gam_r * pr[4], pr[0])));
+  const R ic = fdiv<R>((R)1, c);
+  const R mach_l = ul * ic, mach_r = ur * ic;
+  R mlp, mrn, plp, prn;
+  const R alpha = (R)(3 / 16.0);
+  if (fabs(mach_l) > (R)1) {
+    mlp = (R)0.5 * (mach_l + fabs(mach_l));
+    plp = fdiv<R>(mlp, mach_l);
+  } else {
+    const R a = (mach_l + (R)1) * (mach_l + (R)1) * (R)0.25;
+    const R b = (mach_l * mach_l - (R)1) * (mach_l * mach_l - (R)1);
+    mlp = a + (R)0.125 * b;
+    plp = a * ((R)2 - mach_l) + alpha * mach_l * b;
+  }
+  if (fabs(mach_r) > (R)1) {
+    mrn = (R)0.5 * (mach_r - fabs(mach_r));
+    prn = fdiv<R>(mrn, mach_r);
+  } else {
+    const R a = (mach_r - (R)1) * (mach_r - (R)1) * (R)0.25;
+    const R b = (mach_r * mach_r - (R)1) * (mach_r * mach_r - (R)1);
+    mrn = -a - (R)0.125 * b;
+    prn = a * ((R)2 + mach_r) - alpha * mach_r * b;
+  }
+  const R p_coeff = plp * pl[4] + prn * pr[4];
+  const R m_half = mlp + mrn;
+  const R mach_pos = (R)0.5 * (m_half + fabs(m_half));
+  const R mach_neg = (R)0.5 * (m_half - fabs(m_half));
+  const R mass_flux = c * (pl[0] * mach_pos + pr[0] * mach_neg);
+  const R coeff = mass_flux * gk;
+  const bool up = mass_flux >= (R)0;
+  fc[0] = coeff;
+  fc[1] = coeff * (up ? pl[1] : pr[1]) + p_coeff * k1;
+  fc[2] = coeff * (up ? pl[2] : pr[2]) + p_coeff * k2;
+  fc[3] = coeff * (up ? pl[3] : pr[3]) + p_coeff * k3;
+  fc[4] = up ? fdiv<R>(coeff * (El + pl[4]), pl[0]) : fdiv<R>(coeff * (Er + pr[4]), pr[0]);
+#pragma unroll
+  for (int l = 5; l < NV; ++l) fc[l] = coeff * (up ? pl[l] : pr[l]);
+}
+
+// viscous face flux (src/ViscousScheme.cuh:22-219) on values; dl/dr[e][g] =
+// D_e of gradient variable g at L / R (row e == DIR unused)
+template <typename R, int NT, int DIR>
+__device__ __forceinline__ void viscous_face_v(const ostc_param *__restrict__ p, const FaceCell<R, NT> &cl,
+                                               const FaceCell<R, NT> &cr, const R (&dl)[3][4 + NT],
+                                               const R (&dr)[3][4 + NT], R *__restrict__ fv) {
+  constexpr int NG = 4 + NT;
+  R mm[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) mm[q] = (R)0.5 * (cl.m[q] + cr.m[q]);
+  const R xi_x = mm[0], xi_y = mm[1], xi_z = mm[2];
+  const R eta_x = mm[3], eta_y = mm[4], eta_z = mm[5];
+  const R zeta_x = mm[6], zeta_y = mm[7], zeta_z = mm[8];
+  R d[NG][3];
+#pragma unroll
+  for (int gq = 0; gq < NG; ++gq)
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      d[gq][e] = e == DIR ? grad_var(cr, gq) - grad_var(cl, gq) : (R)0.25 * (dl[e][gq] + dr[e][gq]);
+  const R *du = d[0], *dv = d[1], *dw = d[2], *dt = d[3];
+  const R u_x = du[0] * xi_x + du[1] * eta_x + du[2] * zeta_x;
+  const R u_y = du[0] * xi_y + du[1] * eta_y + du[2] * zeta_y;
+  const R u_z = du[0] * xi_z + du[1] * eta_z + du[2] * zeta_z;
+  const R v_x = dv[0] * xi_x + dv[1] * eta_x + dv[2] * zeta_x;
+  const R v_y = dv[0] * xi_y + dv[1] * eta_y + dv[2] * zeta_y;
+  const R v_z = dv[0] * xi_z + dv[1] * eta_z + dv[2] * zeta_z;
+  const R w_x = dw[0] * xi_x + dw[1] * eta_x + dw[2] * zeta_x;
+  const R w_y = dw[0] * xi_y + dw[1] * eta_y + dw[2] * zeta_y;
+  const R w_z = dw[0] * xi_z + dw[1] * eta_z + dw[2] * zeta_z;
+  const R mul = (R)0.5 * (cl.mul + cr.mul);
+  R mut = 0;
+  if constexpr (NT > 0) mut = (R)0.5 * (cl.mut + cr.mut);
+  const R vis = mul + mut;
+  R tau_xx = fdiv<R>(vis * ((R)4 * u_x - (R)2 * v_y - (R)2 * w_z), (R)3);
+  R tau_yy = fdiv<R>(vis * ((R)4 * v_y - (R)2 * u_x - (R)2 * w_z), (R)3);
+  R tau_zz = fdiv<R>(vis * ((R)4 * w_z - (R)2 * u_x - (R)2 * v_y), (R)3);
+  const R tau_xy = vis * (u_y + v_x), tau_xz = vis * (u_z + w_x), tau_yz = vis * (v_z + w_y);
+  if constexpr (NT > 0) {
+    const R tt = (R)(-2.0 / 3) * (R)0.5 * (cl.pv[0] * cl.pv[5] + cr.pv[0] * cr.pv[5]);
+    tau_xx += tt;
+    tau_yy += tt;
+    tau_zz += tt;
+  }
+  const R ja = cl.jac, jb = cr.jac;
+  const R kx = (R)0.5 * (cl.m[DIR * 3] * ja + cr.m[DIR * 3] * jb);
+  const R ky = (R)0.5 * (cl.m[DIR * 3 + 1] * ja + cr.m[DIR * 3 + 1] * jb);
+  const R kz = (R)0.5 * (cl.m[DIR * 3 + 2] * ja + cr.m[DIR * 3 + 2] * jb);
+  fv[0] = 0;
+  fv[1] = kx * tau_xx + ky * tau_xy + kz * tau_xz;
+  fv[2] = kx * tau_xy + ky * tau_yy + kz * tau_yz;
+  fv[3] = kx * tau_xz + ky * tau_yz + kz * tau_zz;
+  const R um = (R)0.5 * (cl.pv[1] + cr.pv[1]);
+  const R vm = (R)0.5 * (cl.pv[2] + cr.pv[2]);
+  const R wm = (R)0.5 * (cl.pv[3] + cr.pv[3]);
+  const R t_x = dt[0] * xi_x + dt[1] * eta_x + dt[2] * zeta_x;
+  const R t_y = dt[0] * xi_y + dt[1] * eta_y + dt[2] * zeta_y;
+  const R t_z = dt[0] * xi_z + dt[1] * eta_z + dt[2] * zeta_z;
+  R cond = (R)0.5 * (cl.lam + cr.lam);
+  if constexpr (NT > 0) cond += (R)0.5 * (cl.lamt + cr.lamt);
+  fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + cond * (kx * t_x + ky * t_y + kz * t_z);
+  if constexpr (NT > 0) {
+    const R *dk = d[4], *dom = d[5];
+    const R k_x = dk[0] * xi_x + dk[1] * eta_x + dk[2] * zeta_x;
+    const R k_y = dk[0] * xi_y + dk[1] * eta_y + dk[2] * zeta_y;
+    const R k_z = dk[0] * xi_z + dk[1] * eta_z + dk[2] * zeta_z;
+    const R o_x = dom[0] * xi_x + dom[1] * eta_x + dom[2] * zeta_x;
+    const R o_y = dom[0] * xi_y + dom[1] * eta_y + dom[2] * zeta_y;
+    const R o_z = dom[0] * xi_z + dom[1] * eta_z + dom[2] * zeta_z;
+    const R wd = (R)0.5 * (cl.wd + cr.wd);
+    R f1 = 1;
+    if (wd > (R)1e-25) {
+      const R km = (R)0.5 * (cl.pv[5] + cr.pv[5]);
+      const R om = (R)0.5 * (cl.pv[6] + cr.pv[6]);
+      const R param1 = fdiv<R>(sqrt(km), (R)0.09 * om * wd);
+      const R rhom = (R)0.5 * (cl.pv[0] + cr.pv[0]);
+      const R d2 = wd * wd;
+      const R param2 = fdiv<R>((R)500 * mul, rhom * d2 * om);
+      const R cd = dmax((R)1e-20, fdiv<R>((R)2 * rhom * (R)sst::sigma_omega2, om) * (k_x * o_x + k_y * o_y + k_z * o_z));
+      const R param3 = fdiv<R>((R)4 * rhom * (R)sst::sigma_omega2 * km, cd * d2);
+      const R arg1 = dmin(dmax(param1, param2), param3);
+      f1 = tanh(arg1 * arg1 * arg1 * arg1);
+    }
+    const R sigma_k = (R)sst::sigma_k2 + (R)sst::delta_sigma_k * f1;
+    const R sigma_o = (R)sst::sigma_omega2 + (R)sst::delta_sigma_omega * f1;
+    fv[5] = (mul + mut * sigma_k) * (kx * k_x + ky * k_y + kz * k_z);
+    fv[6] = (mul + mut * sigma_o) * (kx * o_x + ky * o_y + kz * o_z);
+  }
+}
+
+template <typename R, int NT, int DIR, bool INV, bool VISC>
+__device__ __forceinline__ void face_flux_v(const ostc_param *__restrict__ p, const R (&pv)[4][5 + NT],
+                                            const FaceCell<R, NT> &cl, const FaceCell<R, NT> &cr,
+                                            const R (&dl)[3][4 + NT], const R (&dr)[3][4 + NT], R *__restrict__ g) {
+  constexpr int NV = 5 + NT;
+  if constexpr (INV) {
+    inviscid_face_v<R, NT, DIR>(p, pv, cl, cr, g);
+  } else {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) g[l] = 0;
+  }
+  if constexpr (VISC) {
+    R v[NV];
+    viscous_face_v<R, NT, DIR>(p, cl, cr, dl, dr, v);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) g[l] -= v[l];
+  }
+}
+
+constexpr int kSegUseful = 60;  // written lanes 2..61 of a 64-lane segment
+
+template <typename R, int NT, int MARCH, bool INV, bool VISC, bool ACCUM>
+__global__ __launch_bounds__(256) void k_flux_march(ZoneView<R> z, const ostc_param *__restrict__ p, int mlen) {
+  constexpr int NV = 5 + NT, NG = 4 + NT;
+  constexpr int X = 3 - MARCH;  // the direction whose central differences are loaded
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const uint3 bk = xcd_block();
+  const int ext[3] = {z.mx, z.my, z.mz};
+  const int i = (int)bk.x * kSegUseful - 2 + lane;
+  const int ic = min(i, z.mx + 1);
+  const bool useful = lane >= 2 && lane < 2 + kSegUseful && i < z.mx;
+  const int cx = (int)bk.y * (int)blockDim.y + w;
+  if (cx >= ext[X]) return;
+  const int m0 = (int)bk.z * mlen, m1 = min(m0 + mlen, ext[MARCH]);
+  const int64_t sX = X == 1 ? (int64_t)z.nx : (int64_t)z.nx * z.ny;
+  const int64_t P = z.plane;
+  auto cell = [&](int m) -> int64_t { return MARCH == 1 ? z.idx(ic, m, cx) : z.idx(ic, cx, m); };
+  auto load_dx = [&](int64_t q, R *dx) {
+#pragma unroll
+    for (int gq = 0; gq < NG; ++gq) {
+      const R *f = gq < 3 ? z.bv + (1 + gq) * P : (gq == 3 ? z.bv + 5 * P : z.sv + (gq - 4) * P);
+      dx[gq] = f[q + sX] - f[q - sX];
+    }
+  };
+  // window: pvA = row c-2 (pv only); cw[0..2] = rows c-1, c, c+1; dxw = their
+  // D_X.  (Reading row c+2 one step ahead instead costs the k kernel a wave
+  // per SIMD and measured slower.)
+  R pvA[NV];
+  FaceCell<R, NT> cw[3];
+  R dxw[3][NG];
+  R gprev[NV], diprev[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) gprev[l] = diprev[l] = 0;
+  // prologue: rows m0-2, m0-1, m0
+  {
+    FaceCell<R, NT> c0;
+    face_cell_load<R, NT>(z, cell(m0 - 2), c0, VISC);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) pvA[l] = c0.pv[l];
+  }
+  face_cell_load<R, NT>(z, cell(m0 - 1), cw[0], VISC);
+  face_cell_load<R, NT>(z, cell(m0), cw[1], VISC);
+  if (VISC) {
+    load_dx(cell(m0 - 1), dxw[0]);
+    load_dx(cell(m0), dxw[1]);
+  }
+  for (int r = m0 + 1; r <= m1 + 1; ++r) {
+    // load row r = c+1
+    face_cell_load<R, NT>(z, cell(r), cw[2], VISC);
+    if (VISC) load_dx(cell(r), dxw[2]);
+    const int c = r - 1;
+    // derivative sets of rows c-1 (L of the marched face) and c (R)
+    R dL[3][NG], dR[3][NG];
+#pragma unroll
+    for (int gq = 0; gq < NG; ++gq) {
+      const R f0 = grad_var(cw[0], gq), f1 = grad_var(cw[1], gq);
+      dL[0][gq] = lane_next(f0) - lane_prev(f0);
+      dR[0][gq] = lane_next(f1) - lane_prev(f1);
+      dL[X][gq] = dxw[0][gq];
+      dR[X][gq] = dxw[1][gq];
+      dL[MARCH][gq] = 0;
+      // D_MARCH of row c (cross term of its i-faces)
+      dR[MARCH][gq] = grad_var(cw[2], gq) - f0;
+    }
+    R acc[NV];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) acc[l] = 0;
+    if constexpr (MARCH == 1) {
+      // i-faces of row c: each lane the face on its left (L = lane-1, R = lane)
+      FaceCell<R, NT> cm;
+      face_cell_shift<R, NT>(cw[1], cm, false);
+      R pvi[4][NV];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        pvi[2][l] = cw[1].pv[l];
+        pvi[1][l] = cm.pv[l];
+        pvi[0][l] = lane_prev(cm.pv[l]);
+        pvi[3][l] = lane_next(cw[1].pv[l]);
+      }
+      R dli[3][NG], dri[3][NG];
+#pragma unroll
+      for (int gq = 0; gq < NG; ++gq) {
+        dri[0][gq] = 0;
+        dri[1][gq] = dR[1][gq];
+        dri[2][gq] = dR[2][gq];
+        dli[0][gq] = 0;
+        dli[1][gq] = lane_prev(dR[1][gq]);
+        dli[2][gq] = lane_prev(dR[2][gq]);
+      }
+      R gi[NV];
+      face_flux_v<R, NT, 0, INV, VISC>(p, pvi, cm, cw[1], dli, dri, gi);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) acc[l] = lane_next(gi[l]) - gi[l];  // d_i(c)
+    }
+    // marched face c-1/2 between rows c-1 and c
+    {
+      R pvm[4][NV];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        pvm[0][l] = pvA[l];
+        pvm[1][l] = cw[0].pv[l];
+        pvm[2][l] = cw[1].pv[l];
+        pvm[3][l] = cw[2].pv[l];
+      }
+      R dRm[3][NG];
+#pragma unroll
+      for (int gq = 0; gq < NG; ++gq) {
+        dRm[0][gq] = dR[0][gq];
+        dRm[X][gq] = dR[X][gq];
+        dRm[MARCH][gq] = 0;
+      }
+      R gm[NV];
+      face_flux_v<R, NT, MARCH, INV, VISC>(p, pvm, cw[0], cw[1], dL, dRm, gm);
+      // row c-1 is complete: d_M(c-1) = G(c-1/2) - G(c-3/2)
+      const int cm1 = c - 1;
+      if (useful && cm1 >= m0 && cm1 < m1) {
+        const int64_t q = cell(cm1);
+#pragma unroll
+        for (int l = 0; l < NV; ++l) {
+          const R dmv = gm[l] - gprev[l];
+          R v;
+          if constexpr (MARCH == 1) {
+            const R base = ACCUM ? z.dq[q + l * P] - diprev[l] : -diprev[l];
+            v = base - dmv;
+          } else {
+            v = z.dq[q + l * P] - dmv;
+          }
+          z.dq[q + l * P] = v;
+        }
+      }
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        gprev[l] = gm[l];
+        diprev[l] = acc[l];
+      }
+    }
+    // shift the window
+#pragma unroll
+    for (int l = 0; l < NV; ++l) pvA[l] = cw[0].pv[l];
+    cw[0] = cw[1];
+    cw[1] = cw[2];
+#pragma unroll
+    for (int gq = 0; gq < NG; ++gq) {
+      dxw[0][gq] = dxw[1][gq];
+      dxw[1][gq] = dxw[2][gq];
+    }
+  }
+}
+
+template <typename R, int NT>
+static int launch_march(const ZoneView<R> &z, const ostc_param *dp, bool inv, bool visc, bool first_writer,
+                        hipStream_t st) {
+  const int segs = (z.mx + kSegUseful - 1) / kSegUseful;
+  // rows marched per wave (launch tuning; 3 extra rows are loaded per chunk)
+  static const int mlen = [] {
+    const char *e = std::getenv("OSTC_FLUX_MLEN");
+    const int v = e ? std::atoi(e) : 8;
+    return v < 1 ? 1 : v;
+  }();
+  const dim3 block(64, 4);
+#define OSTC_MARCH(M, I, V, A) \
+  hipLaunchKernelGGL((k_flux_march<R, NT, M, I, V, A>), grid, block, 0, st, z, dp, mlen)
+  for (int m = 1; m <= 2; ++m) {
+    if (m == 2 && z.mz == 1) break;
+    const int X = 3 - m;
+    const int extX = X == 1 ? z.my : z.mz, extM = m == 1 ? z.my : z.mz;
+    const dim3 grid((unsigned)segs, (unsigned)((extX + 3) / 4), (unsigned)((extM + mlen - 1) / mlen));
+    const bool acc = !(first_writer && m == 1);
+    prof_mark(m == 1 ? P_FLUX_IJ : P_FLUX_K, st, true);
+    if (m == 1) {
+      if (inv && visc) { if (acc) OSTC_MARCH(1, true, true, true); else OSTC_MARCH(1, true, true, false); }
+      else if (inv) { if (acc) OSTC_MARCH(1, true, false, true); else OSTC_MARCH(1, true, false, false); }
+      else { if (acc) OSTC_MARCH(1, false, true, true); else OSTC_MARCH(1, false, true, false); }
+    } else {
+      if (inv && visc) OSTC_MARCH(2, true, true, true);
+      else if (inv) OSTC_MARCH(2, true, false, true);
+      else OSTC_MARCH(2, false, true, true);
+    }
+    prof_mark(m == 1 ? P_FLUX_IJ : P_FLUX_K, st, false);
+    if (hipGetLastError() != hipSuccess) return OSTC_ERR_HIP;
+  }
+#undef OSTC_MARCH
+  return OSTC_OK;
+}
+
 template <typename R, int NS, int NT>
 static int launch_sweeps(const ZoneView<R> &z, const ostc_param *dp, bool inv, bool visc, bool first_writer,
                          hipStream_t st) {
@@ -427,8 +913,8 @@ static int launch_sweeps(const ZoneView<R> &z, const ostc_param *dp, bool inv, b
 template <typename R>
 int flux_sweeps(const ZoneView<R> &z, const ostc_param *dp, int ns, int nt, bool inv, bool visc, bool first_writer,
                 hipStream_t st) {
-  if (ns == 0 && nt == 0) return launch_sweeps<R, 0, 0>(z, dp, inv, visc, first_writer, st);
-  if (ns == 0 && nt == 2) return launch_sweeps<R, 0, 2>(z, dp, inv, visc, first_writer, st);
+  if (ns == 0 && nt == 0) return launch_march<R, 0>(z, dp, inv, visc, first_writer, st);
+  if (ns == 0 && nt == 2) return launch_march<R, 2>(z, dp, inv, visc, first_writer, st);
   if (ns == 9 && nt == 0) return launch_sweeps<R, 9, 0>(z, dp, inv, visc, first_writer, st);
   if (ns == 9 && nt == 2) return launch_sweeps<R, 9, 2>(z, dp, inv, visc, first_writer, st);
   return OSTC_ERR_ARG;
