@@ -35,7 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # profiles/kernels_rans_256x128x64_fr_192x96x96_fourblock_4x256x192x128.md.  Re-measured whenever
 # the kernel changes; a stage without an entry reports null.
 MEASURED_TRAFFIC = {
-    ("config2_air_sst_dplur", "dplur_sweep"): 496.6e6,
+    ("config2_air_sst_dplur", "dplur_sweep"): 471.1e6,
+    ("config2_air_sst_dplur", "flux_ij"): 402.8e6,
+    ("config2_air_sst_dplur", "flux_k"): 402.2e6,
+    ("config2_air_sst_dplur", "source"): 312.0e6,
+    ("config2_air_sst_dplur", "update"): 273.7e6,
 }
 
 
@@ -381,6 +385,8 @@ def main():
             st["alg_bytes_per_launch"] = byts
             st["achieved_GBs"] = byts / (avg_ms * 1e-3) / 1e9
             st["frac_hbm_peak"] = st["achieved_GBs"] / HBM_PEAK_GBS
+        if (case.name, n) in MEASURED_TRAFFIC:
+            st["traffic_bytes_per_launch"] = MEASURED_TRAFFIC[(case.name, n)]
         stages[n] = st
     dom = max((n for n in stages if "alg_bytes_per_launch" in stages[n]), key=lambda n: stages[n]["total_ms"])
     d = stages[dom]
