@@ -28,7 +28,8 @@ enum ostc_status {
   OSTC_ERR_HIP = 2,       /* a HIP runtime call failed */
   OSTC_ERR_ALLOC = 3,     /* device allocation failed */
   OSTC_ERR_NAN = 4,       /* residual is NaN (reference aborts: src/Driver.cu:372-377) */
-  OSTC_ERR_STATE = 5      /* call out of order (e.g. zone not initialised) */
+  OSTC_ERR_STATE = 5,     /* call out of order (e.g. zone not initialised) */
+  OSTC_ERR_IO = 6         /* a file could not be read or written */
 };
 
 /* Mixture / turbulence models: src/Define.h:14-27 */
@@ -239,13 +240,21 @@ const char *ostc_case_error(void);
  * steps (res[4] normalised by the step-1 values, err_max their max,
  * src/Driver.cu:291-380); simulate = steady_simulation up to total_step or
  * convergence, with io != 0 printing the reference's screen lines and
- * writing history.dat and output/message/residual_scale.txt under the case
- * directory.  The driver keeps no pointer to the case. */
+ * writing history.dat, output/message/residual_scale.txt and
+ * output/field/flowfield.plt under the case directory.  The driver keeps no
+ * pointer to the case. */
 typedef struct ostc_driver ostc_driver;
 int ostc_driver_create(const ostc_case *c, int precision, ostc_driver **out);
 int ostc_driver_initialize(ostc_driver *d);
 int ostc_driver_step(ostc_driver *d, double res[4], int *res_computed, double *err_max);
 int ostc_driver_simulate(ostc_driver *d, int io, int *steps_done, double *err_max);
+/* the flow-field file of MPIIO::print_field (src/MPIIO.hpp:60-548): Tecplot
+ * binary V112, one ORDERED zone per block without ghost layers, doubles;
+ * variables x y z density u v w pressure temperature mach, species, tke
+ * omega mut (RANS).  path NULL = <case dir>/output/field/flowfield.plt, the
+ * file simulate(io != 0) writes every output_file steps and at convergence
+ * and a setup with initial = 1 restarts from. */
+int ostc_driver_write_flowfield(ostc_driver *d, const char *path);
 int ostc_driver_get_zone(ostc_driver *d, int block, ostc_zone **out);
 int ostc_driver_get_ctx(ostc_driver *d, ostc_ctx **out);
 int ostc_driver_get_step(const ostc_driver *d, int *step);

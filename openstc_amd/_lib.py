@@ -210,6 +210,7 @@ SIGNATURES = {
     "ostc_driver_initialize": (C.c_int, [_P]),
     "ostc_driver_step": (C.c_int, [_P, _D, _I, _D]),
     "ostc_driver_simulate": (C.c_int, [_P, C.c_int, _I, _D]),
+    "ostc_driver_write_flowfield": (C.c_int, [_P, C.c_char_p]),
     "ostc_driver_get_zone": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "ostc_driver_get_ctx": (C.c_int, [_P, C.POINTER(_P)]),
     "ostc_driver_get_step": (C.c_int, [_P, _I]),

@@ -249,6 +249,12 @@ class Driver:
                     "ostc_driver_simulate")
         return done.value, err.value
 
+    def write_flowfield(self, path: str | None = None):
+        """The reference's flow-field file (Tecplot binary); default
+        <case>/output/field/flowfield.plt."""
+        self._check(self.lib.ostc_driver_write_flowfield(self.h, path.encode() if path else None),
+                    "ostc_driver_write_flowfield")
+
     def zone(self, b: int):
         z = C.c_void_p()
         self._check(self.lib.ostc_driver_get_zone(self.h, b, C.byref(z)), "ostc_driver_get_zone")

@@ -749,8 +749,8 @@ int read_case(const char *dir_c, ostc_case *c) {
   if (!need.missing.empty()) return case_fail("setup is missing " + need.missing);
   if (parallel) return case_fail("parallel = 1: one process drives every block here (no parallel faces)");
   if (!c->steady) return case_fail("steady = 0: only the steady time loop is on this path");
-  if (c->initial != 0)
-    return case_fail("initial = 1 (restart from output/field) is not supported; inject fields with ostc_case_field");
+  if (c->initial != 0 && c->initial != 1)
+    return case_fail("initial must be 0 (from the setup's inflows) or 1 (restart from output/field/flowfield.plt)");
   if (c->dimension != 3) return case_fail("dimension must be 3");
   if (p.inviscid_scheme / 10 == 5) return case_fail("WENO schemes are not on this path");
   if (p.inviscid_scheme != 3) return case_fail("inviscid_scheme must be 3 (AUSM+)");
@@ -820,6 +820,10 @@ int read_case(const char *dir_c, ostc_case *c) {
   cx.sst = p.rans_model == 2;
   if (int e = boundary_programs(cx)) return e;
   if (int e = initial_fields(cx)) return e;
+  if (c->initial == 1) {
+    std::string err;
+    if (ostc_host::restart_fields(*c, &err)) return case_fail("restart: " + err);
+  }
   if (!cx.have_inf) {
     InflowState fs;
     if (int e = make_inflow(cx, "freestream", &fs)) {

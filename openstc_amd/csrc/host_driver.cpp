@@ -6,7 +6,10 @@
 //  * ostc_driver_initialize: initialize_computation (:43-104) with the wall
 //    distance of acquire_wall_distance (:122-189);
 //  * ostc_driver_step / ostc_driver_simulate: steady_simulation (:191-289),
-//    compute_residual (:291-380) and steady_screen_output (:382-401).
+//    compute_residual (:291-380) and steady_screen_output (:382-401);
+//  * ostc_driver_write_flowfield: MPIIO::print_field (src/MPIIO.hpp), the
+//    flow-field file simulate writes every output_file steps and at
+//    convergence (src/Driver.cu:283-286).
 // Model dispatch (src/main.cu:22-64) is the runtime model of ostc_param.
 #include <chrono>
 #include <cmath>
@@ -22,7 +25,8 @@ namespace {
 thread_local std::string g_drv_err;
 
 int drv_fail(int st, const std::string &msg) {
-  g_drv_err = msg + (st == OSTC_ERR_ARG || st == OSTC_ERR_STATE ? std::string() : std::string(": ") + ostc_last_error());
+  g_drv_err = msg + (st == OSTC_ERR_ARG || st == OSTC_ERR_STATE || st == OSTC_ERR_IO ? std::string()
+                                                                                      : std::string(": ") + ostc_last_error());
   return st;
 }
 
@@ -48,6 +52,9 @@ struct ostc_driver {
   std::vector<ostc_inner_face> inner;
   std::vector<int> n_inner;
   std::vector<double> wall_xyz;
+  std::vector<std::string> spec_names;  // mechanism order
+  std::vector<int> dims;                // mx, my, mz per block
+  int ngg = 2, output_file = 1;
   double res[4] = {0, 0, 0, 0}, res_scale[4] = {1, 1, 1, 1};
   bool initialized = false;
   std::chrono::steady_clock::time_point t0, t_last;
@@ -71,6 +78,8 @@ int ostc_driver_create(const ostc_case *c, int precision, ostc_driver **out) {
   d->output_screen = c->output_screen;
   d->convergence = c->convergence;
   d->wall_xyz = c->wall_xyz;
+  d->output_file = c->output_file;
+  for (int k = 0; k < c->param.n_spec; ++k) d->spec_names.emplace_back(c->mech.spec_names[k]);
   auto cleanup = [&](int st) {
     delete d;
     return st;
@@ -80,6 +89,7 @@ int ostc_driver_create(const ostc_case *c, int precision, ostc_driver **out) {
     return cleanup(st);
   }
   const int g = c->ngg;
+  d->ngg = g;
   for (size_t b = 0; b < c->blocks.size(); ++b) {
     const ostc_host::CaseBlock &blk = c->blocks[b];
     ostc_zone_desc desc{blk.mx, blk.my, blk.mz, g, precision, (int)b};
@@ -107,6 +117,7 @@ int ostc_driver_create(const ostc_case *c, int precision, ostc_driver **out) {
       drv_fail(st, "ostc_zone_set_bc");
       return cleanup(st);
     }
+    d->dims.insert(d->dims.end(), {blk.mx, blk.my, blk.mz});
     d->n_inner.push_back((int)blk.inner.size());
     d->inner.insert(d->inner.end(), blk.inner.begin(), blk.inner.end());
     d->n_grid += (long)blk.mx * blk.my * blk.mz;
@@ -190,8 +201,16 @@ int ostc_driver_simulate(ostc_driver *d, int io, int *steps_done, double *err_ma
       if (steps_done) *steps_done = done;
       return st;
     }
+    if (computed) err = e;
+    const bool converged = computed && err < d->convergence;
+    if (io && (d->step % d->output_file == 0 || converged)) {
+      const int wst = ostc_driver_write_flowfield(d, nullptr);
+      if (wst != OSTC_OK) {
+        if (steps_done) *steps_done = done;
+        return wst;
+      }
+    }
     if (!computed) continue;
-    err = e;
     if (io) {
       if (d->step == 1) {
         std::error_code ec;
@@ -222,6 +241,57 @@ int ostc_driver_simulate(ostc_driver *d, int io, int *steps_done, double *err_ma
   DRV_TRY(ostc_device_sync(), "synchronise");
   if (steps_done) *steps_done = done;
   if (err_max) *err_max = err;
+  return OSTC_OK;
+}
+
+int ostc_driver_write_flowfield(ostc_driver *d, const char *path) {
+  if (!d) return drv_fail(OSTC_ERR_ARG, "null argument");
+  const std::string file = path ? std::string(path) : d->dir + "/output/field/flowfield.plt";
+  const bool rans = d->param.turb_method == OSTC_RANS;
+  const int ns = d->param.n_spec, nsc = d->param.n_scalar;
+  std::vector<std::string> names{"x", "y", "z", "density", "u", "v", "w", "pressure", "temperature", "mach"};
+  names.insert(names.end(), d->spec_names.begin(), d->spec_names.end());
+  if (rans) {
+    names.emplace_back("tke");
+    names.emplace_back("omega");
+    names.emplace_back("mut");
+  }
+  std::vector<ostc_host::PltZone> zones;
+  for (size_t b = 0; b < d->zones.size(); ++b) {
+    ostc_zone *z = d->zones[b];
+    const int g = d->ngg, mx = d->dims[3 * b], my = d->dims[3 * b + 1], mz = d->dims[3 * b + 2];
+    const long nx = mx + 2 * g, ny = my + 2 * g, plane = nx * ny * (mz + 2 * g);
+    std::vector<double> buf((size_t)plane * std::max(6, nsc));
+    ostc_host::PltZone pz;
+    pz.mx = mx, pz.my = my, pz.mz = mz;
+    auto interior = [&](const double *src) {
+      std::vector<double> out((size_t)mx * my * mz);
+      for (int k = 0; k < mz; ++k)
+        for (int j = 0; j < my; ++j)
+          for (int i = 0; i < mx; ++i)
+            out[((size_t)k * my + j) * mx + i] = src[((long)(k + g) * ny + (j + g)) * nx + (i + g)];
+      return out;
+    };
+    for (int f : {OSTC_F_X, OSTC_F_Y, OSTC_F_Z}) {
+      DRV_TRY(ostc_zone_download(z, f, buf.data(), 1), "download grid");
+      pz.vars.push_back(interior(buf.data()));
+    }
+    DRV_TRY(ostc_zone_download(z, OSTC_F_BV, buf.data(), 6), "download bv");
+    for (int l = 0; l < 6; ++l) pz.vars.push_back(interior(buf.data() + (size_t)l * plane));
+    DRV_TRY(ostc_zone_download(z, OSTC_F_MACH, buf.data(), 1), "download mach");
+    pz.vars.push_back(interior(buf.data()));
+    if (nsc > 0) {
+      DRV_TRY(ostc_zone_download(z, OSTC_F_SV, buf.data(), nsc), "download sv");
+      for (int l = 0; l < ns + (rans ? 2 : 0); ++l) pz.vars.push_back(interior(buf.data() + (size_t)l * plane));
+    }
+    if (rans) {
+      DRV_TRY(ostc_zone_download(z, OSTC_F_MUT, buf.data(), 1), "download mut");
+      pz.vars.push_back(interior(buf.data()));
+    }
+    zones.push_back(std::move(pz));
+  }
+  std::string err;
+  if (ostc_host::write_plt(file, names, zones, &err)) return drv_fail(OSTC_ERR_IO, err);
   return OSTC_OK;
 }
 

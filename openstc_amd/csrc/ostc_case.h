@@ -63,6 +63,24 @@ struct CaseBlock {
   long plane(int ngg) const { return (long)(mx + 2 * ngg) * (my + 2 * ngg) * (mz + 2 * ngg); }
 };
 
+// one ORDERED zone of a Tecplot binary flow-field file: vars[v][i + mx*(j + my*k)]
+struct PltZone {
+  int mx = 0, my = 0, mz = 0;
+  std::vector<std::vector<double>> vars;
+};
+
+}  // namespace ostc_host
+
+struct ostc_case;
+
+namespace ostc_host {
+// the reference's flow-field file (host_output.cpp); 0 = ok, else *err says why
+int write_plt(const std::string &path, const std::vector<std::string> &names, const std::vector<PltZone> &zones,
+              std::string *err);
+int read_plt(const std::string &path, std::vector<std::string> *names, std::vector<PltZone> *zones,
+             std::string *err);
+// setup `initial = 1`: overwrite the initial bv/sv from <dir>/output/field/flowfield.plt
+int restart_fields(ostc_case &c, std::string *err);
 }  // namespace ostc_host
 
 struct ostc_case {

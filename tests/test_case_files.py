@@ -211,13 +211,126 @@ def test_setup_grammar_rules(tmp_path):
 @pytest.mark.parametrize("edit,msg", [
     ({"1_grid_information.txt": lambda t: t.replace("int dimension = 3", "int dimension = 2")}, "dimension"),
     ({"0_global_control.txt": lambda t: t.replace("bool parallel = 0", "bool parallel = 1")}, "parallel"),
-    ({"0_global_control.txt": lambda t: t.replace("int initial = 0", "int initial = 1")}, "initial"),
+    ({"0_global_control.txt": lambda t: t.replace("int initial = 0", "int initial = 2")}, "initial"),
+    ({"0_global_control.txt": lambda t: t.replace("int initial = 0", "int initial = 1")}, "restart"),
     ({"2_scheme.txt": lambda t: t.replace("real    cfl", "real    cfl_typo")}, "cfl"),
     ({"5_boundary_condition.txt": lambda t: t.replace("struct  freestream", "struct  other")}, "freestream"),
 ])
 def test_unsupported_or_incomplete_setup_is_rejected(tmp_path, edit, msg):
     d, _ = write_setup(tmp_path, edit)
     with pytest.raises(L.OstcError, match=msg):
+        case_files.CaseDir(d)
+
+
+# ---- the reference's flow-field file (Tecplot binary V112, MPIIO layout),
+# written and read here independently of the library (src/MPIIO.hpp:60-548)
+def plt_write(path, names, zones):
+    """zones: list of (mx, my, mz, [var arrays, i fastest])"""
+    def s(txt):
+        return np.array([ord(ch) for ch in txt] + [0], "<i4").tobytes()
+    out = [b"#!TDV112", np.array([1, 0], "<i4").tobytes(), s("Solution file"), np.int32(len(names)).tobytes()]
+    out += [s(n) for n in names]
+    for b, (mx, my, mz, _) in enumerate(zones):
+        out += [np.float32(299.0).tobytes(), s(f"zone {b}"), np.array([-1, -2], "<i4").tobytes(),
+                np.float64(0).tobytes(), np.array([-1, 0, 0, 0, 0, mx, my, mz, 0], "<i4").tobytes()]
+    out.append(np.float32(357.0).tobytes())
+    for mx, my, mz, vs in zones:
+        out += [np.float32(299.0).tobytes(), np.full(len(names), 2, "<i4").tobytes(),
+                np.array([0, 0, -1], "<i4").tobytes()]
+        out += [np.array([v.min(), v.max()], "<f8").tobytes() for v in vs]
+        out += [np.ascontiguousarray(v, "<f8").tobytes() for v in vs]
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "wb") as f:
+        f.write(b"".join(out))
+
+
+def plt_read(path):
+    buf = open(path, "rb").read()
+    at = [0]
+
+    def take(fmt, n=1):
+        a = np.frombuffer(buf, fmt, n, at[0])
+        at[0] += a.nbytes
+        return a
+
+    def s():
+        chars = []
+        while True:
+            c = int(take("<i4")[0])
+            if c == 0:
+                return "".join(chars)
+            chars.append(chr(c))
+    assert buf[:8] == b"#!TDV112"
+    at[0] = 8
+    assert list(take("<i4", 2)) == [1, 0]
+    assert s() == "Solution file"
+    nv = int(take("<i4")[0])
+    names = [s() for _ in range(nv)]
+    dims = []
+    while True:
+        m = float(take("<f4")[0])
+        if m == 357.0:
+            break
+        assert m == 299.0
+        name = s()
+        head = take("<i4", 2)
+        take("<f8")
+        rest = take("<i4", 9)
+        assert list(head) == [-1, -2] and list(rest[:5]) == [-1, 0, 0, 0, 0] and rest[8] == 0
+        dims.append((name, tuple(int(v) for v in rest[5:8])))
+    zones = []
+    for name, (mx, my, mz) in dims:
+        assert float(take("<f4")[0]) == 299.0
+        assert list(take("<i4", nv)) == [2] * nv and list(take("<i4", 3)) == [0, 0, -1]
+        mm = take("<f8", 2 * nv).reshape(nv, 2)
+        vs = [take("<f8", mx * my * mz).reshape(mz, my, mx) for _ in range(nv)]
+        for v, (lo, hi) in zip(vs, mm):
+            assert lo == v.min() and hi == v.max()
+        zones.append((name, vs))
+    assert at[0] == len(buf)
+    return names, zones
+
+
+def test_restart_reads_the_flowfield_file(tmp_path):
+    """initial = 1 (src/Initialize.cuh:58-200): interior bv/sv from
+    output/field/flowfield.plt by variable label (any case, aliases rho/p/T/k;
+    unknown labels skipped), ghosts and missing variables from the setup's
+    initialisation; a mismatching zone is an error."""
+    case = cases.config2(mx=12, my=10, mz=6)
+    d = case_files.write_case(case, str(tmp_path / "c"))
+    base = case_files.CaseDir(d)
+    bv0, sv0 = base.field(0, "BV", 6), base.field(0, "SV", 2)
+    path = os.path.join(d, "input_files", "setup", "0_global_control.txt")
+    text = open(path).read()
+    open(path, "w").write(text.replace("int initial = 0", "int initial = 1"))
+    rng = np.random.default_rng(11)
+    mx, my, mz = 12, 10, 6
+    vals = {n: rng.uniform(0.5, 2.0, (mz, my, mx)) for n in ("RHO", "u", "V", "w", "p", "Temperature", "k", "OMEGA")}
+    names = ["x", "y", "z"] + list(vals) + ["mach", "mut", "bogus"]
+    arrays = [rng.normal(size=(mz, my, mx)) for _ in range(3)] + list(vals.values()) + \
+        [rng.normal(size=(mz, my, mx)) for _ in range(3)]
+    plt_write(os.path.join(d, "output", "field", "flowfield.plt"), names, [(mx, my, mz, arrays)])
+    cd = case_files.CaseDir(d)
+    bv = cd.field(0, "BV", 6)
+    sv = cd.field(0, "SV", 2)
+    g = cd.ngg
+    inner = (slice(g, -g),) * 3
+    for l, n in enumerate(["RHO", "u", "V", "w", "p", "Temperature"]):
+        assert np.array_equal(bv[l][inner], vals[n]), n
+    assert np.array_equal(sv[0][inner], vals["k"]) and np.array_equal(sv[1][inner], vals["OMEGA"])
+    # ghosts keep the setup's initialisation
+    ghost = np.ones(bv0[0].shape, bool)
+    ghost[inner] = False
+    assert np.array_equal(bv[:, ghost], bv0[:, ghost]) and np.array_equal(sv[:, ghost], sv0[:, ghost])
+    # a file without the turbulence variables: k, omega keep the inflow values
+    plt_write(os.path.join(d, "output", "field", "flowfield.plt"), names[:9], [(mx, my, mz, arrays[:9])])
+    cd2 = case_files.CaseDir(d)
+    assert np.array_equal(cd2.field(0, "SV", 2), sv0)
+    assert np.array_equal(cd2.field(0, "BV", 6)[0][inner], vals["RHO"])
+    # wrong zone size
+    plt_write(os.path.join(d, "output", "field", "flowfield.plt"), names,
+              [(mx, my, mz - 1, [a[:-1] for a in arrays])])
+    with pytest.raises(L.OstcError, match="does not match"):
         case_files.CaseDir(d)
 
 
@@ -269,3 +382,52 @@ def test_command_line_runner(tmp_path):
     hist = open(os.path.join(d, "history.dat")).read().split("\n")
     assert hist[0] == "step\terror_max" and hist[1].startswith("1\t")
     assert os.path.exists(os.path.join(d, "output", "message", "residual_scale.txt"))
+    # the flow-field file of the last output step (output_file divides total_step)
+    names, zones = plt_read(os.path.join(d, "output", "field", "flowfield.plt"))
+    assert names == ["x", "y", "z", "density", "u", "v", "w", "pressure", "temperature", "mach"]
+    assert len(zones) == 1 and zones[0][1][0].shape == (8, 16, 24)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,precision", [("config2", L.FP64), ("config3", L.FP32)])
+def test_flowfield_file_and_restart_round_trip(tmp_path, name, precision):
+    """The driver's flow-field file (MPIIO::print_field) read back by the
+    independent reader above equals the zone's interior fields and grid; a
+    restart (initial = 1) from it reproduces the zone's bv/sv exactly."""
+    case = getattr(cases, name)(mx=12, my=10, mz=6)
+    d = case_files.write_case(case, str(tmp_path / name))
+    cd = case_files.CaseDir(d)
+    inject(cd, case)
+    print("[gpu-driver] flowfield", name, flush=True)
+    drv = case_files.Driver(cd, precision)
+    drv.initialize()
+    for _ in range(3):
+        drv.step()
+    drv.write_flowfield()
+    names, zones = plt_read(os.path.join(d, "output", "field", "flowfield.plt"))
+    p = cd.param()
+    ns, rans = p.n_spec, p.turb_method == L.RANS
+    spec = cases.species_names(cases.load_mechanism(True)[1]) if ns else []
+    assert names == ["x", "y", "z", "density", "u", "v", "w", "pressure", "temperature", "mach"] + spec + \
+        (["tke", "omega", "mut"] if rans else [])
+    g = cd.ngg
+    inner = (slice(g, -g),) * 3
+    vs = dict(zip(names, zones[0][1]))
+    for l, n in enumerate(["density", "u", "v", "w", "pressure", "temperature"]):
+        assert np.array_equal(vs[n], drv.download(0, "BV", 6)[l][inner]), n
+    assert np.array_equal(vs["mach"], drv.download(0, "MACH", 1)[0][inner])
+    sv = drv.download(0, "SV", p.n_scalar) if p.n_scalar else None
+    for k, n in enumerate(spec + (["tke", "omega"] if rans else [])):
+        assert np.array_equal(vs[n], sv[k][inner]), n
+    for n, f in (("x", "X"), ("y", "Y"), ("z", "Z")):
+        assert np.array_equal(vs[n], cd.field(0, f, 1)[0][inner])
+    bv = drv.download(0, "BV", 6)
+    drv.close()
+    # restart
+    path = os.path.join(d, "input_files", "setup", "0_global_control.txt")
+    text = open(path).read()
+    open(path, "w").write(text.replace("int initial = 0", "int initial = 1"))
+    cd2 = case_files.CaseDir(d)
+    assert np.array_equal(cd2.field(0, "BV", 6)[:, g:-g, g:-g, g:-g], bv[:, g:-g, g:-g, g:-g])
+    if p.n_scalar:
+        assert np.array_equal(cd2.field(0, "SV", p.n_scalar)[:, g:-g, g:-g, g:-g], sv[:, g:-g, g:-g, g:-g])
