@@ -214,6 +214,12 @@ typedef struct ostc_case_info {
   double convergence_criteria, grid_scale;
   int mx[OSTC_MAX_BLOCK], my[OSTC_MAX_BLOCK], mz[OSTC_MAX_BLOCK];
   int n_inner[OSTC_MAX_BLOCK], n_neg_jac[OSTC_MAX_BLOCK];
+  /* restart (initial = 1): the step the flow-field file was written at
+   * (output/message/step.txt, src/Initialize.cuh:206-210) and the residual
+   * scales of that run (output/message/residual_scale.txt,
+   * src/Driver.cu:25-31); 0 and 1 otherwise */
+  int start_step, reserved;
+  double res_scale[4];
 } ostc_case_info;
 int ostc_case_read(const char *case_dir, ostc_case **out);
 int ostc_case_destroy(ostc_case *c);

@@ -123,7 +123,8 @@ class CaseInfo(C.Structure):
     _fields_ = [("n_block", C.c_int), ("ngg", C.c_int), ("total_step", C.c_int), ("output_screen", C.c_int),
                 ("output_file", C.c_int), ("n_wall_point", C.c_int), ("convergence_criteria", C.c_double),
                 ("grid_scale", C.c_double), ("mx", C.c_int * MAX_BLOCK), ("my", C.c_int * MAX_BLOCK),
-                ("mz", C.c_int * MAX_BLOCK), ("n_inner", C.c_int * MAX_BLOCK), ("n_neg_jac", C.c_int * MAX_BLOCK)]
+                ("mz", C.c_int * MAX_BLOCK), ("n_inner", C.c_int * MAX_BLOCK), ("n_neg_jac", C.c_int * MAX_BLOCK),
+                ("start_step", C.c_int), ("reserved", C.c_int), ("res_scale", C.c_double * 4)]
 
 
 class ZoneDesc(C.Structure):

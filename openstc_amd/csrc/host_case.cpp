@@ -887,6 +887,8 @@ int ostc_case_get_info(const ostc_case *c, ostc_case_info *info) {
     info->n_inner[b] = (int)c->blocks[b].inner.size();
     info->n_neg_jac[b] = c->blocks[b].n_neg_jac;
   }
+  info->start_step = c->start_step;
+  for (int l = 0; l < 4; ++l) info->res_scale[l] = c->res_scale[l];
   return OSTC_OK;
 }
 

@@ -79,6 +79,11 @@ int ostc_driver_create(const ostc_case *c, int precision, ostc_driver **out) {
   d->convergence = c->convergence;
   d->wall_xyz = c->wall_xyz;
   d->output_file = c->output_file;
+  // restart: continue the step count and keep the earlier run's residual
+  // scales (src/Driver.cu:25-31,197-198)
+  d->step = c->start_step;
+  d->total_step = c->total_step + c->start_step;
+  for (int l = 0; l < 4; ++l) d->res_scale[l] = c->res_scale[l];
   for (int k = 0; k < c->param.n_spec; ++k) d->spec_names.emplace_back(c->mech.spec_names[k]);
   auto cleanup = [&](int st) {
     delete d;
@@ -292,6 +297,15 @@ int ostc_driver_write_flowfield(ostc_driver *d, const char *path) {
   }
   std::string err;
   if (ostc_host::write_plt(file, names, zones, &err)) return drv_fail(OSTC_ERR_IO, err);
+  if (!path) {
+    // print_field also records the step (src/MPIIO.hpp:421-426)
+    std::error_code ec;
+    std::filesystem::create_directories(d->dir + "/output/message", ec);
+    FILE *f = std::fopen((d->dir + "/output/message/step.txt").c_str(), "w");
+    if (!f) return drv_fail(OSTC_ERR_IO, "cannot write output/message/step.txt");
+    std::fprintf(f, "%d", d->step);
+    std::fclose(f);
+  }
   return OSTC_OK;
 }
 

@@ -237,6 +237,18 @@ std::string upper(std::string s) {
 }  // namespace
 
 int restart_fields(ostc_case &c, std::string *err) {
+  // the step the file was written at and that run's residual scales (the
+  // reference reads both without checking; missing files leave 0 and 1)
+  if (FILE *f = std::fopen((c.dir + "/output/message/step.txt").c_str(), "r")) {
+    if (std::fscanf(f, "%d", &c.start_step) != 1) c.start_step = 0;
+    std::fclose(f);
+  }
+  if (FILE *f = std::fopen((c.dir + "/output/message/residual_scale.txt").c_str(), "r")) {
+    double v[4];
+    if (std::fscanf(f, "%lf %lf %lf %lf", &v[0], &v[1], &v[2], &v[3]) == 4)
+      for (int l = 0; l < 4; ++l) c.res_scale[l] = v[l];
+    std::fclose(f);
+  }
   const std::string path = c.dir + "/output/field/flowfield.plt";
   std::vector<std::string> names;
   std::vector<PltZone> zones;

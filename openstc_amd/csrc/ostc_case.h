@@ -90,6 +90,8 @@ struct ostc_case {
   ostc_mechanism_info mech{};
   int ngg = 2, dimension = 3;
   int total_step = 0, output_screen = 1, output_file = 1, initial = 0, steady = 1;
+  int start_step = 0;                            // restart: step of the flow-field file
+  double res_scale[4] = {1, 1, 1, 1};            // restart: residual scales of that run
   double convergence = 0, grid_scale = 1;
   std::vector<ostc_host::CaseBlock> blocks;
   std::vector<ostc_host::InflowState> inflows;  // BC slots, in slot order

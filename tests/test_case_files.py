@@ -310,7 +310,15 @@ def test_restart_reads_the_flowfield_file(tmp_path):
     arrays = [rng.normal(size=(mz, my, mx)) for _ in range(3)] + list(vals.values()) + \
         [rng.normal(size=(mz, my, mx)) for _ in range(3)]
     plt_write(os.path.join(d, "output", "field", "flowfield.plt"), names, [(mx, my, mz, arrays)])
+    # no step.txt / residual_scale.txt: step 0, scales 1
     cd = case_files.CaseDir(d)
+    assert cd.info.start_step == 0 and list(cd.info.res_scale) == [1.0] * 4
+    msg = os.path.join(d, "output", "message")
+    os.makedirs(msg, exist_ok=True)
+    open(os.path.join(msg, "step.txt"), "w").write("120")
+    open(os.path.join(msg, "residual_scale.txt"), "w").write("0.5\n2e-05\n1234.5\n0.25\n")
+    cd = case_files.CaseDir(d)
+    assert cd.info.start_step == 120 and list(cd.info.res_scale) == [0.5, 2e-05, 1234.5, 0.25]
     bv = cd.field(0, "BV", 6)
     sv = cd.field(0, "SV", 2)
     g = cd.ngg
@@ -428,6 +436,12 @@ def test_flowfield_file_and_restart_round_trip(tmp_path, name, precision):
     text = open(path).read()
     open(path, "w").write(text.replace("int initial = 0", "int initial = 1"))
     cd2 = case_files.CaseDir(d)
+    assert cd2.info.start_step == 3  # output/message/step.txt of the written file
     assert np.array_equal(cd2.field(0, "BV", 6)[:, g:-g, g:-g, g:-g], bv[:, g:-g, g:-g, g:-g])
+    drv2 = case_files.Driver(cd2, precision)
+    step = C.c_int()
+    L.check(drv2.lib.ostc_driver_get_step(drv2.h, C.byref(step)), "get_step")
+    assert step.value == 3
+    drv2.close()
     if p.n_scalar:
         assert np.array_equal(cd2.field(0, "SV", p.n_scalar)[:, g:-g, g:-g, g:-g], sv[:, g:-g, g:-g, g:-g])
