@@ -220,6 +220,10 @@ typedef struct ostc_case_info {
    * src/Driver.cu:25-31); 0 and 1 otherwise */
   int start_step, reserved;
   double res_scale[4];
+  /* density and speed of the setup struct named freestream (rho_inf, v_inf
+   * of src/BoundCond.cpp:96-99); the wall post-process divides by
+   * 0.5 rho_inf v_inf^2 */
+  double rho_inf, v_inf;
 } ostc_case_info;
 int ostc_case_read(const char *case_dir, ostc_case **out);
 int ostc_case_destroy(ostc_case *c);
@@ -261,6 +265,14 @@ int ostc_driver_simulate(ostc_driver *d, int io, int *steps_done, double *err_ma
  * file simulate(io != 0) writes every output_file steps and at convergence
  * and a setup with initial = 1 restarts from. */
 int ostc_driver_write_flowfield(ostc_driver *d, const char *path);
+/* Driver::post_process (src/Driver.cu:404-415): the procedures listed in the
+ * setup's post_process array; 0 = wall_friction_heatflux_2d
+ * (src/PostProcess.cu:6-47), which writes <case>/output/wall/
+ * friction_heatflux.dat ("x cf qw" per i of the j = 0, k = 0 wall line, one
+ * file rewritten per block so the last block's line remains, as the
+ * reference does).  ostc_driver_simulate(io = 1) calls it after every
+ * flow-field file. */
+int ostc_driver_post_process(ostc_driver *d);
 int ostc_driver_get_zone(ostc_driver *d, int block, ostc_zone **out);
 int ostc_driver_get_ctx(ostc_driver *d, ostc_ctx **out);
 int ostc_driver_get_step(const ostc_driver *d, int *step);
@@ -274,6 +286,14 @@ int ostc_compute_cv_from_bv(ostc_ctx *ctx, ostc_zone *zone, void *stream);
 int ostc_initialize_mut(ostc_ctx *ctx, ostc_zone *zone, void *stream);
 /* compute_wall_distance (src/Driver.cu:461-482); wall_xyz holds n points */
 int ostc_wall_distance(ostc_ctx *ctx, ostc_zone *zone, const double *wall_xyz, int n_wall_point, void *stream);
+
+/* ---- post-processing ---- */
+/* wall_friction_heatFlux_2d (src/PostProcess.cu:49-70): for i < mx on the
+ * j = 0 wall of the k = 0 line, cf[i] = mu_l u_par |grad eta| / dyn_pressure
+ * and qw[i] = lambda (T(i,1,0) - T(i,0,0)) |grad eta| (u_par: the velocity at
+ * j = 1 along xi).  cf and qw are DEVICE arrays of mx doubles; dyn_pressure
+ * is 0.5 rho_inf v_inf^2 of the freestream.  Stream-ordered. */
+int ostc_wall_friction_heatflux(ostc_zone *zone, double dyn_pressure, double *cf, double *qw, void *stream);
 
 /* ---- per-iteration stages (src/Driver.cu:232-272) ---- */
 /* compute_source<M,T> (src/SourceTerm.cuh:11-40); writes dq (no memset needed) */

@@ -124,7 +124,8 @@ class CaseInfo(C.Structure):
                 ("output_file", C.c_int), ("n_wall_point", C.c_int), ("convergence_criteria", C.c_double),
                 ("grid_scale", C.c_double), ("mx", C.c_int * MAX_BLOCK), ("my", C.c_int * MAX_BLOCK),
                 ("mz", C.c_int * MAX_BLOCK), ("n_inner", C.c_int * MAX_BLOCK), ("n_neg_jac", C.c_int * MAX_BLOCK),
-                ("start_step", C.c_int), ("reserved", C.c_int), ("res_scale", C.c_double * 4)]
+                ("start_step", C.c_int), ("reserved", C.c_int), ("res_scale", C.c_double * 4),
+                ("rho_inf", C.c_double), ("v_inf", C.c_double)]
 
 
 class ZoneDesc(C.Structure):
@@ -156,6 +157,7 @@ SIGNATURES = {
     "ostc_compute_cv_from_bv": (C.c_int, [_P, _P, _P]),
     "ostc_initialize_mut": (C.c_int, [_P, _P, _P]),
     "ostc_wall_distance": (C.c_int, [_P, _P, _D, C.c_int, _P]),
+    "ostc_wall_friction_heatflux": (C.c_int, [_P, C.c_double, _P, _P, _P]),
     "ostc_compute_source": (C.c_int, [_P, _P, _P]),
     "ostc_inviscid_flux": (C.c_int, [_P, _P, _P]),
     "ostc_viscous_flux": (C.c_int, [_P, _P, _P]),
@@ -212,6 +214,7 @@ SIGNATURES = {
     "ostc_driver_step": (C.c_int, [_P, _D, _I, _D]),
     "ostc_driver_simulate": (C.c_int, [_P, C.c_int, _I, _D]),
     "ostc_driver_write_flowfield": (C.c_int, [_P, C.c_char_p]),
+    "ostc_driver_post_process": (C.c_int, [_P]),
     "ostc_driver_get_zone": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "ostc_driver_get_ctx": (C.c_int, [_P, C.POINTER(_P)]),
     "ostc_driver_get_step": (C.c_int, [_P, _I]),

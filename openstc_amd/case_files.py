@@ -255,6 +255,11 @@ class Driver:
         self._check(self.lib.ostc_driver_write_flowfield(self.h, path.encode() if path else None),
                     "ostc_driver_write_flowfield")
 
+    def post_process(self):
+        """The setup's post_process procedures (0: wall friction and heat
+        flux -> <case>/output/wall/friction_heatflux.dat)."""
+        self._check(self.lib.ostc_driver_post_process(self.h), "ostc_driver_post_process")
+
     def zone(self, b: int):
         z = C.c_void_p()
         self._check(self.lib.ostc_driver_get_zone(self.h, b, C.byref(z)), "ostc_driver_get_zone")

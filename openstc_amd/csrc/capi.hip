@@ -441,6 +441,13 @@ int ostc_wall_distance(ostc_ctx *c, ostc_zone *z, const double *wall_xyz, int n,
   return r;
 }
 
+int ostc_wall_friction_heatflux(ostc_zone *z, double dyn_pressure, double *cf, double *qw, void *s) {
+  if (!z || !cf || !qw) return fail(OSTC_ERR_ARG, "null argument");
+  if (!(dyn_pressure > 0)) return fail(OSTC_ERR_ARG, "dyn_pressure must be positive");
+  return ZONE_CALL(z, wall_friction_heatflux(z->vf, dyn_pressure, cf, qw, S(s)),
+                   wall_friction_heatflux(z->vd, dyn_pressure, cf, qw, S(s)));
+}
+
 int ostc_compute_source(ostc_ctx *c, ostc_zone *z, void *s) {
   if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
   bool wrote = false;

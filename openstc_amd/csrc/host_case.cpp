@@ -831,6 +831,9 @@ int read_case(const char *dir_c, ostc_case *c) {
       return e;
     }
   }
+  c->rho_inf = cx.rho_inf;
+  c->v_inf = cx.v_inf;
+  if (auto pp = s.int_arrays.find("post_process"); pp != s.int_arrays.end()) c->post_process = pp->second;
   p.limit_ll[0] = 1e-6 * cx.rho_inf;
   p.limit_ul[0] = 1e+3 * cx.rho_inf;
   for (int l = 1; l < 4; ++l) {
@@ -889,6 +892,8 @@ int ostc_case_get_info(const ostc_case *c, ostc_case_info *info) {
   }
   info->start_step = c->start_step;
   for (int l = 0; l < 4; ++l) info->res_scale[l] = c->res_scale[l];
+  info->rho_inf = c->rho_inf;
+  info->v_inf = c->v_inf;
   return OSTC_OK;
 }
 

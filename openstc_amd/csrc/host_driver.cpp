@@ -11,6 +11,7 @@
 //    flow-field file simulate writes every output_file steps and at
 //    convergence (src/Driver.cu:283-286).
 // Model dispatch (src/main.cu:22-64) is the runtime model of ostc_param.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -55,6 +56,8 @@ struct ostc_driver {
   std::vector<std::string> spec_names;  // mechanism order
   std::vector<int> dims;                // mx, my, mz per block
   int ngg = 2, output_file = 1;
+  double dyn_pressure = 0;               // 0.5 rho_inf v_inf^2
+  std::vector<int> post_process;          // setup array post_process
   double res[4] = {0, 0, 0, 0}, res_scale[4] = {1, 1, 1, 1};
   bool initialized = false;
   std::chrono::steady_clock::time_point t0, t_last;
@@ -79,6 +82,8 @@ int ostc_driver_create(const ostc_case *c, int precision, ostc_driver **out) {
   d->convergence = c->convergence;
   d->wall_xyz = c->wall_xyz;
   d->output_file = c->output_file;
+  d->dyn_pressure = 0.5 * c->rho_inf * c->v_inf * c->v_inf;
+  d->post_process = c->post_process;
   // restart: continue the step count and keep the earlier run's residual
   // scales (src/Driver.cu:25-31,197-198)
   d->step = c->start_step;
@@ -209,7 +214,8 @@ int ostc_driver_simulate(ostc_driver *d, int io, int *steps_done, double *err_ma
     if (computed) err = e;
     const bool converged = computed && err < d->convergence;
     if (io && (d->step % d->output_file == 0 || converged)) {
-      const int wst = ostc_driver_write_flowfield(d, nullptr);
+      int wst = ostc_driver_write_flowfield(d, nullptr);
+      if (wst == OSTC_OK) wst = ostc_driver_post_process(d);
       if (wst != OSTC_OK) {
         if (steps_done) *steps_done = done;
         return wst;
@@ -305,6 +311,51 @@ int ostc_driver_write_flowfield(ostc_driver *d, const char *path) {
     if (!f) return drv_fail(OSTC_ERR_IO, "cannot write output/message/step.txt");
     std::fprintf(f, "%d", d->step);
     std::fclose(f);
+  }
+  return OSTC_OK;
+}
+
+int ostc_driver_post_process(ostc_driver *d) {
+  if (!d) return drv_fail(OSTC_ERR_ARG, "null argument");
+  for (int proc : d->post_process) {
+    if (proc != 0) continue;  // the reference knows procedure 0 only
+    // wall_friction_heatflux_2d (src/PostProcess.cu:6-47)
+    const std::string dir = d->dir + "/output/wall";
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    size_t mx_max = 0;
+    for (size_t b = 0; b < d->zones.size(); ++b) mx_max = std::max(mx_max, (size_t)d->dims[3 * b]);
+    void *dev = nullptr;
+    DRV_TRY(ostc_device_alloc(&dev, (long)(2 * mx_max * sizeof(double))), "allocate cf/qw");
+    double *cf = static_cast<double *>(dev), *qw = cf + mx_max;
+    std::vector<double> h_cf(mx_max), h_qw(mx_max), xg;
+    int st = OSTC_OK;
+    for (size_t b = 0; b < d->zones.size() && st == OSTC_OK; ++b) {
+      ostc_zone *z = d->zones[b];
+      const int g = d->ngg, mx = d->dims[3 * b], my = d->dims[3 * b + 1], mz = d->dims[3 * b + 2];
+      const long nx = mx + 2 * g, ny = my + 2 * g;
+      xg.resize((size_t)nx * ny * (mz + 2 * g));
+      if ((st = ostc_wall_friction_heatflux(z, d->dyn_pressure, cf, qw, nullptr)) != OSTC_OK ||
+          (st = ostc_memcpy_d2h(h_cf.data(), cf, (long)(mx * sizeof(double)))) != OSTC_OK ||
+          (st = ostc_memcpy_d2h(h_qw.data(), qw, (long)(mx * sizeof(double)))) != OSTC_OK ||
+          (st = ostc_zone_download(z, OSTC_F_X, xg.data(), 1)) != OSTC_OK) {
+        drv_fail(st, "wall friction/heat flux");
+        break;
+      }
+      // one file, rewritten for every block (the reference's behaviour)
+      FILE *f = std::fopen((dir + "/friction_heatflux.dat").c_str(), "w");
+      if (!f) {
+        st = drv_fail(OSTC_ERR_IO, "cannot write " + dir + "/friction_heatflux.dat");
+        break;
+      }
+      std::fprintf(f, "variables = \"x\", \"cf\", \"qw\"\n");
+      // std::ostream's default floating-point output: %g, 6 significant digits
+      for (int i = 0; i < mx; ++i)
+        std::fprintf(f, "%g\t%g\t%g\n", xg[((long)g * ny + g) * nx + i + g], h_cf[i], h_qw[i]);
+      std::fclose(f);
+    }
+    ostc_device_free(dev);
+    if (st != OSTC_OK) return st;
   }
   return OSTC_OK;
 }

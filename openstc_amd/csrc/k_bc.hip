@@ -366,6 +366,28 @@ dim3 face_grid(const FaceLaunch &f, int ngg, dim3 &block) {
   return dim3((na + 63) / 64, (nb + 3) / 4, nl);
 }
 
+// Wall friction coefficient and heat flux along the j = 0 wall of the k = 0
+// line (wall_friction_heatFlux_2d, src/PostProcess.cu:49-70): per i,
+//   u_par = (xi_x u + xi_y v) / |(xi_x, xi_y)|  at j = 1,
+//   cf    = mu_l(i,0,0) u_par |(eta_x, eta_y)| / q_dyn,
+//   qw    = lambda(i,0,0) (T(i,1,0) - T(i,0,0)) |(eta_x, eta_y)|,
+// evaluated in FP64 from the stored fields (metrics at the wall point).
+template <typename R>
+__global__ __launch_bounds__(128) void k_wall_cf_qw(ZoneView<R> z, double dyn_pressure, double *__restrict__ cf,
+                                                    double *__restrict__ qw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= z.mx) return;
+  const int64_t w = z.idx(i, 0, 0), a = z.idx(i, 1, 0);
+  const double xi_x = z.m(w, 1, 1), xi_y = z.m(w, 1, 2);
+  const double eta_x = z.m(w, 2, 1), eta_y = z.m(w, 2, 2);
+  const double u = z.bv[a + z.plane], v = z.bv[a + 2 * z.plane];
+  const double u_par = (xi_x * u + xi_y * v) / sqrt(xi_x * xi_x + xi_y * xi_y);
+  const double grad_eta = sqrt(eta_x * eta_x + eta_y * eta_y);
+  cf[i] = (double)z.mul[w] * (u_par * grad_eta) / dyn_pressure;
+  const double t1 = z.bv[a + 5 * z.plane], t0 = z.bv[w + 5 * z.plane];
+  qw[i] = (double)z.lambda[w] * (t1 - t0) * grad_eta;
+}
+
 }  // namespace
 
 template <typename R>
@@ -442,6 +464,12 @@ int inner_halo(const ZoneView<R> &z, const ZoneView<R> &t, const ostc_param *dp,
   return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;
 }
 
+template <typename R>
+int wall_friction_heatflux(const ZoneView<R> &z, double dyn_pressure, double *cf, double *qw, hipStream_t st) {
+  hipLaunchKernelGGL((k_wall_cf_qw<R>), dim3((z.mx + 127) / 128), dim3(128), 0, st, z, dyn_pressure, cf, qw);
+  return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;
+}
+
 template int apply_bc<float>(const ZoneView<float> &, const ostc_param *, const CellOpts &, const ostc_bc &,
                              const ostc_inflow *, hipStream_t);
 template int apply_bc<double>(const ZoneView<double> &, const ostc_param *, const CellOpts &, const ostc_bc &,
@@ -450,5 +478,7 @@ template int inner_halo<float>(const ZoneView<float> &, const ZoneView<float> &,
                                const CellOpts &, const ostc_inner_face &, hipStream_t);
 template int inner_halo<double>(const ZoneView<double> &, const ZoneView<double> &, const ostc_param *,
                                 const CellOpts &, const ostc_inner_face &, hipStream_t);
+template int wall_friction_heatflux<float>(const ZoneView<float> &, double, double *, double *, hipStream_t);
+template int wall_friction_heatflux<double>(const ZoneView<double> &, double, double *, double *, hipStream_t);
 
 }  // namespace ostc

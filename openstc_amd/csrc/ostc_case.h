@@ -93,6 +93,8 @@ struct ostc_case {
   int start_step = 0;                            // restart: step of the flow-field file
   double res_scale[4] = {1, 1, 1, 1};            // restart: residual scales of that run
   double convergence = 0, grid_scale = 1;
+  double rho_inf = 0, v_inf = 0;                 // the freestream struct's density and speed
+  std::vector<int> post_process;                // setup array post_process (procedure labels)
   std::vector<ostc_host::CaseBlock> blocks;
   std::vector<ostc_host::InflowState> inflows;  // BC slots, in slot order
   std::vector<double> wall_xyz;                 // wall points for the wall distance

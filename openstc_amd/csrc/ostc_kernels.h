@@ -76,6 +76,9 @@ int apply_bc(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, cons
 template <typename R>
 int inner_halo(const ZoneView<R> &z, const ZoneView<R> &t, const ostc_param *dp, const CellOpts &o,
                const ostc_inner_face &f, hipStream_t st);
+// cf[i], qw[i] (i < mx, device FP64) along the j = 0, k = 0 wall line
+template <typename R>
+int wall_friction_heatflux(const ZoneView<R> &z, double dyn_pressure, double *cf, double *qw, hipStream_t st);
 
 // k_chem.hip
 int chem_plain_step(const ostc_param *dp, int ns, int nr, long n, const double *t, const double *p, const double *y,

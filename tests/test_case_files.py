@@ -445,3 +445,79 @@ def test_flowfield_file_and_restart_round_trip(tmp_path, name, precision):
     drv2.close()
     if p.n_scalar:
         assert np.array_equal(cd2.field(0, "SV", p.n_scalar)[:, g:-g, g:-g, g:-g], sv[:, g:-g, g:-g, g:-g])
+
+
+def test_post_process_array_is_read(tmp_path):
+    """A setup with post_process { 0 } reads, and the case exposes the
+    freestream density and speed of the procedure's dynamic pressure (the
+    procedure itself runs on the GPU)."""
+    case = cases.config5(mx=16, my=12, mz=10, nblocks=2, steps=2)
+    d = case_files.write_case(case, str(tmp_path / "pp"))
+    path = os.path.join(d, "input_files", "setup", "6_post_process.txt")
+    open(path, "w").write(open(os.path.join(SAMPLE, "6_post_process.txt")).read().replace("// 0", "0"))
+    cd = case_files.CaseDir(d)
+    assert cd.info.n_block == 2
+    # the freestream state behind the dynamic pressure is the one the flow
+    # limits are formed from (src/DParameter.cu:82-87)
+    p = cd.param()
+    assert cd.info.rho_inf > 0 and cd.info.v_inf > 0
+    assert p.limit_ll[0] == 1e-6 * cd.info.rho_inf and p.limit_ul[1] == 1e+3 * cd.info.v_inf
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32])
+def test_wall_friction_heatflux_post_process(tmp_path, precision):
+    """Procedure 0 (src/PostProcess.cu:6-70) on an isothermal-wall case
+    (config5, 2 blocks): the device entry point equals the numpy oracle on
+    the downloaded fields of every block, and simulate(io) leaves
+    output/wall/friction_heatflux.dat holding the last block's line."""
+    import orc_post
+    case = cases.config5(mx=16, my=12, mz=10, nblocks=2, steps=2)
+    d = case_files.write_case(case, str(tmp_path / "pp"))
+    setup = os.path.join(d, "input_files", "setup")
+    open(os.path.join(setup, "6_post_process.txt"), "w").write(
+        open(os.path.join(SAMPLE, "6_post_process.txt")).read().replace("// 0", "0"))
+    ctl = os.path.join(setup, "0_global_control.txt")
+    text = open(ctl).read()
+    out_line = [ln for ln in text.split("\n") if "output_file" in ln][0]
+    open(ctl, "w").write(text.replace(out_line, "int output_file = 1"))
+    cd = case_files.CaseDir(d)
+    inject(cd, case)
+    print(f"[gpu-driver] post_process config5 2 blocks precision {precision}", flush=True)
+    drv = case_files.Driver(cd, precision)
+    drv.initialize()
+    drv.simulate(io=True)
+    g = cd.ngg
+    lib = drv.lib
+    dev = C.c_void_p()
+    mx = cd.info.mx[0]
+    L.check(lib.ostc_device_alloc(C.byref(dev), 2 * mx * 8), "alloc")
+    try:
+        # dynamic pressure of the freestream struct (src/PostProcess.cu:25-27)
+        q = 0.5 * cd.info.rho_inf * cd.info.v_inf ** 2
+        assert q > 0
+        for b in range(2):
+            metric = drv.download(b, "METRIC", 9)
+            bv = drv.download(b, "BV", 6)
+            mul = drv.download(b, "MUL", 1)[0]
+            lam = drv.download(b, "LAMBDA", 1)[0]
+            cf_ref, qw_ref = orc_post.wall_friction_heatflux(metric, bv, mul, lam, g, q)
+            L.check(lib.ostc_wall_friction_heatflux(drv.zone(b), q, dev, C.c_void_p(dev.value + mx * 8), None),
+                    "ostc_wall_friction_heatflux")
+            got = np.empty(2 * mx)
+            L.check(lib.ostc_memcpy_d2h(got.ctypes.data_as(C.c_void_p), dev, 2 * mx * 8), "d2h")
+            cf, qw = got[:mx], got[mx:]
+            assert np.abs(qw_ref).max() > 0 and np.abs(cf_ref).max() > 0
+            assert np.allclose(cf, cf_ref, rtol=1e-12, atol=1e-14 * np.abs(cf_ref).max()), b
+            assert np.allclose(qw, qw_ref, rtol=1e-12, atol=1e-14 * np.abs(qw_ref).max()), b
+        x_line = cd.field(1, "X", 1)[0][g, g, g:g + mx]
+        want = orc_post.friction_heatflux_file(x_line, cf_ref, qw_ref).split("\n")
+        have = open(os.path.join(d, "output", "wall", "friction_heatflux.dat")).read().split("\n")
+        assert have[0] == want[0] and len(have) == len(want)
+        a = np.array([[float(v) for v in ln.split("\t")] for ln in have[1:-1]])
+        e = np.array([[float(v) for v in ln.split("\t")] for ln in want[1:-1]])
+        # 6 significant digits: the last one may round the other way
+        assert np.all(np.abs(a - e) <= 1e-5 * np.abs(e) + 1e-300)
+    finally:
+        lib.ostc_device_free(dev)
+        drv.close()
