@@ -99,6 +99,9 @@ def alg_scalars(nv: int, rans: bool, dplur: bool):
         "local_dt": 9 + 3 + 3 + (1 if rans else 0) + 4,
         "dplur_dq0": nv + 1 + 3 + (2 if rans else 0) + nv,
         "dplur_sweep": nv + 5 + 1 + nsc + 9 + 3 + 1 + nv + (2 if rans else 0) + nv,
+        # first sweep of a non-reacting zone: reads dq in place of dq0 and
+        # writes dq0 in place of reading it -- the same count
+        "dplur_sweep_first": nv + 5 + 1 + nsc + 9 + 3 + 1 + nv + (2 if rans else 0) + nv,
         "update": nv + nv + 2 + nv + 6 + nsc + 1,
         "limit_flow": 5 + nsc,
         "properties": 6,

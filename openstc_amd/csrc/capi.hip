@@ -232,11 +232,9 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   o.turb_implicit = param->turb_implicit;
   o.implicit_method = param->implicit_method;
   o.dplur_steps = param->dplur_inner_step;
-  // launch tuning of the DPLUR sweep: tile height (4 or 8 rows) and the number
-  // of workgroups the k chunks are sized for (environment overrides)
-  o.dplur_tj = 4;
+  // launch tuning of the DPLUR sweep: the number of workgroups its k chunks
+  // are sized for (environment override)
   o.dplur_wg = 1024;
-  if (const char *e = std::getenv("OSTC_DPLUR_TJ")) o.dplur_tj = std::atoi(e) == 4 ? 4 : 8;
   if (const char *e = std::getenv("OSTC_DPLUR_WG")) o.dplur_wg = std::max(64, std::atoi(e));
   *out = c;
   return OSTC_OK;
@@ -645,7 +643,8 @@ int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_fac
 
 static const char *kStageNames[P_COUNT] = {"source", "flux_i", "flux_j", "flux_k", "local_dt", "dplur_dq0",
                                            "dplur_sweep", "explicit_implicit", "update", "limit_flow", "bc",
-                                           "inner_halo", "properties", "store_last", "residual", "flux_ij"};
+                                           "inner_halo", "properties", "store_last", "residual", "flux_ij",
+                                           "dplur_sweep_first"};
 
 int ostc_profile(ostc_ctx *c, int enable) {
   (void)c;

@@ -387,13 +387,24 @@ __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__
 
 // Diagonal solve and write of one cell after a DPLUR sweep has gathered its
 // off-diagonal sum acc (src/DPLURHPP.cuh:193-249); EPI_for_dqk
-// (src/FiniteRateChem.cu:255-284) with the LU factors from k_dq0.
-template <typename R, int NS, int NT, bool CHEM>
+// (src/FiniteRateChem.cu:255-284) with the LU factors from k_dq0.  FIRST
+// (non-reacting zones): the cell's dq0 comes in dq0v[0..NV) and is stored for
+// the later sweeps; dq0v[NV], dq0v[NV+1] hold its SST Jacobian diagonal.
+template <typename R, int NS, int NT, bool CHEM, bool FIRST>
 __device__ __forceinline__ void dplur_finish(const ZoneView<R> &z, const ostc_param *__restrict__ p, int64_t id,
-                                             const R *acc, R dt, R diag, R *__restrict__ dout) {
+                                             const R *acc, R dt, R diag, R *__restrict__ dout, const R *dq0v) {
+  static_assert(!(FIRST && CHEM && NS > 0), "the chemistry block needs k_dq0's LU factors");
   const int64_t P = z.plane;
   const int nflow = (CHEM && NS > 0) ? 5 : 5 + NS;
-  for (int l = 0; l < nflow; ++l) dout[l * P + id] = z.dq0[l * P + id] + dt * acc[l] / diag;
+  auto q0 = [&](int l) -> R {
+    if constexpr (FIRST) {
+      z.dq0[l * P + id] = dq0v[l];
+      return dq0v[l];
+    } else {
+      return z.dq0[l * P + id];
+    }
+  };
+  for (int l = 0; l < nflow; ++l) dout[l * P + id] = q0(l) + dt * acc[l] / diag;
   if constexpr (CHEM && NS > 0) {
     const int meth = p->chem_src_method;
     const double dtd = (double)dt, dg = (double)diag;
@@ -422,11 +433,13 @@ __device__ __forceinline__ void dplur_finish(const ZoneView<R> &z, const ostc_pa
   }
   if constexpr (NT > 0) {
     if (p->turb_implicit == 1) {
-      dout[(NS + 5) * P + id] = z.dq0[(NS + 5) * P + id] + dt * acc[5 + NS] / (diag - dt * z.turb_jac[id]);
-      dout[(NS + 6) * P + id] = z.dq0[(NS + 6) * P + id] + dt * acc[6 + NS] / (diag - dt * z.turb_jac[P + id]);
+      const R tj0 = FIRST ? dq0v[5 + NS + NT] : z.turb_jac[id];
+      const R tj1 = FIRST ? dq0v[6 + NS + NT] : z.turb_jac[P + id];
+      dout[(NS + 5) * P + id] = q0(NS + 5) + dt * acc[5 + NS] / (diag - dt * tj0);
+      dout[(NS + 6) * P + id] = q0(NS + 6) + dt * acc[6 + NS] / (diag - dt * tj1);
     } else {
-      dout[(NS + 5) * P + id] = z.dq0[(NS + 5) * P + id] + dt * acc[5 + NS] / diag;
-      dout[(NS + 6) * P + id] = z.dq0[(NS + 6) * P + id] + dt * acc[6 + NS] / diag;
+      dout[(NS + 5) * P + id] = q0(NS + 5) + dt * acc[5 + NS] / diag;
+      dout[(NS + 6) * P + id] = q0(NS + 6) + dt * acc[6 + NS] / diag;
     }
   }
 }
@@ -439,15 +452,45 @@ struct JdCell {
   R d[NV];
   R sv[NS + NT];
   R u, v, w, e, h, b2, bb;
+  R dt, diag, tj0, tj1;  // first sweep only: the cell's dq0 scaling
 };
 
+// compute_DQ_0's diagonal scaling without a chemistry block
+// (src/DPLURHPP.cuh:13-76, SST rows src/SST.cu:196-204): d = dq / diag, the
+// same operations k_dq0 performs.  The first sweep of a non-reacting zone
+// applies it to every dq it loads instead of reading a stored dq0, and keeps
+// dt, diag and the SST Jacobian diagonal for the cell's own diagonal solve.
 template <typename R, int NS, int NT>
+__device__ __forceinline__ void dq0_scale(const ZoneView<R> &z, const ostc_param *__restrict__ p, int64_t q,
+                                          JdCell<R, NS, NT> &c) {
+  const int64_t P = z.plane;
+  c.dt = z.dt[q];
+  c.diag = (R)1 + c.dt * (z.spectr[q] + z.spectr[P + q] + z.spectr[2 * P + q]);
+  c.tj0 = c.tj1 = 0;
+#pragma unroll
+  for (int l = 0; l < 5 + NS; ++l) c.d[l] = c.d[l] / c.diag;
+  if constexpr (NT > 0) {
+    if (p->turb_implicit == 1) {
+      c.tj0 = z.turb_jac[q];
+      c.tj1 = z.turb_jac[P + q];
+      c.d[NS + 5] = c.d[NS + 5] / (c.diag - c.dt * c.tj0);
+      c.d[NS + 6] = c.d[NS + 6] / (c.diag - c.dt * c.tj1);
+    } else {
+      c.d[NS + 5] = c.d[NS + 5] / c.diag;
+      c.d[NS + 6] = c.d[NS + 6] / c.diag;
+    }
+  }
+}
+
+// FIRST: din is the right-hand side dq, scaled to dq0 on load
+template <typename R, int NS, int NT, bool FIRST>
 __device__ __forceinline__ void jd_load(const ZoneView<R> &z, const ostc_param *__restrict__ p,
                                         const R *__restrict__ din, int64_t q, JdCell<R, NS, NT> &c) {
   constexpr int NV = 5 + NS + NT;
   const int64_t P = z.plane;
 #pragma unroll
   for (int l = 0; l < NV; ++l) c.d[l] = din[l * P + q];
+  if constexpr (FIRST) dq0_scale<R, NS, NT>(z, p, q, c);
 #pragma unroll
   for (int l = 0; l < NS + NT; ++l) c.sv[l] = z.sv[l * P + q];
   c.u = z.bv[P + q];
@@ -515,8 +558,11 @@ __device__ __forceinline__ void jd_dir(const ZoneView<R> &z, int64_t q, int dir,
 // right of every row by 2*TJ lanes of the first wave in one pass, handed over
 // through LDS (single-lane halo loads per row would double the tile's vector
 // memory instructions, which bound this kernel).  Off-diagonal sums are
-// accumulated in the reference's order (i-, j-, k-, i+, j+, k+).
-template <typename R, int NS, int NT, bool CHEM, int TJ>
+// accumulated in the reference's order (i-, j-, k-, i+, j+, k+).  FIRST
+// (first sweep of a non-reacting zone): din is the right-hand side dq, scaled
+// to dq0 as it is loaded, and each cell's dq0 is stored for the later sweeps
+// (no separate dq0 pass).
+template <typename R, int NS, int NT, bool CHEM, int TJ, bool FIRST>
 __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_param *__restrict__ p,
                                                const R *__restrict__ din, R *__restrict__ dout, int kc) {
   constexpr int NV = 5 + NS + NT;
@@ -533,23 +579,24 @@ __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_par
   const R *sr = z.spectr;
   JdCell<R, NS, NT> c;
   R pk[NV], acc[NV], accp[NV];
+  R dq0p[FIRST ? NV + 2 : 1];  // FIRST: dq0 (+ SST diagonal) of the cell below, finished with the next plane
   R dtp = 0, diagp = 1;
   // P_k of the plane below the chunk
   if (k0 > 0) {
     const int64_t q = z.idx(ii, jj, k0 - 1);
-    jd_load<R, NS, NT>(z, p, din, q, c);
+    jd_load<R, NS, NT, FIRST>(z, p, din, q, c);
     jd_dir<R, NS, NT>(z, q, 2, c, sr[2 * P + q], pk, nullptr);
   }
   for (int k = k0; k < k1; ++k) {
     const int64_t id = z.idx(ii, jj, k);
-    jd_load<R, NS, NT>(z, p, din, id, c);
+    jd_load<R, NS, NT, FIRST>(z, p, din, id, c);
     R mk[NV], pnext[NV];
     jd_dir<R, NS, NT>(z, id, 2, c, sr[2 * P + id], pnext, mk);
     // finish the cell below: its plus-side k neighbour is this cell
     if (k > k0) {
 #pragma unroll
       for (int l = 0; l < NV; ++l) accp[l] -= mk[l];
-      if (ok) dplur_finish<R, NS, NT, CHEM>(z, p, id - sk, accp, dtp, diagp, dout);
+      if (ok) dplur_finish<R, NS, NT, CHEM, FIRST>(z, p, id - sk, accp, dtp, diagp, dout, dq0p);
     }
     // minus-side terms in the reference order: i-, j-, k-
     R pi[NV], mi[NV];
@@ -579,7 +626,7 @@ __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_par
         const int64_t q = z.idx(ih, jr, k);
         JdCell<R, NS, NT> e;
         R hp[NV], hm[NV];
-        jd_load<R, NS, NT>(z, p, din, q, e);
+        jd_load<R, NS, NT, FIRST>(z, p, din, q, e);
         jd_dir<R, NS, NT>(z, q, 0, e, sr[q], hp, hm);
 #pragma unroll
         for (int l = 0; l < NV; ++l) si[side][r][l] = side ? hm[l] : hp[l];
@@ -590,13 +637,13 @@ __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_par
     if (ty == 0 && j > 0) {
       const int64_t q = z.idx(ii, j - 1, k);
       JdCell<R, NS, NT> e;
-      jd_load<R, NS, NT>(z, p, din, q, e);
+      jd_load<R, NS, NT, FIRST>(z, p, din, q, e);
       jd_dir<R, NS, NT>(z, q, 1, e, sr[P + q], jp, nullptr);
     }
     if (ty == TJ - 1 && j + 1 < z.my) {
       const int64_t q = z.idx(ii, j + 1, k);
       JdCell<R, NS, NT> e;
-      jd_load<R, NS, NT>(z, p, din, q, e);
+      jd_load<R, NS, NT, FIRST>(z, p, din, q, e);
       jd_dir<R, NS, NT>(z, q, 1, e, sr[P + q], nullptr, jm);
     }
     __syncthreads();
@@ -630,21 +677,30 @@ __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_par
       accp[l] = acc[l];
       pk[l] = pnext[l];
     }
-    dtp = z.dt[id];
-    diagp = (R)1 + dtp * (sr[id] + sr[P + id] + sr[2 * P + id]);
+    if constexpr (FIRST) {
+      dtp = c.dt;
+      diagp = c.diag;
+#pragma unroll
+      for (int l = 0; l < NV; ++l) dq0p[l] = c.d[l];
+      dq0p[NV] = c.tj0;
+      dq0p[NV + 1] = c.tj1;
+    } else {
+      dtp = z.dt[id];
+      diagp = (R)1 + dtp * (sr[id] + sr[P + id] + sr[2 * P + id]);
+    }
   }
   // last plane of the chunk: plus-side k neighbour from the plane above
   if (k1 > k0) {
     const int kl = k1 - 1;
     if (k1 < z.mz) {
       const int64_t q = z.idx(ii, jj, k1);
-      jd_load<R, NS, NT>(z, p, din, q, c);
+      jd_load<R, NS, NT, FIRST>(z, p, din, q, c);
       R mk[NV];
       jd_dir<R, NS, NT>(z, q, 2, c, sr[2 * P + q], nullptr, mk);
 #pragma unroll
       for (int l = 0; l < NV; ++l) accp[l] -= mk[l];
     }
-    if (ok) dplur_finish<R, NS, NT, CHEM>(z, p, z.idx(ii, jj, kl), accp, dtp, diagp, dout);
+    if (ok) dplur_finish<R, NS, NT, CHEM, FIRST>(z, p, z.idx(ii, jj, kl), accp, dtp, diagp, dout, dq0p);
   }
 }
 
@@ -1166,43 +1222,44 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
     }
     return last_err();
   }
-  // DPLUR: dq -> dq0; sweeps ping-pong dq0 -> dqk -> dq -> dqk ...
-  prof_mark(P_DQ0, st, true);
-  if (chem) {
-    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp));
-  } else {
-    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp));
+  // DPLUR: dq -> dq0; sweeps ping-pong dq0 -> dqk -> dq -> dqk ...  Without
+  // a chemistry block dq0 is a diagonal scaling, which the first sweep applies
+  // to the dq it loads (and stores dq0 for the later sweeps): no dq0 pass.
+  const bool fuse_dq0 = !chem && o.dplur_steps > 0;
+  if (!fuse_dq0) {
+    prof_mark(P_DQ0, st, true);
+    if (chem) {
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp));
+    } else {
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp));
+    }
+    prof_mark(P_DQ0, st, false);
+    if (last_err()) return OSTC_ERR_HIP;
   }
-  prof_mark(P_DQ0, st, false);
-  if (last_err()) return OSTC_ERR_HIP;
-  // k-marching sweeps: (64 x TJ) tiles, k split into chunks so that about
+  // k-marching sweeps: (64 x 4) tiles, k split into chunks so that about
   // `target` workgroups are in flight; at least 4 planes per chunk
-  const int tj = o.dplur_tj == 8 ? 8 : 4;
+  constexpr int tj = 4;
   const int target = o.dplur_wg > 0 ? o.dplur_wg : 1024;
   const int tiles = ((z.mx + 63) / 64) * ((z.my + tj - 1) / tj);
   const int nkc = std::max(1, std::min((target + tiles - 1) / tiles, (z.mz + 3) / 4));
   const int kc = (z.mz + nkc - 1) / nkc;
   const dim3 sgrid((unsigned)((z.mx + 63) / 64), (unsigned)((z.my + tj - 1) / tj), (unsigned)((z.mz + kc - 1) / kc));
   const dim3 sblock(64, tj, 1);
-  R *in = z.dq0, *out = z.dqk;
+  R *in = fuse_dq0 ? z.dq : z.dq0, *out = z.dqk;
   for (int s = 0; s < o.dplur_steps; ++s) {
-    prof_mark(P_DPLUR, st, true);
+    const int stage = fuse_dq0 && s == 0 ? P_DPLUR_FIRST : P_DPLUR;
+    prof_mark(stage, st, true);
     if (chem) {
-      if (tj == 4)
-        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true, 4>), sgrid, sblock, 0, st, z, dp,
-                                                 (const R *)in, out, kc));
-      else
-        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true, 8>), sgrid, sblock, 0, st, z, dp,
-                                                 (const R *)in, out, kc));
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, true, tj, false>), sgrid, sblock, 0, st, z, dp,
+                                               (const R *)in, out, kc));
+    } else if (s == 0) {
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false, tj, true>), sgrid, sblock, 0, st, z, dp,
+                                               (const R *)in, out, kc));
     } else {
-      if (tj == 4)
-        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false, 4>), sgrid, sblock, 0, st, z, dp,
-                                                 (const R *)in, out, kc));
-      else
-        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false, 8>), sgrid, sblock, 0, st, z, dp,
-                                                 (const R *)in, out, kc));
+      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dplur<R, NS_, NT_, false, tj, false>), sgrid, sblock, 0, st, z,
+                                               dp, (const R *)in, out, kc));
     }
-    prof_mark(P_DPLUR, st, false);
+    prof_mark(stage, st, false);
     if (last_err()) return OSTC_ERR_HIP;
     in = out;
     out = (out == z.dqk) ? z.dq : z.dqk;

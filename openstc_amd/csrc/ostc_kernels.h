@@ -9,7 +9,7 @@ namespace ostc {
 // ostc_profile); prof_mark is a no-op when profiling is off.
 enum ProfStage {
   P_SOURCE = 0, P_FLUX_I, P_FLUX_J, P_FLUX_K, P_DT, P_DQ0, P_DPLUR, P_EXPL_IMPL, P_UPDATE, P_LIMIT, P_BC,
-  P_HALO, P_PROPS, P_STORE, P_RESID, P_FLUX_IJ, P_COUNT
+  P_HALO, P_PROPS, P_STORE, P_RESID, P_FLUX_IJ, P_DPLUR_FIRST, P_COUNT
 };
 void prof_mark(int stage, hipStream_t st, bool begin);
 
@@ -35,7 +35,7 @@ struct CellOpts {
   int mix_model;   // OSTC_AIR / OSTC_MIXTURE / OSTC_FR
   int chem;        // chem_src_method
   int rans_model, turb_implicit, implicit_method, dplur_steps;
-  int dplur_tj, dplur_wg;  // DPLUR sweep tile height (4 or 8) and workgroup target (launch tuning)
+  int dplur_wg;            // DPLUR sweep workgroup target (launch tuning)
 };
 template <typename R>
 int cv_from_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
