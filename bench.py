@@ -95,7 +95,9 @@ def alg_scalars(nv: int, rans: bool, dplur: bool):
     flux = 6 + nsc + 9 + 1 + 2 + (turb - 2 if rans else 0) + 2 * nv  # bv, sv, metric, J, mu, lambda (+mut, lt, d), dq RMW
     return {
         "flux_i": flux, "flux_j": flux, "flux_k": flux, "flux_ij": flux,
-        "source": (4 + nsc + 9 + 1 + 1 + 1) + (2 + nv + 2) if rans else nv,
+        # SST: + the local time step formed in the same kernel (sound speed
+        # read, spectral radii 3 + dt written)
+        "source": (4 + nsc + 9 + 1 + 1 + 1) + (2 + nv + 2) + (1 + 3 + 1) if rans else nv,
         "local_dt": 9 + 3 + 3 + (1 if rans else 0) + 4,
         "dplur_dq0": nv + 1 + 3 + (2 if rans else 0) + nv,
         "dplur_sweep": nv + 5 + 1 + nsc + 9 + 3 + 1 + nv + (2 if rans else 0) + nv,

@@ -573,18 +573,23 @@ int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_fac
     }
     if (r) return r;
     bool wrote = false;
+    // SST zones without chemistry: the source kernel also forms the local
+    // time step (it holds the metric, density, viscosity and mut)
+    const bool dt_in_source = source_has_dt(o);
     prof_mark(P_SOURCE, st, true);
-    r = ZONE_CALL(z, source_terms(z->vf, c->dev, o, st, false, &wrote),
-                  source_terms(z->vd, c->dev, o, st, false, &wrote));
+    r = ZONE_CALL(z, source_terms(z->vf, c->dev, o, st, false, &wrote, dt_in_source),
+                  source_terms(z->vd, c->dev, o, st, false, &wrote, dt_in_source));
     prof_mark(P_SOURCE, st, false);
     if (r) return r;
     r = ZONE_CALL(z, flux_sweeps(z->vf, c->dev, o.ns, o.nt, true, visc, !wrote, st),
                   flux_sweeps(z->vd, c->dev, o.ns, o.nt, true, visc, !wrote, st));
     if (r) return r;
-    prof_mark(P_DT, st, true);
-    r = ZONE_CALL(z, local_time_step(z->vf, c->dev, o, st), local_time_step(z->vd, c->dev, o, st));
-    prof_mark(P_DT, st, false);
-    if (r) return r;
+    if (!dt_in_source) {
+      prof_mark(P_DT, st, true);
+      r = ZONE_CALL(z, local_time_step(z->vf, c->dev, o, st), local_time_step(z->vd, c->dev, o, st));
+      prof_mark(P_DT, st, false);
+      if (r) return r;
+    }
     if (o.implicit_method == 0) prof_mark(P_EXPL_IMPL, st, true);
     {
       // implicit treatment, then the update with the T-from-e solve

@@ -153,13 +153,62 @@ __global__ __launch_bounds__(256) void k_init_mut(ZoneView<R> z, const ostc_para
   z.mut[id] = mut;
 }
 
+// ------------------------------------------------------------------ dt
+
+// local_time_step (src/TimeAdvanceFunc.cuh:21-71) of one cell from its metric
+// m, velocity, density, laminar and eddy viscosity; the viscous radius is not
+// stored (the reference never reads visc_spectr_rad)
+template <typename R, int NS, int NT>
+__device__ __forceinline__ void local_dt_cell(const ZoneView<R> &z, const ostc_param *__restrict__ p, int64_t id,
+                                              const R *m, R u, R v, R w, R rho, R mul, R mut) {
+  const int64_t P = z.plane;
+  const R gx = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const R ge = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+  const R gz = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+  const R U = u * m[0] + v * m[1] + w * m[2];
+  const R V = u * m[3] + v * m[4] + w * m[5];
+  const R W = u * m[6] + v * m[7] + w * m[8];
+  const R a = z.sound[id];
+  const bool three = z.mz > 1;
+  const R r0 = fabs(U) + a * gx, r1 = fabs(V) + a * ge, r2 = three ? fabs(W) + a * gz : (R)0;
+  z.spectr[id] = r0;
+  z.spectr[P + id] = r1;
+  z.spectr[2 * P + id] = r2;
+  const R sri = r0 + r1 + r2;
+  R gam = (R)kGammaAir;
+  if constexpr (NS > 0) gam = z.gamma[id];
+  const R c1 = dmax(gam, (R)(4.0 / 3.0)) / rho;
+  R c2 = mul / (R)p->Pr;
+  if constexpr (NT > 0) c2 += mut / (R)p->Prt;
+  R srv = gx * gx + ge * ge;
+  if (three) srv += gz * gz;
+  srv *= c1 * c2;
+  z.dt[id] = (R)p->cfl / (sri + srv);
+}
+
+template <typename R, int NS, int NT>
+__global__ __launch_bounds__(256) void k_local_dt(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  R m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = z.metric[q * P + id];
+  local_dt_cell<R, NS, NT>(z, p, id, m, z.bv[P + id], z.bv[2 * P + id], z.bv[3 * P + id], z.bv[id], z.mul[id],
+                           NT > 0 ? z.mut[id] : (R)0);
+}
+
 // ------------------------------------------------------------------ source
 
 // compute_source (src/SourceTerm.cuh:11-40): SST source and mut
 // (src/SST.cu:60-178), turbulent conductivity, finite-rate chemistry
-// (src/FiniteRateChem.cu:7-63, FP64).  First writer of dq.
-template <typename R, int NS, int NT, bool CHEM>
+// (src/FiniteRateChem.cu:7-63, FP64).  First writer of dq.  DT (SST zones in
+// the time step): also the local time step of the cell from the metric,
+// density, viscosity and mut this kernel holds -- local_time_step reads
+// nothing the flux stages between them write.
+template <typename R, int NS, int NT, bool CHEM, bool DT>
 __global__ __launch_bounds__(256) void k_source(ZoneView<R> z, const ostc_param *__restrict__ p) {
+  static_assert(!DT || NT > 0, "the fused time step needs the SST branch's metric and mut");
   constexpr int NV = 5 + NS + NT;
   const auto c = cell_index(z, 0);
   if (!c.ok) return;
@@ -228,6 +277,7 @@ __global__ __launch_bounds__(256) void k_source(ZoneView<R> z, const ostc_param 
       z.lambda_t[id] = mut * z.cp[id] / (R)p->Prt;
     else
       z.lambda_t[id] = mut * (R)(kGammaAir * kRu / kMwAir / (kGammaAir - 1)) / (R)p->Prt;
+    if constexpr (DT) local_dt_cell<R, NS, NT>(z, p, id, m, u[id], v[id], w[id], rho, z.mul[id], mut);
   }
   if constexpr (CHEM && NS > 0) {
     double y[NS], omega[NS];
@@ -253,43 +303,6 @@ __global__ __launch_bounds__(256) void k_source(ZoneView<R> z, const ostc_param 
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) z.dq[l * P + id] = dq[l];
-}
-
-// ------------------------------------------------------------------ dt
-
-// local_time_step (src/TimeAdvanceFunc.cuh:21-71); the viscous radius is not
-// stored (the reference never reads visc_spectr_rad)
-template <typename R, int NS, int NT>
-__global__ __launch_bounds__(256) void k_local_dt(ZoneView<R> z, const ostc_param *__restrict__ p) {
-  const auto c = cell_index(z, 0);
-  if (!c.ok) return;
-  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
-  R m[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) m[q] = z.metric[q * P + id];
-  const R gx = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
-  const R ge = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
-  const R gz = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
-  const R u = z.bv[P + id], v = z.bv[2 * P + id], w = z.bv[3 * P + id];
-  const R U = u * m[0] + v * m[1] + w * m[2];
-  const R V = u * m[3] + v * m[4] + w * m[5];
-  const R W = u * m[6] + v * m[7] + w * m[8];
-  const R a = z.sound[id];
-  const bool three = z.mz > 1;
-  const R r0 = fabs(U) + a * gx, r1 = fabs(V) + a * ge, r2 = three ? fabs(W) + a * gz : (R)0;
-  z.spectr[id] = r0;
-  z.spectr[P + id] = r1;
-  z.spectr[2 * P + id] = r2;
-  const R sri = r0 + r1 + r2;
-  R gam = (R)kGammaAir;
-  if constexpr (NS > 0) gam = z.gamma[id];
-  const R c1 = dmax(gam, (R)(4.0 / 3.0)) / z.bv[id];
-  R c2 = z.mul[id] / (R)p->Pr;
-  if constexpr (NT > 0) c2 += z.mut[id] / (R)p->Prt;
-  R srv = gx * gx + ge * ge;
-  if (three) srv += gz * gz;
-  srv *= c1 * c2;
-  z.dt[id] = (R)p->cfl / (sri + srv);
 }
 
 // ------------------------------------------------------------------ implicit
@@ -1184,15 +1197,21 @@ int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o
 
 template <typename R>
 int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, bool force,
-                 bool *wrote) {
+                 bool *wrote, bool with_dt) {
   const bool chem = o.mix_model == OSTC_FR;
   *wrote = false;
+  if (with_dt && !source_has_dt(o)) return OSTC_ERR_ARG;
   if (o.nt == 0 && !chem && !force) return OSTC_OK;
   const auto g = cell_grid(z.mx, z.my, z.mz);
   if (chem) {
-    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp));
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, true, false>), g.grid, g.block, 0, st, z, dp));
+  } else if (with_dt) {
+    if (o.ns == 0)
+      hipLaunchKernelGGL((k_source<R, 0, 2, false, true>), g.grid, g.block, 0, st, z, dp);
+    else
+      hipLaunchKernelGGL((k_source<R, 9, 2, false, true>), g.grid, g.block, 0, st, z, dp);
   } else {
-    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp));
+    OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, false, false>), g.grid, g.block, 0, st, z, dp));
   }
   *wrote = true;
   return last_err();
@@ -1335,7 +1354,7 @@ int wall_distance(const ZoneView<R> &z, const double *wall_xyz, int n_wall, hipS
   template int physical_properties<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);  \
   template int initialize_mut<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);       \
   template int source_terms<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t, bool,   \
-                                bool *);                                                                        \
+                                bool *, bool);                                                                  \
   template int local_time_step<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);      \
   template int implicit_treatment<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t,    \
                                      R **);                                                                     \

@@ -43,10 +43,13 @@ template <typename R>
 int physical_properties(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
 template <typename R>
 int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
-// writes dq (first writer; zeros when the model has no source and force is set)
+// writes dq (first writer; zeros when the model has no source and force is set);
+// with_dt: also the local time step (spectr, dt), for the zones where
+// source_has_dt holds -- the time step then skips local_time_step
 template <typename R>
 int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, bool force,
-                 bool *wrote);
+                 bool *wrote, bool with_dt = false);
+inline bool source_has_dt(const CellOpts &o) { return o.nt > 0 && o.mix_model != OSTC_FR; }
 template <typename R>
 int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
 // leaves the final increment in *result (one of z.dq/z.dq0/z.dqk)
