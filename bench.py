@@ -35,11 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # profiles/kernels_rans_256x128x64_fr_192x96x96_fourblock_4x256x192x128.md.  Re-measured whenever
 # the kernel changes; a stage without an entry reports null.
 MEASURED_TRAFFIC = {
-    ("config2_air_sst_dplur", "dplur_sweep"): 471.1e6,
-    ("config2_air_sst_dplur", "flux_ij"): 402.8e6,
-    ("config2_air_sst_dplur", "flux_k"): 402.2e6,
-    ("config2_air_sst_dplur", "source"): 312.0e6,
-    ("config2_air_sst_dplur", "update"): 273.7e6,
+    ("config2_air_sst_dplur", "dplur_sweep"): 478.8e6,
+    ("config2_air_sst_dplur", "dplur_sweep_first"): 494.3e6,
+    ("config2_air_sst_dplur", "flux_ij"): 402.7e6,
+    ("config2_air_sst_dplur", "flux_k"): 402.3e6,
+    ("config2_air_sst_dplur", "source"): 354.7e6,
+    ("config2_air_sst_dplur", "update"): 273.5e6,
 }
 
 
