@@ -101,6 +101,26 @@ def test_chem_source_and_plain_step_vs_oracle():
     ctx.close()
 
 
+def test_config3_nonreacting_mixture_fp64():
+    """The 9-species mixture without reactions (mix_model = MIXTURE): species
+    transport with DPLUR, whose first sweep forms dq0 itself -- stages and 3
+    iterations against the oracle."""
+    case = cases.config3(mx=28, my=20, mz=10, cfl=0.2)
+    case.param.mix_model = L.MIXTURE
+    say("config3 28x20x10 FP64 non-reacting mixture, DPLUR, stages + 3 steps")
+    pr = Pair(case, precision=L.FP64)
+    pr.init()
+    stage_check(pr, 1e-10, sync32=False)
+    for _ in range(3):
+        pr.gpu.step()
+        pr.orc.step()
+    for f in ("BV", "SV"):
+        e = pr.err(f, halo=2)
+        say("after the stages and 3 steps", f, e)
+        assert e < 1e-8, (f, e)
+    pr.close()
+
+
 def test_config3_fp64_stages():
     case = cases.config3(mx=28, my=20, mz=10)
     say("config3 28x20x10 FP64 stages (FR, EPI, DPLUR)")

@@ -112,6 +112,28 @@ def test_config1_fp64_iterations():
     pr.close()
 
 
+def test_config1_fp64_dplur_laminar():
+    """Laminar inviscid air with DPLUR (3 sweeps): the first sweep forms dq0
+    from dq itself (no separate dq0 pass) -- stages and 6 iterations against
+    the oracle."""
+    case = cases.config1(mx=40, my=24, mz=8)
+    case.param.implicit_method = 1
+    case.param.dplur_inner_step = 3
+    say("config1 40x24x8 FP64, DPLUR x3, stages + 6 steps")
+    pr = Pair(case)
+    pr.init()
+    stage_check(pr, TOL64, sync32=False)
+    for s in range(6):
+        rg = pr.gpu.step(output_screen=3)
+        ro = pr.orc.step(output_screen=3)
+        if rg is not None:
+            assert abs(rg - ro) <= 1e-8 * max(1.0, abs(ro)), (s, rg, ro)
+    e = pr.err("BV", halo=2)
+    say("bv err after the stages and 6 steps", e)
+    assert e < 1e-9
+    pr.close()
+
+
 def test_config2_fp64_stages():
     case = cases.config2(mx=48, my=32, mz=12, perturb=True)
     say("config2 48x32x12 FP64 stages")
