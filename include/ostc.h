@@ -331,9 +331,13 @@ int ostc_residual_sums(ostc_ctx *ctx, ostc_zone **zones, int n, double sums[4], 
 int ostc_zone_counters(ostc_zone *zone, long long out[4], int reset, void *stream);
 
 /* One steady iteration of Driver::steady_simulation over n zones
- * (src/Driver.cu:218-281) in the reference's stage order.  inner faces of
- * block b are inner[off_b .. off_b+n_inner[b]).  Returns the residual sums
- * in res_sums[4] when step%output_screen==0 or step==1 (else leaves them). */
+ * (src/Driver.cu:218-281) in the reference's stage order.  Two stages are
+ * fused with the same per-cell arithmetic: in SST zones without chemistry the
+ * source kernel also forms the local time step (nothing the flux stages
+ * write enters it), and in non-reacting zones the first DPLUR sweep forms
+ * dq0 from dq as it loads it.  inner faces of block b are
+ * inner[off_b .. off_b+n_inner[b]).  Returns the residual sums in
+ * res_sums[4] when step%output_screen==0 or step==1 (else leaves them). */
 int ostc_steady_step(ostc_ctx *ctx, ostc_zone **zones, int n, const ostc_inner_face *inner, const int *n_inner,
                      int step, int output_screen, double res_sums[4], int *res_computed, void *stream);
 
