@@ -121,6 +121,32 @@ def test_config3_nonreacting_mixture_fp64():
     pr.close()
 
 
+@pytest.mark.parametrize("name,chem_method", [("config3", 1), ("config3", 2), ("config2", 0)])
+def test_explicit_point_implicit_fp64(name, chem_method):
+    """Explicit time stepping with the point-implicit treatment of the stiff
+    sources (src/ImplicitTreatmentHPP.cuh:10-59): EPI (chemSrcMethod 1) and
+    DA (2) chemistry on the finite-rate mixing layer, the SST rows on the
+    flat plate -- stages and 3 iterations against the oracle."""
+    if name == "config3":
+        case = cases.config3(mx=28, my=20, mz=10, cfl=0.2, chem_method=chem_method)
+    else:
+        case = cases.config2(mx=48, my=32, mz=12, cfl=0.5, perturb=True)
+    case.param.implicit_method = 0
+    case.param.dplur_inner_step = 0
+    say(f"{name} FP64 explicit + point-implicit (chem method {chem_method}), stages + 3 steps")
+    pr = Pair(case, precision=L.FP64)
+    pr.init()
+    stage_check(pr, 1e-10, sync32=False)
+    for _ in range(3):
+        pr.gpu.step()
+        pr.orc.step()
+    for f in ("BV", "SV"):
+        e = pr.err(f, halo=2)
+        say("after the stages and 3 steps", f, e)
+        assert e < 1e-8, (f, e)
+    pr.close()
+
+
 def test_config3_fp64_stages():
     case = cases.config3(mx=28, my=20, mz=10)
     say("config3 28x20x10 FP64 stages (FR, EPI, DPLUR)")
