@@ -134,6 +134,31 @@ def test_config1_fp64_dplur_laminar():
     pr.close()
 
 
+@pytest.mark.parametrize("name,recon", [("config2", 1), ("config2", 3), ("config3", 1), ("config3", 3)])
+def test_reconstruction_variants_fp64(name, recon):
+    """First-order (1) and NND-2 (3) reconstruction (src/Reconstruction.cu:
+    5-46, the reference's sample setup uses 3): the marching air/SST flux
+    kernels and the 9-species mixture sweep kernels -- stages and 3
+    iterations against the oracle."""
+    if name == "config2":
+        case = cases.config2(mx=48, my=32, mz=12, perturb=True)
+    else:
+        case = cases.config3(mx=28, my=20, mz=10, cfl=0.2)
+    case.param.reconstruction = recon
+    say(f"{name} FP64 reconstruction {recon}, stages + 3 steps")
+    pr = Pair(case, precision=L.FP64)
+    pr.init()
+    stage_check(pr, 1e-10, sync32=False)
+    for _ in range(3):
+        pr.gpu.step()
+        pr.orc.step()
+    for f in ("BV", "SV"):
+        e = pr.err(f, halo=2)
+        say("after the stages and 3 steps", f, e)
+        assert e < 1e-8, (f, e)
+    pr.close()
+
+
 def test_config2_fp64_stages():
     case = cases.config2(mx=48, my=32, mz=12, perturb=True)
     say("config2 48x32x12 FP64 stages")
