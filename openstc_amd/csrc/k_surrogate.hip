@@ -241,6 +241,16 @@ __global__ __launch_bounds__(kWaves * 64) void k_sur_forward(SurParams s, Src sr
   __shared__ bf16x8 sW[kSurTiles * 64];
   __shared__ __attribute__((aligned(16))) float sB[kSurBias];
   __shared__ float sStage[kWaves][64 * kStageLd];
+  static_assert(kWaves * 64 >= kFbLists, "workgroup 0 clears every sub-list counter");
+  if constexpr (!TRAIN) {
+    // clear the fallback sub-list counters and statistics rows that the
+    // decode kernel (next on the stream) appends to
+    if (blockIdx.x == 0 && threadIdx.x < kFbLists && io.fb_count) {
+      io.fb_count[threadIdx.x * kFbStride] = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) io.stats[threadIdx.x * kStatStride + r] = 0;
+    }
+  }
   for (int q = threadIdx.x; q < kSurTiles * 64; q += blockDim.x) sW[q] = wpk[q];
   for (int q = threadIdx.x; q < kSurBias; q += blockDim.x) sB[q] = bias[q];
   __syncthreads();
@@ -519,13 +529,6 @@ __global__ __launch_bounds__(256) void k_zone_plain(const ostc_param *__restrict
   for (int k = 0; k < kNs; ++k) dyo[c * kNs + k] = dy[k];
 }
 
-__global__ __launch_bounds__(kFbLists) void k_sur_reset(int *count, unsigned long long *stats) {
-  const int l = threadIdx.x;
-  count[l * kFbStride] = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) stats[l * kStatStride + r] = 0;
-}
-
 // ---- training kernels -----------------------------------------------------
 // dL/dY of L = sum_b sum_o w_o (y - t)^2 / B; per-block partial losses.
 __global__ __launch_bounds__(256) void k_sur_loss(const float *__restrict__ yout, const float *__restrict__ target,
@@ -728,7 +731,6 @@ static void launch_forward(const SurState &st, const Src &src, const TrainOut &t
 template <typename Src>
 static int infer_impl(const SurState &st, const ostc_param *dp, const Src &src, double *dy, float *raw,
                       hipStream_t s) {
-  hipLaunchKernelGGL(k_sur_reset, dim3(1), dim3(kFbLists), 0, s, st.fb_count, st.stats);
   TrainOut tr{};
   InferOut io{st.out12, dy, st.fb_count, st.fb_list, fb_list_cap(st.fb_cap), st.stats, raw};
   launch_forward<Src, false>(st, src, tr, io, s);
