@@ -35,12 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # profiles/kernels_rans_256x128x64_fr_192x96x96_fourblock_4x256x192x128.md.  Re-measured whenever
 # the kernel changes; a stage without an entry reports null.
 MEASURED_TRAFFIC = {
-    ("config2_air_sst_dplur", "dplur_sweep"): 478.8e6,
-    ("config2_air_sst_dplur", "dplur_sweep_first"): 494.3e6,
-    ("config2_air_sst_dplur", "flux_ij"): 402.7e6,
-    ("config2_air_sst_dplur", "flux_k"): 402.3e6,
-    ("config2_air_sst_dplur", "source"): 354.7e6,
-    ("config2_air_sst_dplur", "update"): 273.5e6,
+    ("sst_flat_plate_dplur", "dplur_sweep"): 478.8e6,
+    ("sst_flat_plate_dplur", "dplur_sweep_first"): 494.3e6,
+    ("sst_flat_plate_dplur", "flux_ij"): 402.7e6,
+    ("sst_flat_plate_dplur", "flux_k"): 402.3e6,
+    ("sst_flat_plate_dplur", "source"): 354.7e6,
+    ("sst_flat_plate_dplur", "update"): 273.5e6,
 }
 
 
@@ -173,7 +173,7 @@ def chemistry_leg(steps: int, warmup: int, eval_iter: int, reps: int, weights: s
     from openstc_amd import _lib as L
     from openstc_amd.runtime import Simulation
 
-    case = cases.config3(cfl=0.2, seed=5)
+    case = cases.mixing_layer(cfl=0.2, seed=5)
     cells = sum(b.mx * b.my * b.mz for b in case.blocks)
     print(f"[bench] {case.name}: {cells} cells, setup", file=sys.stderr, flush=True)
     sim = Simulation(case)
@@ -257,7 +257,7 @@ def scale_leg(steps: int, warmup: int, reps: int, dims=(256, 192, 128), nblocks:
 
     mx, my, mz = dims
     t0 = time.perf_counter()
-    case = cases.config5(mx=mx, my=my, mz=mz, nblocks=nblocks)
+    case = cases.jet_crossflow(mx=mx, my=my, mz=mz, nblocks=nblocks)
     cells = sum(b.mx * b.my * b.mz for b in case.blocks)
     print(f"[bench] {case.name}: {cells} cells, setup", file=sys.stderr, flush=True)
     sim = Simulation(case)
@@ -346,9 +346,9 @@ def main():
     from openstc_amd.runtime import Simulation
 
     if a.config == 2:
-        build = lambda: cases.config2()  # noqa: E731
+        build = lambda: cases.sst_plate()  # noqa: E731
     else:
-        build = lambda: cases.config1(mx=256, my=256, mz=64)  # noqa: E731
+        build = lambda: cases.warped_box(mx=256, my=256, mz=64)  # noqa: E731
     case = build()
     cells = sum(b.mx * b.my * b.mz for b in case.blocks)
     print(f"[bench] {case.name}: {cells} cells, setup", file=sys.stderr, flush=True)

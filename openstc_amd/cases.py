@@ -270,7 +270,7 @@ def perturb_interior(arr, blk: Block, seed: int, var: int, ngg: int, rel: float 
         arr[sl] += absval * u
 
 
-def config1(mx=64, my=64, mz=8, steps=100, precision=L.FP64, seed=1) -> Case:
+def warped_box(mx=64, my=64, mz=8, steps=100, precision=L.FP64, seed=1) -> Case:
     """Plain method, small: single-species air, laminar inviscid, one warped
     64x64x8 block, Mach 2 freestream (p=101325 Pa, T=300 K) with +-1% seeded
     perturbation of rho,u,v,w,p; MUSCL+minmod+AUSM+, explicit Euler, 100 steps;
@@ -304,19 +304,19 @@ def config1(mx=64, my=64, mz=8, steps=100, precision=L.FP64, seed=1) -> Case:
     setup = setup_spec(steps, steps, 0.5, False, 0, 2, False,
                        bcs=[("freestream", dict(type="inflow", label=5, mach=2.0, pressure=101325.0,
                                                 temperature=300.0, u=1.0, v=0.0, w=0.0))])
-    return Case("config1_air_inviscid_explicit", param, [blk], ngg, precision, steps, steps, None, seed,
+    return Case("warped_box_air_euler_explicit", param, [blk], ngg, precision, steps, steps, None, seed,
                 dict(mach=2.0, p=101325.0, T=300.0, setup=setup))
 
 
-def config1_split(mx=64, my=64, mz=8, split=None, steps=100, precision=L.FP64, seed=1) -> Case:
-    """The configuration-1 warped box cut at i = split into two blocks that
+def warped_box_split(mx=64, my=64, mz=8, split=None, steps=100, precision=L.FP64, seed=1) -> Case:
+    """The warped box cut at i = split into two blocks that
     share the node plane i = split (block 1 = i <= split, block 2 = i >= split),
     connected by an inner face each way (the reference's inner{rank}.txt
     records, registered by libostc); inflow on block 1's i = 0, outflow on block
     2's i = max, symmetry elsewhere; same perturbed Mach 2 field."""
     ngg = 2
     split = mx // 2 if split is None else split
-    full = config1(mx, my, mz, steps, precision, seed)
+    full = warped_box(mx, my, mz, steps, precision, seed)
     gb = full.blocks[0]
     fs = air_inflow(2.0, 101325.0, 300.0)
     from .runtime import mesh_geometry, register_inner_face
@@ -349,7 +349,7 @@ def config1_split(mx=64, my=64, mz=8, split=None, steps=100, precision=L.FP64, s
         lo, hi = i0, i1 + 2 * ngg + 1
         blk.bv0 = full.blocks[0].bv0[:, :, :, lo:hi].copy()
         blk.sv0 = full.blocks[0].sv0[:, :, :, lo:hi].copy()
-    return Case("config1_split_2blocks", full.param, blocks, ngg, precision, steps, steps, None, seed,
+    return Case("warped_box_split_2blocks", full.param, blocks, ngg, precision, steps, steps, None, seed,
                 dict(split=split, setup=full.meta["setup"]))
 
 
@@ -372,7 +372,7 @@ def _tanh_stretch(n, height, first):
     return height * (1 - np.tanh(beta * (1 - s)) / math.tanh(beta)), beta
 
 
-def config2(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, cfl=5.0, height=0.1,
+def sst_plate(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, cfl=5.0, height=0.1,
             span=0.25, seed=2, perturb=False) -> Case:
     """Viscous RANS: air, k-omega SST, flat plate with tanh wall clustering
     (first cell y+~1 from the turbulent flat-plate Cf correlation at x=0.5 m),
@@ -421,7 +421,7 @@ def config2(mx=256, my=128, mz=64, steps=1000, precision=L.FP32, dplur_steps=4, 
                                                 u=1.0, v=0.0, w=0.0, turbulence_intensity=0.01,
                                                 turb_viscosity_ratio=1.0)),
                             ("wall", dict(type="wall", label=2, thermal_type="adiabatic"))])
-    return Case("config2_air_sst_dplur", param, [blk], ngg, precision, steps, 100, wall, seed,
+    return Case("sst_flat_plate_dplur", param, [blk], ngg, precision, steps, 100, wall, seed,
                 dict(mach=mach, p=p_inf, T=t_inf, rho=rho, u=u_inf, y_first=y_first, beta=beta, setup=setup))
 
 
@@ -558,7 +558,7 @@ def mixture_param(mech: L.Param, rans: bool, implicit: bool, dplur_steps: int, c
     return p
 
 
-def config3(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=4, cfl=1.0, chem_method=1,
+def mixing_layer(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=4, cfl=1.0, chem_method=1,
             lx=0.2, ly=0.1, lz=0.1, seed=3) -> Case:
     """Finite-rate chemistry: 9-species/19-reaction H2-air, Cartesian box with
     a seeded mixing layer: phi = (1+tanh((y-y0)/(2*dy)))/2 between H2 (Y=1,
@@ -623,11 +623,11 @@ def config3(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=4, cf
                                                 u=1.0, v=0.0, w=0.0, O2=0.233, N2=0.767)),
                             ("fuel", dict(type="inflow", label=9, mach=1.2, pressure=101325.0, temperature=1000.0,
                                           u=1.0, v=0.0, w=0.0, H2=1.0))])
-    return Case("config3_h2air_fr_epi_dplur", param, [blk], ngg, precision, steps, 100, None, seed,
+    return Case("h2air_mixing_layer_fr_epi_dplur", param, [blk], ngg, precision, steps, 100, None, seed,
                 dict(species=names, air=air.__dict__, fuel=fuel.__dict__, mech_info=info, setup=setup))
 
 
-def config5(mx=256, my=192, mz=128, nblocks=4, steps=200, precision=L.FP32, dplur_steps=4, cfl=1.0, chem_method=1,
+def jet_crossflow(mx=256, my=192, mz=128, nblocks=4, steps=200, precision=L.FP32, dplur_steps=4, cfl=1.0, chem_method=1,
             lx=0.4, ly=0.1, lz=0.05, seed=5) -> Case:
     """Full path at scale: SST RANS + 9-species finite-rate chemistry on
     `nblocks` blocks of mx x my x mz joined along i by inner faces (shared node
@@ -738,5 +738,5 @@ def config5(mx=256, my=192, mz=128, nblocks=4, steps=200, precision=L.FP32, dplu
                                          u=0.0, v=1.0, w=0.0, H2=1.0, turbulence_intensity=0.01,
                                          turb_viscosity_ratio=1.0)),
                             ("wall", dict(type="wall", label=2, thermal_type="isothermal", temperature=300.0))])
-    return Case(f"config5_h2jet_vitiated_fr_sst_{nblocks}x{mx}x{my}x{mz}", param, blocks, ngg, precision, steps, 100,
+    return Case(f"h2_jet_vitiated_crossflow_fr_sst_{nblocks}x{mx}x{my}x{mz}", param, blocks, ngg, precision, steps, 100,
                 wall, seed, dict(species=names, air=air.__dict__, jet=jet.__dict__, mech_info=info, setup=setup))

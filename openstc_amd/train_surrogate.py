@@ -65,8 +65,8 @@ def main(argv=None):
 
     def build(seed):
         if jet:
-            return cases.config5(mx=mx, my=my, mz=mz, cfl=a.cfl, seed=seed)
-        return cases.config3(mx=mx, my=my, mz=mz, cfl=a.cfl, seed=seed)
+            return cases.jet_crossflow(mx=mx, my=my, mz=mz, cfl=a.cfl, seed=seed)
+        return cases.mixing_layer(mx=mx, my=my, mz=mz, cfl=a.cfl, seed=seed)
 
     train_seed, val_seed = (6, 7) if jet else (3, 4)
     log(f"[train_surrogate] harvest {a.workload} {mx}x{my}x{mz}, {a.steps} iterations, CFL {a.cfl}")

@@ -63,7 +63,7 @@ def test_host_geometry_bit_exact_with_oracle():
     from openstc_amd import cases
     from openstc_amd.runtime import block_geometry
 
-    c = cases.config1(mx=20, my=17, mz=9)
+    c = cases.warped_box(mx=20, my=17, mz=9)
     blk = c.blocks[0]
     ob = O.OracleBlock(blk.mx, blk.my, blk.mz, 2, O.copy_struct(O.OrcParam, c.param))
     ob.geometry(blk.x, blk.y, blk.z)

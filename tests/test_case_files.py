@@ -49,11 +49,11 @@ def struct_diffs(a, b, name=""):
 
 
 BUILDERS = {
-    "config1": lambda: cases.config1(mx=20, my=16, mz=8, steps=3),
-    "config2": lambda: cases.config2(mx=24, my=20, mz=8, steps=3),
-    "config3": lambda: cases.config3(mx=16, my=12, mz=8, steps=3),
-    "config1_split": lambda: cases.config1_split(mx=20, my=16, mz=8, steps=3),
-    "config5": lambda: cases.config5(mx=16, my=12, mz=10, nblocks=4, steps=3),
+    "warped_box": lambda: cases.warped_box(mx=20, my=16, mz=8, steps=3),
+    "sst_plate": lambda: cases.sst_plate(mx=24, my=20, mz=8, steps=3),
+    "mixing_layer": lambda: cases.mixing_layer(mx=16, my=12, mz=8, steps=3),
+    "warped_box_split": lambda: cases.warped_box_split(mx=20, my=16, mz=8, steps=3),
+    "jet_crossflow": lambda: cases.jet_crossflow(mx=16, my=12, mz=10, nblocks=4, steps=3),
 }
 
 
@@ -165,7 +165,7 @@ def test_reference_sample_setup(tmp_path):
 
 
 def write_setup(tmp_path, edit):
-    case = cases.config1(mx=12, my=10, mz=6, steps=3)
+    case = cases.warped_box(mx=12, my=10, mz=6, steps=3)
     d = case_files.write_case(case, str(tmp_path / "c"))
     for fn, fun in edit.items():
         path = os.path.join(d, "input_files", "setup", fn)
@@ -296,7 +296,7 @@ def test_restart_reads_the_flowfield_file(tmp_path):
     output/field/flowfield.plt by variable label (any case, aliases rho/p/T/k;
     unknown labels skipped), ghosts and missing variables from the setup's
     initialisation; a mismatching zone is an error."""
-    case = cases.config2(mx=12, my=10, mz=6)
+    case = cases.sst_plate(mx=12, my=10, mz=6)
     d = case_files.write_case(case, str(tmp_path / "c"))
     base = case_files.CaseDir(d)
     bv0, sv0 = base.field(0, "BV", 6), base.field(0, "SV", 2)
@@ -350,9 +350,9 @@ def inject(cd, case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,precision,steps", [("config1", L.FP64, 5), ("config2", L.FP32, 3),
-                                                   ("config3", L.FP32, 2), ("config1_split", L.FP64, 3),
-                                                   ("config5", L.FP32, 2)])
+@pytest.mark.parametrize("name,precision,steps", [("warped_box", L.FP64, 5), ("sst_plate", L.FP32, 3),
+                                                   ("mixing_layer", L.FP32, 2), ("warped_box_split", L.FP64, 3),
+                                                   ("jet_crossflow", L.FP32, 2)])
 def test_driver_matches_python_loop(tmp_path, name, precision, steps):
     from openstc_amd.runtime import Simulation
     print(f"[gpu-driver] {name} precision {precision} steps {steps}", flush=True)
@@ -380,7 +380,7 @@ def test_driver_matches_python_loop(tmp_path, name, precision, steps):
 
 @pytest.mark.gpu
 def test_command_line_runner(tmp_path):
-    case = cases.config1(mx=24, my=16, mz=8, steps=4)
+    case = cases.warped_box(mx=24, my=16, mz=8, steps=4)
     d = case_files.write_case(case, str(tmp_path / "cli"))
     exe = os.path.join(os.path.dirname(L.LIB_PATH), "ostc_run")
     print("[gpu-driver] ostc_run", d, flush=True)
@@ -397,7 +397,7 @@ def test_command_line_runner(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,precision", [("config2", L.FP64), ("config3", L.FP32)])
+@pytest.mark.parametrize("name,precision", [("sst_plate", L.FP64), ("mixing_layer", L.FP32)])
 def test_flowfield_file_and_restart_round_trip(tmp_path, name, precision):
     """The driver's flow-field file (MPIIO::print_field) read back by the
     independent reader above equals the zone's interior fields and grid; a
@@ -451,7 +451,7 @@ def test_post_process_array_is_read(tmp_path):
     """A setup with post_process { 0 } reads, and the case exposes the
     freestream density and speed of the procedure's dynamic pressure (the
     procedure itself runs on the GPU)."""
-    case = cases.config5(mx=16, my=12, mz=10, nblocks=2, steps=2)
+    case = cases.jet_crossflow(mx=16, my=12, mz=10, nblocks=2, steps=2)
     d = case_files.write_case(case, str(tmp_path / "pp"))
     path = os.path.join(d, "input_files", "setup", "6_post_process.txt")
     open(path, "w").write(open(os.path.join(SAMPLE, "6_post_process.txt")).read().replace("// 0", "0"))
@@ -468,11 +468,11 @@ def test_post_process_array_is_read(tmp_path):
 @pytest.mark.parametrize("precision", [L.FP64, L.FP32])
 def test_wall_friction_heatflux_post_process(tmp_path, precision):
     """Procedure 0 (src/PostProcess.cu:6-70) on an isothermal-wall case
-    (config5, 2 blocks): the device entry point equals the numpy oracle on
+    (jet_crossflow, 2 blocks): the device entry point equals the numpy oracle on
     the downloaded fields of every block, and simulate(io) leaves
     output/wall/friction_heatflux.dat holding the last block's line."""
     import orc_post
-    case = cases.config5(mx=16, my=12, mz=10, nblocks=2, steps=2)
+    case = cases.jet_crossflow(mx=16, my=12, mz=10, nblocks=2, steps=2)
     d = case_files.write_case(case, str(tmp_path / "pp"))
     setup = os.path.join(d, "input_files", "setup")
     open(os.path.join(setup, "6_post_process.txt"), "w").write(
@@ -483,7 +483,7 @@ def test_wall_friction_heatflux_post_process(tmp_path, precision):
     open(ctl, "w").write(text.replace(out_line, "int output_file = 1"))
     cd = case_files.CaseDir(d)
     inject(cd, case)
-    print(f"[gpu-driver] post_process config5 2 blocks precision {precision}", flush=True)
+    print(f"[gpu-driver] post_process jet_crossflow 2 blocks precision {precision}", flush=True)
     drv = case_files.Driver(cd, precision)
     drv.initialize()
     drv.simulate(io=True)

@@ -101,13 +101,13 @@ def test_chem_source_and_plain_step_vs_oracle():
     ctx.close()
 
 
-def test_config3_nonreacting_mixture_fp64():
+def test_mixing_layer_nonreacting_mixture_fp64():
     """The 9-species mixture without reactions (mix_model = MIXTURE): species
     transport with DPLUR, whose first sweep forms dq0 itself -- stages and 3
     iterations against the oracle."""
-    case = cases.config3(mx=28, my=20, mz=10, cfl=0.2)
+    case = cases.mixing_layer(mx=28, my=20, mz=10, cfl=0.2)
     case.param.mix_model = L.MIXTURE
-    say("config3 28x20x10 FP64 non-reacting mixture, DPLUR, stages + 3 steps")
+    say("mixing_layer 28x20x10 FP64 non-reacting mixture, DPLUR, stages + 3 steps")
     pr = Pair(case, precision=L.FP64)
     pr.init()
     stage_check(pr, 1e-10, sync32=False)
@@ -121,16 +121,16 @@ def test_config3_nonreacting_mixture_fp64():
     pr.close()
 
 
-@pytest.mark.parametrize("name,chem_method", [("config3", 1), ("config3", 2), ("config2", 0)])
+@pytest.mark.parametrize("name,chem_method", [("mixing_layer", 1), ("mixing_layer", 2), ("sst_plate", 0)])
 def test_explicit_point_implicit_fp64(name, chem_method):
     """Explicit time stepping with the point-implicit treatment of the stiff
     sources (src/ImplicitTreatmentHPP.cuh:10-59): EPI (chemSrcMethod 1) and
     DA (2) chemistry on the finite-rate mixing layer, the SST rows on the
     flat plate -- stages and 3 iterations against the oracle."""
-    if name == "config3":
-        case = cases.config3(mx=28, my=20, mz=10, cfl=0.2, chem_method=chem_method)
+    if name == "mixing_layer":
+        case = cases.mixing_layer(mx=28, my=20, mz=10, cfl=0.2, chem_method=chem_method)
     else:
-        case = cases.config2(mx=48, my=32, mz=12, cfl=0.5, perturb=True)
+        case = cases.sst_plate(mx=48, my=32, mz=12, cfl=0.5, perturb=True)
     case.param.implicit_method = 0
     case.param.dplur_inner_step = 0
     say(f"{name} FP64 explicit + point-implicit (chem method {chem_method}), stages + 3 steps")
@@ -147,9 +147,9 @@ def test_explicit_point_implicit_fp64(name, chem_method):
     pr.close()
 
 
-def test_config3_fp64_stages():
-    case = cases.config3(mx=28, my=20, mz=10)
-    say("config3 28x20x10 FP64 stages (FR, EPI, DPLUR)")
+def test_mixing_layer_fp64_stages():
+    case = cases.mixing_layer(mx=28, my=20, mz=10)
+    say("mixing_layer 28x20x10 FP64 stages (FR, EPI, DPLUR)")
     pr = Pair(case, precision=L.FP64)
     pr.init()
     for f in ("BV", "SV", "CV", "MUL", "GAMMA", "RHO_D"):
@@ -160,9 +160,9 @@ def test_config3_fp64_stages():
     pr.close()
 
 
-def test_config3_fp64_iterations():
-    case = cases.config3(mx=28, my=20, mz=10, cfl=0.2)
-    say("config3 28x20x10 FP64 4 steps")
+def test_mixing_layer_fp64_iterations():
+    case = cases.mixing_layer(mx=28, my=20, mz=10, cfl=0.2)
+    say("mixing_layer 28x20x10 FP64 4 steps")
     pr = Pair(case, precision=L.FP64)
     pr.init()
     for s in range(4):
@@ -175,9 +175,9 @@ def test_config3_fp64_iterations():
     pr.close()
 
 
-def test_config3_fp32_stages():
-    case = cases.config3(mx=28, my=20, mz=10)
-    say("config3 28x20x10 FP32 stages")
+def test_mixing_layer_fp32_stages():
+    case = cases.mixing_layer(mx=28, my=20, mz=10)
+    say("mixing_layer 28x20x10 FP32 stages")
     pr = Pair(case, precision=L.FP32)
     pr.init()
     stage_check(pr, 1e-4, sync32=True)

@@ -85,9 +85,9 @@ def stage_check(pr: Pair, tol, sync32: bool):
     assert not bad, bad
 
 
-def test_config1_fp64_init_and_stages():
-    case = cases.config1(mx=40, my=24, mz=8)
-    say("config1 40x24x8 FP64 stages")
+def test_warped_box_fp64_init_and_stages():
+    case = cases.warped_box(mx=40, my=24, mz=8)
+    say("warped_box 40x24x8 FP64 stages")
     pr = Pair(case)
     pr.init()
     for f in ("BV", "CV", "SOUND", "MUL"):
@@ -96,9 +96,9 @@ def test_config1_fp64_init_and_stages():
     pr.close()
 
 
-def test_config1_fp64_iterations():
-    case = cases.config1(mx=40, my=24, mz=8)
-    say("config1 40x24x8 FP64 10 steps")
+def test_warped_box_fp64_iterations():
+    case = cases.warped_box(mx=40, my=24, mz=8)
+    say("warped_box 40x24x8 FP64 10 steps")
     pr = Pair(case)
     pr.init()
     for s in range(10):
@@ -112,14 +112,14 @@ def test_config1_fp64_iterations():
     pr.close()
 
 
-def test_config1_fp64_dplur_laminar():
+def test_warped_box_fp64_dplur_laminar():
     """Laminar inviscid air with DPLUR (3 sweeps): the first sweep forms dq0
     from dq itself (no separate dq0 pass) -- stages and 6 iterations against
     the oracle."""
-    case = cases.config1(mx=40, my=24, mz=8)
+    case = cases.warped_box(mx=40, my=24, mz=8)
     case.param.implicit_method = 1
     case.param.dplur_inner_step = 3
-    say("config1 40x24x8 FP64, DPLUR x3, stages + 6 steps")
+    say("warped_box 40x24x8 FP64, DPLUR x3, stages + 6 steps")
     pr = Pair(case)
     pr.init()
     stage_check(pr, TOL64, sync32=False)
@@ -134,16 +134,16 @@ def test_config1_fp64_dplur_laminar():
     pr.close()
 
 
-@pytest.mark.parametrize("name,recon", [("config2", 1), ("config2", 3), ("config3", 1), ("config3", 3)])
+@pytest.mark.parametrize("name,recon", [("sst_plate", 1), ("sst_plate", 3), ("mixing_layer", 1), ("mixing_layer", 3)])
 def test_reconstruction_variants_fp64(name, recon):
     """First-order (1) and NND-2 (3) reconstruction (src/Reconstruction.cu:
     5-46, the reference's sample setup uses 3): the marching air/SST flux
     kernels and the 9-species mixture sweep kernels -- stages and 3
     iterations against the oracle."""
-    if name == "config2":
-        case = cases.config2(mx=48, my=32, mz=12, perturb=True)
+    if name == "sst_plate":
+        case = cases.sst_plate(mx=48, my=32, mz=12, perturb=True)
     else:
-        case = cases.config3(mx=28, my=20, mz=10, cfl=0.2)
+        case = cases.mixing_layer(mx=28, my=20, mz=10, cfl=0.2)
     case.param.reconstruction = recon
     say(f"{name} FP64 reconstruction {recon}, stages + 3 steps")
     pr = Pair(case, precision=L.FP64)
@@ -159,9 +159,9 @@ def test_reconstruction_variants_fp64(name, recon):
     pr.close()
 
 
-def test_config2_fp64_stages():
-    case = cases.config2(mx=48, my=32, mz=12, perturb=True)
-    say("config2 48x32x12 FP64 stages")
+def test_sst_plate_fp64_stages():
+    case = cases.sst_plate(mx=48, my=32, mz=12, perturb=True)
+    say("sst_plate 48x32x12 FP64 stages")
     pr = Pair(case, precision=L.FP64)
     pr.init()
     for f in ("BV", "SV", "CV", "MUT", "WALL_DIST"):
@@ -171,9 +171,9 @@ def test_config2_fp64_stages():
     pr.close()
 
 
-def test_config2_fp64_iterations():
-    case = cases.config2(mx=48, my=32, mz=12, perturb=True)
-    say("config2 48x32x12 FP64 5 steps")
+def test_sst_plate_fp64_iterations():
+    case = cases.sst_plate(mx=48, my=32, mz=12, perturb=True)
+    say("sst_plate 48x32x12 FP64 5 steps")
     pr = Pair(case, precision=L.FP64)
     pr.init()
     for s in range(5):
@@ -186,19 +186,19 @@ def test_config2_fp64_iterations():
     pr.close()
 
 
-def test_config2_fp32_stages():
-    case = cases.config2(mx=48, my=32, mz=12, perturb=True)
-    say("config2 48x32x12 FP32 stages")
+def test_sst_plate_fp32_stages():
+    case = cases.sst_plate(mx=48, my=32, mz=12, perturb=True)
+    say("sst_plate 48x32x12 FP32 stages")
     pr = Pair(case, precision=L.FP32)
     pr.init()
     stage_check(pr, TOL32, sync32=True)
     pr.close()
 
 
-def test_config1_fp32_vs_fp64_trajectory():
+def test_warped_box_fp32_vs_fp64_trajectory():
     """FP32 bulk vs FP64 over 20 explicit steps: field error stays small."""
-    case = cases.config1(mx=40, my=24, mz=8)
-    say("config1 FP32 vs oracle 20 steps")
+    case = cases.warped_box(mx=40, my=24, mz=8)
+    say("warped_box FP32 vs oracle 20 steps")
     pr = Pair(case, precision=L.FP32)
     pr.init()
     for s in range(20):

@@ -182,8 +182,8 @@ def test_fallback_paths(env):
 
 def test_zone_plain_step_equals_batch(env):
     ctx = env[0]
-    case = cases.config3(mx=24, my=16, mz=8)
-    say("zone plain chemistry on config3 24x16x8 FP64")
+    case = cases.mixing_layer(mx=24, my=16, mz=8)
+    say("zone plain chemistry on mixing_layer 24x16x8 FP64")
     from openstc_amd.runtime import Simulation
     sim = Simulation(case, precision=L.FP64)
     sim.initialize_computation()

@@ -5,10 +5,10 @@ ghost grids (src/Mesh.cpp:940-985) and the inner-face halo exchange
 CPU: registration against hand-derived records; the product's multi-block
 geometry bit-exact against the oracle's two-phase restatement; ghost
 coordinates across the inner face equal the neighbour block's points.
-GPU: the two-block split of the configuration-1 box against the oracle
+GPU: the two-block split of the warped Euler box against the oracle
 (initialisation incl. the exchanged ghost layers, then 10 explicit steps),
 FP64 scaled error <= 1e-11; the four-block SST + finite-rate jet case
-(configuration 5 at 16x12x10 per block) after 3 DPLUR iterations: BV and
+(the jet-in-crossflow case at 16x12x10 per block) after 3 DPLUR iterations: BV and
 k/omega scaled error <= 1e-11, species mass fractions within 1e-10
 absolute."""
 import ctypes as C
@@ -49,7 +49,7 @@ def test_register_inner_face_known_answers():
 
 
 def test_mesh_geometry_matches_oracle_and_neighbours():
-    case = cases.config1_split(mx=24, my=16, mz=8)
+    case = cases.warped_box_split(mx=24, my=16, mz=8)
     orc = O.OracleSimulation(case)
     g = case.ngg
     for b, blk in enumerate(case.blocks):
@@ -69,9 +69,9 @@ def test_mesh_geometry_matches_oracle_and_neighbours():
 
 
 @pytest.mark.gpu
-def test_config1_split_fp64_vs_oracle():
-    case = cases.config1_split(mx=32, my=24, mz=8)
-    print("[gpu-mb] config1 split 2 blocks FP64", flush=True)
+def test_warped_box_split_fp64_vs_oracle():
+    case = cases.warped_box_split(mx=32, my=24, mz=8)
+    print("[gpu-mb] warped_box split 2 blocks FP64", flush=True)
     pr = Pair(case, precision=L.FP64)
     pr.init()
     for b in range(2):
@@ -90,9 +90,9 @@ def test_config1_split_fp64_vs_oracle():
 
 
 @pytest.mark.gpu
-def test_config5_small_fp64_vs_oracle():
-    case = cases.config5(mx=16, my=12, mz=10, nblocks=4, steps=3, precision=L.FP64, cfl=0.5)
-    print("[gpu-mb] config5 4 blocks 16x12x10 FP64, SST + FR, 3 steps", flush=True)
+def test_jet_crossflow_small_fp64_vs_oracle():
+    case = cases.jet_crossflow(mx=16, my=12, mz=10, nblocks=4, steps=3, precision=L.FP64, cfl=0.5)
+    print("[gpu-mb] jet_crossflow 4 blocks 16x12x10 FP64, SST + FR, 3 steps", flush=True)
     pr = Pair(case, precision=L.FP64)
     pr.init()
     for b in range(4):
