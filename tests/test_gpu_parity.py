@@ -208,3 +208,21 @@ def test_warped_box_fp32_vs_fp64_trajectory():
     say("bv scaled err FP32 after 20 steps", e)
     assert e < 1e-4
     pr.close()
+
+
+def test_sst_plate_fp32_vs_fp64_trajectory():
+    """FP32 bulk SST + DPLUR over 5 steps of the fused time step (local time
+    step in the source kernel, dq0 in the first sweep) against the FP64
+    oracle: field error stays at FP32 level."""
+    case = cases.sst_plate(mx=48, my=32, mz=12, perturb=True)
+    say("sst_plate FP32 vs oracle 5 steps")
+    pr = Pair(case, precision=L.FP32)
+    pr.init()
+    for _ in range(5):
+        pr.gpu.step()
+        pr.orc.step()
+    for f in ("BV", "SV"):
+        e = pr.err(f)
+        say("scaled err FP32 after 5 steps", f, e)
+        assert e < 1e-4, (f, e)
+    pr.close()
