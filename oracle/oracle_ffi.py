@@ -7,6 +7,7 @@ two layouts agree field by field.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import math
 import os
@@ -114,65 +115,82 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        lib = C.CDLL(LIB_PATH)
-        P, I, D = C.c_void_p, C.c_int, C.c_double
-        Z = C.POINTER(OrcZone)
-        Pp = C.POINTER(OrcParam)
-        sig = {
-            "orc_ghost_grid": (None, [I, I, I, I, _DP, _DP, _DP]),
-            "orc_jac_metric": (I, [I, I, I, I, _DP, _DP, _DP, _DP, _DP, C.POINTER(I)]),
-            "orc_trim_ghost": (None, [I, I, I, I, _DP, _DP]),
-            "orc_inner_ghost_grid": (None, [C.POINTER(OrcInnerFace), I, I, I, I, _DP, _DP, _DP, I, I, I, _DP, _DP,
-                                            _DP]),
-            "orc_enthalpy": (None, [D, _DP, Pp]),
-            "orc_enthalpy_and_cp": (None, [D, _DP, _DP, Pp]),
-            "orc_cp": (None, [D, _DP, Pp]),
-            "orc_gibbs_div_rt": (None, [D, Pp, _DP]),
-            "orc_sutherland": (D, [D]),
-            "orc_transport_point": (None, [Pp, D, D, _DP, _DP, _DP, _DP, _DP]),
-            "orc_mixture_viscosity": (D, [Pp, D, D, _DP]),
-            "orc_compute_cv_from_bv": (None, [Z, Pp]),
-            "orc_update_physical_properties": (None, [Z, Pp]),
-            "orc_initialize_mut": (None, [Z, Pp]),
-            "orc_compute_source": (None, [Z, Pp]),
-            "orc_inviscid_flux": (None, [Z, Pp]),
-            "orc_inviscid_flux_dir": (None, [Z, Pp, I]),
-            "orc_ausm_face": (None, [Pp, _DP, _DP, _DP, D, D, _DP]),
-            "orc_viscous_flux": (None, [Z, Pp]),
-            "orc_local_time_step": (None, [Z, Pp]),
-            "orc_implicit_treatment": (None, [Z, Pp]),
-            "orc_jac_times_dq_point": (None, [Z, Pp, I, I, I, I, D, _DP]),
-            "orc_update_cv_and_bv": (I, [Z, Pp]),
-            "orc_limit_flow": (I, [Z, Pp]),
-            "orc_apply_bc": (None, [Z, Pp, C.POINTER(OrcBC)]),
-            "orc_inner_communication": (None, [Z, I, C.POINTER(OrcInnerFace), Pp]),
-            "orc_store_last_step": (None, [Z]),
-            "orc_square_of_dbv": (None, [Z]),
-            "orc_sum_dbv": (None, [Z, _DP]),
-            "orc_wall_distance": (None, [Z, _DP, I]),
-            "orc_steady_step": (D, [Z, I, Pp, C.POINTER(OrcBC), C.POINTER(OrcInnerFace), C.POINTER(I), I, I, _DP,
-                                    C.c_long, C.POINTER(I)]),
-            "orc_chem_source_point": (None, [Pp, D, D, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
-            "orc_solve_lu": (None, [_DP, _DP, I]),
-            "orc_plain_chem_step": (None, [Pp, D, D, _DP, D, _DP]),
-            "orc_plain_chem_batch": (None, [Pp, C.c_long, _DP, _DP, _DP, _DP, _DP]),
-            "orc_newton_temperature": (I, [Pp, _DP, D, D, _DP]),
-            "orc_omp_threads": (I, []),
-            "orc_sur_n_params": (I, [I]),
-            "orc_bf16": (D, [D]),
-            "orc_sur_features": (None, [C.POINTER(OrcSurDesc), D, D, _DP, D, C.POINTER(C.c_float)]),
-            "orc_sur_forward": (None, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), I, C.c_long, _DP, _DP, _DP,
-                                       _DP, _DP]),
-            "orc_sur_loss_grad": (D, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), I, C.c_long, _DP, _DP, _DP,
-                                      _DP, C.POINTER(C.c_float), _DP]),
-            "orc_sur_decode_check": (I, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), D, D, _DP, D, _DP]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = lib
+        _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
+
+
+@contextlib.contextmanager
+def use_library(path: str):
+    """Bind load() to another build of the oracle for the duration of the
+    block (the planted-mistake tests build mutated copies of the oracle)."""
+    global _lib
+    saved = _lib
+    try:
+        _lib = _bind(C.CDLL(path))
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def _bind(lib):
+    """Declare the signatures of the oracle's entry points on a loaded library."""
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    Z = C.POINTER(OrcZone)
+    Pp = C.POINTER(OrcParam)
+    sig = {
+        "orc_ghost_grid": (None, [I, I, I, I, _DP, _DP, _DP]),
+        "orc_jac_metric": (I, [I, I, I, I, _DP, _DP, _DP, _DP, _DP, C.POINTER(I)]),
+        "orc_trim_ghost": (None, [I, I, I, I, _DP, _DP]),
+        "orc_inner_ghost_grid": (None, [C.POINTER(OrcInnerFace), I, I, I, I, _DP, _DP, _DP, I, I, I, _DP, _DP,
+                                        _DP]),
+        "orc_enthalpy": (None, [D, _DP, Pp]),
+        "orc_enthalpy_and_cp": (None, [D, _DP, _DP, Pp]),
+        "orc_cp": (None, [D, _DP, Pp]),
+        "orc_gibbs_div_rt": (None, [D, Pp, _DP]),
+        "orc_sutherland": (D, [D]),
+        "orc_transport_point": (None, [Pp, D, D, _DP, _DP, _DP, _DP, _DP]),
+        "orc_mixture_viscosity": (D, [Pp, D, D, _DP]),
+        "orc_compute_cv_from_bv": (None, [Z, Pp]),
+        "orc_update_physical_properties": (None, [Z, Pp]),
+        "orc_initialize_mut": (None, [Z, Pp]),
+        "orc_compute_source": (None, [Z, Pp]),
+        "orc_inviscid_flux": (None, [Z, Pp]),
+        "orc_inviscid_flux_dir": (None, [Z, Pp, I]),
+        "orc_ausm_face": (None, [Pp, _DP, _DP, _DP, D, D, _DP]),
+        "orc_viscous_flux": (None, [Z, Pp]),
+        "orc_local_time_step": (None, [Z, Pp]),
+        "orc_implicit_treatment": (None, [Z, Pp]),
+        "orc_jac_times_dq_point": (None, [Z, Pp, I, I, I, I, D, _DP]),
+        "orc_update_cv_and_bv": (I, [Z, Pp]),
+        "orc_limit_flow": (I, [Z, Pp]),
+        "orc_apply_bc": (None, [Z, Pp, C.POINTER(OrcBC)]),
+        "orc_inner_communication": (None, [Z, I, C.POINTER(OrcInnerFace), Pp]),
+        "orc_store_last_step": (None, [Z]),
+        "orc_square_of_dbv": (None, [Z]),
+        "orc_sum_dbv": (None, [Z, _DP]),
+        "orc_wall_distance": (None, [Z, _DP, I]),
+        "orc_steady_step": (D, [Z, I, Pp, C.POINTER(OrcBC), C.POINTER(OrcInnerFace), C.POINTER(I), I, I, _DP,
+                                C.c_long, C.POINTER(I)]),
+        "orc_chem_source_point": (None, [Pp, D, D, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
+        "orc_solve_lu": (None, [_DP, _DP, I]),
+        "orc_plain_chem_step": (None, [Pp, D, D, _DP, D, _DP]),
+        "orc_plain_chem_batch": (None, [Pp, C.c_long, _DP, _DP, _DP, _DP, _DP]),
+        "orc_newton_temperature": (I, [Pp, _DP, D, D, _DP]),
+        "orc_omp_threads": (I, []),
+        "orc_sur_n_params": (I, [I]),
+        "orc_bf16": (D, [D]),
+        "orc_sur_features": (None, [C.POINTER(OrcSurDesc), D, D, _DP, D, C.POINTER(C.c_float)]),
+        "orc_sur_forward": (None, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), I, C.c_long, _DP, _DP, _DP,
+                                   _DP, _DP]),
+        "orc_sur_loss_grad": (D, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), I, C.c_long, _DP, _DP, _DP,
+                                  _DP, C.POINTER(C.c_float), _DP]),
+        "orc_sur_decode_check": (I, [C.POINTER(OrcSurDesc), C.POINTER(C.c_float), D, D, _DP, D, _DP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
 
 
 def dp(a: np.ndarray):
