@@ -95,6 +95,9 @@ struct ostc_zone {
   int *lf_count = nullptr;
   int64_t *lf_idx = nullptr;
   void *lf_vals = nullptr;
+  // repair-list slot (0/1) of the next checking update in ostc_steady_step:
+  // the zone alternates it itself, so callers may repeat or skip step numbers
+  int lf_slot = 0;
   ostc_bc bc{};
   bool has_bc = false;
   ostc_inflow *d_inflow = nullptr;
@@ -149,6 +152,7 @@ ZoneView<R> make_view(ostc_zone *z) {
   v.lf_count = z->lf_count;
   v.lf_idx = z->lf_idx;
   v.lf_vals = static_cast<R *>(z->lf_vals);
+  v.lf_cap = (int)((int64_t)z->desc.mx * z->desc.my * z->desc.mz);
   return v;
 }
 
@@ -600,22 +604,24 @@ int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_fac
         if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
         if (r) return r;
         prof_mark(P_UPDATE, st, true);
-        r = update_cv_bv(z->vd, c->dev, o, res, st, step & 1);
+        r = update_cv_bv(z->vd, c->dev, o, res, st, z->lf_slot);
       } else {
         float *res = nullptr;
         r = implicit_treatment(z->vf, c->dev, o, st, &res);
         if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
         if (r) return r;
         prof_mark(P_UPDATE, st, true);
-        r = update_cv_bv(z->vf, c->dev, o, res, st, step & 1);
+        r = update_cv_bv(z->vf, c->dev, o, res, st, z->lf_slot);
       }
       prof_mark(P_UPDATE, st, false);
     }
     if (r) return r;
     prof_mark(P_LIMIT, st, true);
-    r = ZONE_CALL(z, limit_flow(z->vf, c->dev, o, st, step & 1), limit_flow(z->vd, c->dev, o, st, step & 1));
+    r = ZONE_CALL(z, limit_flow(z->vf, c->dev, o, st, z->lf_slot), limit_flow(z->vd, c->dev, o, st, z->lf_slot));
     prof_mark(P_LIMIT, st, false);
     if (r) return r;
+    // the update cleared the other slot for the next step's list
+    z->lf_slot ^= 1;
     prof_mark(P_BC, st, true);
     r = ostc_apply_bc(c, z, s);
     prof_mark(P_BC, st, false);

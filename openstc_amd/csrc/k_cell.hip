@@ -811,7 +811,10 @@ __global__ __launch_bounds__(256) void k_update(ZoneView<R> z, const ostc_param 
       const R var[5] = {rho, u, v, w, pr};
       bool bad = limit_bad_flow<R, NS, NT>(p, var);
       if constexpr (NT > 0) bad = bad || limit_bad_turb<R>(p, cv[5 + NS] * rinv, cv[6 + NS] * rinv);
-      if (bad) z.lf_idx[atomicAdd(&z.lf_count[slot], 1)] = id;
+      if (bad) {
+        const int e = atomicAdd(&z.lf_count[slot], 1);
+        if (e < z.lf_cap) z.lf_idx[e] = id;
+      }
     }
   } else {
     const R pr = (R)(kGammaAir - 1) * (cv[4] - (R)0.5 * rho * vel2);
@@ -821,7 +824,10 @@ __global__ __launch_bounds__(256) void k_update(ZoneView<R> z, const ostc_param 
       const R var[5] = {rho, u, v, w, pr};
       bool bad = limit_bad_flow<R, NS, NT>(p, var);
       if constexpr (NT > 0) bad = bad || limit_bad_turb<R>(p, cv[5 + NS] * rinv, cv[6 + NS] * rinv);
-      if (bad) z.lf_idx[atomicAdd(&z.lf_count[slot], 1)] = id;
+      if (bad) {
+        const int e = atomicAdd(&z.lf_count[slot], 1);
+        if (e < z.lf_cap) z.lf_idx[e] = id;
+      }
     }
   }
   z.vel[id] = sqrt(vel2);
@@ -989,6 +995,7 @@ __global__ __launch_bounds__(256) void k_limit_check(ZoneView<R> z, const ostc_p
   R vals[NVAL];
   if (!limit_fix_cell<R, NS, NT>(z, p, c.i, c.j, c.k, vals)) return;
   const int slot = atomicAdd(&z.lf_count[2], 1);
+  if (slot >= z.lf_cap) return;
   z.lf_idx[slot] = z.idx(c.i, c.j, c.k);
   R *out = z.lf_vals + (int64_t)slot * NVAL;
 #pragma unroll
@@ -996,17 +1003,22 @@ __global__ __launch_bounds__(256) void k_limit_check(ZoneView<R> z, const ostc_p
 }
 
 // pass 1 after a checking update: repair the listed cells (the update has
-// already classified every cell)
+// already classified every cell).  An entry whose cell needs no repair after
+// all (never, while the update and this kernel classify the same stored
+// values) is marked -1 and skipped by the write-back.
 template <typename R, int NS, int NT>
 __global__ __launch_bounds__(256) void k_limit_repair(ZoneView<R> z, const ostc_param *__restrict__ p, int slot) {
   constexpr int NSC = NS + NT, NV = 5 + NSC, NVAL = 6 + NSC + NV;
-  const int n = z.lf_count[slot];
+  const int n = min(z.lf_count[slot], z.lf_cap);
   const int64_t nx = z.nx, ny = z.ny;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int64_t id = z.lf_idx[s];
     const int i = (int)(id % nx) - z.g, j = (int)((id / nx) % ny) - z.g, k = (int)(id / (nx * ny)) - z.g;
     R vals[NVAL];
-    limit_fix_cell<R, NS, NT>(z, p, i, j, k, vals);
+    if (!limit_fix_cell<R, NS, NT>(z, p, i, j, k, vals)) {
+      z.lf_idx[s] = -1;
+      continue;
+    }
     R *out = z.lf_vals + (int64_t)s * NVAL;
 #pragma unroll
     for (int l = 0; l < NVAL; ++l) out[l] = vals[l];
@@ -1017,10 +1029,11 @@ __global__ __launch_bounds__(256) void k_limit_repair(ZoneView<R> z, const ostc_
 template <typename R, int NS, int NT>
 __global__ __launch_bounds__(256) void k_limit_apply(ZoneView<R> z, int slot) {
   constexpr int NSC = NS + NT, NV = 5 + NSC, NVAL = 6 + NSC + NV;
-  const int n = z.lf_count[slot];
+  const int n = min(z.lf_count[slot], z.lf_cap);
   const int64_t P = z.plane;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const int64_t id = z.lf_idx[s];
+    if (id < 0) continue;
     const R *v = z.lf_vals + (int64_t)s * NVAL;
 #pragma unroll
     for (int l = 0; l < 6; ++l) z.bv[l * P + id] = v[l];

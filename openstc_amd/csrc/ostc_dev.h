@@ -64,10 +64,13 @@ struct ZoneView {
   double *chem_lu;    // ns*ns planes: LU factors of the DPLUR chemistry block
   int *chem_piv;      // ns planes
   unsigned long long *counters;  // [0] newton cap hits, [1] limit_flow repairs, [2] surrogate fallbacks
-  // limit_flow repair list: count, cell index and final (bv, sv, cv) per entry
+  // limit_flow repair list: count, cell index and final (bv, sv, cv) per
+  // entry; lf_cap entries (one per interior cell); an index of -1 marks an
+  // entry whose cell turned out to need no repair
   int *lf_count;
   int64_t *lf_idx;
   R *lf_vals;
+  int lf_cap;
 
   __host__ __device__ __forceinline__ int64_t idx(int i, int j, int k) const {
     return ((int64_t)(k + g) * ny + (j + g)) * nx + (i + g);

@@ -1,0 +1,141 @@
+"""Whole-trajectory parity of the HIP path against the FP64 oracle on the
+benchmark configurations, including the FP32-bulk zones as they are timed:
+
+  * the warped Euler box at its stated size (64x64x8, MUSCL + minmod +
+    AUSM+, explicit, FP64) for its stated 100 steps, residual history
+    included, and the same run with FP32 bulk storage;
+  * the SST flat plate (48x32x12, DPLUR x4, CFL 5) in FP32 bulk, run by the
+    GPU and by the oracle until the residual has dropped four decades
+    (1000 iterations), converged fields compared;
+  * the finite-rate mixing layer (28x20x10, 9 species / 19 reactions, EPI,
+    DPLUR x4) in FP32 bulk + FP64 chemistry/Newton for 20 iterations;
+  * the four-block jet in crossflow (4 x 16x12x10, SST + finite rate,
+    inner-face halos) in FP32 bulk for 10 iterations.
+
+Errors: BV and k/omega as max|gpu-oracle| / max|oracle| per plane (the
+scaled error of the other parity tests); species as the absolute difference
+of mass fractions (trace radicals sit near round-off levels, where a
+difference scaled by a plane's own maximum measures nothing physical).
+"""
+import numpy as np
+import pytest
+
+from harness import Pair, interior, scaled_err
+from openstc_amd import _lib as L
+from openstc_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-traj]", *a, flush=True)
+
+
+def field_errors(pr: Pair, b=0, halo=0):
+    ns = pr.case.param.n_spec
+    g = pr.case.ngg
+    out = {}
+    a = interior(pr.zg(b).download("BV"), g, halo)
+    o = interior(pr.oarr("BV", b), g, halo)
+    for l, name in enumerate(("rho", "u", "v", "w", "p", "T")):
+        out[name] = scaled_err(a[l:l + 1], o[l:l + 1])
+    if pr.case.param.n_scalar:
+        a = interior(pr.zg(b).download("SV"), g, halo)
+        o = interior(pr.oarr("SV", b), g, halo)
+        if ns:
+            out["Y(abs)"] = float(np.abs(a[:ns] - o[:ns]).max())
+        for l, name in ((ns, "k"), (ns + 1, "omega")):
+            if l < a.shape[0]:
+                out[name] = scaled_err(a[l:l + 1], o[l:l + 1])
+    return out
+
+
+def check(errs, bounds, label):
+    say(label, {k: f"{v:.2e}" for k, v in errs.items()})
+    bad = {k: v for k, v in errs.items() if not v <= bounds.get(k, bounds["*"])}
+    assert not bad, (label, bad)
+
+
+def test_warped_box_stated_size_fp64_100_steps():
+    """The warped Euler box exactly as stated: 64x64x8, FP64, 100 steps,
+    residual every 10 steps."""
+    case = cases.warped_box(mx=64, my=64, mz=8, steps=100)
+    say("warped_box 64x64x8 FP64, 100 steps")
+    pr = Pair(case, precision=L.FP64)
+    pr.init()
+    worst = 0.0
+    for s in range(100):
+        rg = pr.gpu.step(output_screen=10)
+        ro = pr.orc.step(output_screen=10)
+        if rg is not None:
+            worst = max(worst, abs(rg - ro) / max(abs(ro), 1e-300))
+    say("residual history max rel diff", worst)
+    assert worst < 1e-9
+    check(field_errors(pr, halo=2), {"*": 1e-9}, "64x64x8 FP64 after 100 steps")
+    pr.close()
+
+
+def test_warped_box_stated_size_fp32_100_steps():
+    case = cases.warped_box(mx=64, my=64, mz=8, steps=100)
+    say("warped_box 64x64x8 FP32 bulk vs FP64 oracle, 100 steps")
+    pr = Pair(case, precision=L.FP32)
+    pr.init()
+    for s in range(100):
+        pr.gpu.step()
+        pr.orc.step()
+    check(field_errors(pr), {"*": 1e-4}, "64x64x8 FP32 after 100 steps")
+    pr.close()
+
+
+def test_sst_plate_fp32_converged():
+    """FP32-bulk SST + DPLUR run until the residual has dropped four decades
+    (1000 iterations of the 48x32x12 plate; the oracle's residual passes
+    1e-4 at that point) and compared with the FP64 oracle there."""
+    case = cases.sst_plate(mx=48, my=32, mz=12, perturb=True)
+    say("sst_plate 48x32x12 FP32 vs FP64 oracle, 1000 iterations")
+    pr = Pair(case, precision=L.FP32)
+    pr.init()
+    rg = ro = None
+    for s in range(1, 1001):
+        a = pr.gpu.step(output_screen=100)
+        b = pr.orc.step(output_screen=100)
+        if a is not None:
+            rg, ro = a, b
+            if s % 200 == 0:
+                say(f"step {s}: residual gpu {rg:.3e} oracle {ro:.3e}")
+    assert ro <= 1.2e-4 and rg <= 1.2e-4, (rg, ro)
+    check(field_errors(pr), {"*": 1e-4}, "SST plate FP32 converged")
+    pr.close()
+
+
+def test_mixing_layer_fp32_20_iterations():
+    """The finite-rate mixing layer in FP32 bulk with FP64 chemistry and
+    Newton (the precision the finite-rate workload is timed in) for 20
+    DPLUR iterations against the FP64 oracle."""
+    case = cases.mixing_layer(mx=28, my=20, mz=10, cfl=1.0)
+    say("mixing_layer 28x20x10 FP32 bulk + FP64 chemistry vs FP64 oracle, 20 iterations")
+    pr = Pair(case, precision=L.FP32)
+    pr.init()
+    for s in range(1, 21):
+        pr.gpu.step()
+        pr.orc.step()
+        if s in (1, 5, 10, 20):
+            errs = field_errors(pr)
+            say(f"iteration {s}", {k: f"{v:.2e}" for k, v in errs.items()})
+    check(field_errors(pr), {"*": 1e-3}, "mixing layer FP32 after 20 iterations")
+    pr.close()
+
+
+def test_jet_crossflow_fp32_10_iterations():
+    """The four-block SST + finite-rate jet (inner-face halos) in FP32 bulk
+    for 10 iterations against the FP64 oracle."""
+    case = cases.jet_crossflow(mx=16, my=12, mz=10, nblocks=4, steps=10, precision=L.FP32, cfl=1.0)
+    say("jet_crossflow 4 x 16x12x10 FP32 bulk vs FP64 oracle, 10 iterations")
+    pr = Pair(case, precision=L.FP32)
+    pr.init()
+    for s in range(1, 11):
+        pr.gpu.step()
+        pr.orc.step()
+    for b in range(4):
+        check(field_errors(pr, b=b), {"*": 1e-3}, f"jet block {b} FP32 after 10 iterations")
+    pr.close()
