@@ -156,12 +156,12 @@ int ostc_sizeof_bc(void);
 int ostc_block_geometry(int mx, int my, int mz, int ngg, const double *x_in, const double *y_in, const double *z_in,
                         double *x_out, double *y_out, double *z_out, double *jac, double *metric, int *n_neg_jac);
 
-/* InnerFace ctor + register_boundary (src/Mesh.cpp:34-178), 3-D: raw[13] is
- * one record of the reference's inner{rank}.txt read by
+/* InnerFace ctor + register_boundary(ngg, dim) (src/Mesh.cpp:34-178): raw[13]
+ * is one record of the reference's inner{rank}.txt read by
  * Mesh::read_inner_interface (src/Mesh.cpp:709-735): the 1-based ranges
  * i1 i2 j1 j2 k1 k2 of this block, those of the target block and the 1-based
- * target block id. */
-int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_face *out);
+ * target block id.  dim = 3, or 2 for a mesh of 2-D blocks (mz = 1). */
+int ostc_register_inner_face(const int raw[13], int ngg, int dim, ostc_inner_face *out);
 
 /* Geometry of a multi-block mesh (host, FP64): per block ghost grids
  * (Mesh::init_ghost_grid, src/Mesh.cpp:789-943), then the ghost layers across
@@ -319,6 +319,12 @@ int ostc_apply_bc(ostc_ctx *ctx, ostc_zone *zone, void *stream);
 int ostc_inner_halo(ostc_ctx *ctx, ostc_zone **zones, int blk, const ostc_inner_face *face, void *stream);
 /* update_physical_properties<M,T> (src/FieldOperation.cuh:75-109) */
 int ostc_update_physical_properties(ostc_ctx *ctx, ostc_zone *zone, void *stream);
+/* eliminate_k_gradient (src/FieldOperation.cu:42-62): a 2-D block (mz = 1)
+ * copies its k = 0 plane of bv and sv into the k ghost layers; no-op for a
+ * 3-D block.  ostc_steady_step calls it after the inner-face halos, as the
+ * reference's loop does (src/Driver.cu:259-265); a driver calls it once
+ * before the first boundary conditions (src/Driver.cu:59-65). */
+int ostc_eliminate_k_gradient(ostc_ctx *ctx, ostc_zone *zone, void *stream);
 /* store_last_step (src/TimeAdvanceFunc.cu:5-16) */
 int ostc_store_last_step(ostc_ctx *ctx, ostc_zone *zone, void *stream);
 /* compute_square_of_dbv + reduction_of_dv_squared (src/TimeAdvanceFunc.cu:

@@ -170,6 +170,7 @@ SIGNATURES = {
     "ostc_update_physical_properties": (C.c_int, [_P, _P, _P]),
     "ostc_store_last_step": (C.c_int, [_P, _P, _P]),
     "ostc_residual_sums": (C.c_int, [_P, C.POINTER(_P), C.c_int, _D, _P]),
+    "ostc_eliminate_k_gradient": (C.c_int, [_P, _P, _P]),
     "ostc_zone_counters": (C.c_int, [_P, C.POINTER(C.c_longlong), C.c_int, _P]),
     "ostc_steady_step": (C.c_int, [_P, C.POINTER(_P), C.c_int, C.POINTER(InnerFace), _I, C.c_int, C.c_int, _D,
                                    _I, _P]),
@@ -194,7 +195,7 @@ SIGNATURES = {
     "ostc_memcpy_h2d": (C.c_int, [_P, _P, C.c_long]),
     "ostc_memcpy_d2h": (C.c_int, [_P, _P, C.c_long]),
     "ostc_stream_copy": (C.c_int, [_P, _P, C.c_long, _P]),
-    "ostc_register_inner_face": (C.c_int, [C.POINTER(C.c_int), C.c_int, C.POINTER(InnerFace)]),
+    "ostc_register_inner_face": (C.c_int, [C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(InnerFace)]),
     "ostc_mesh_geometry": (C.c_int, [C.c_int, C.c_int, C.POINTER(BlockCoords), C.POINTER(InnerFace), _I]),
     "ostc_read_chemkin": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(Param),
                                     C.POINTER(MechanismInfo)]),

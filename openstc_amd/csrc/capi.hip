@@ -256,7 +256,8 @@ int ostc_ctx_destroy(ostc_ctx *ctx) {
 
 int ostc_zone_create(ostc_ctx *ctx, const ostc_zone_desc *d, ostc_zone **out) {
   if (!ctx || !d || !out) return fail(OSTC_ERR_ARG, "null argument");
-  if (d->mx < 2 || d->my < 2 || d->mz < 2) return fail(OSTC_ERR_ARG, "3-D blocks with mx,my,mz >= 2 only");
+  if (d->mx < 2 || d->my < 2 || d->mz < 1)
+    return fail(OSTC_ERR_ARG, "blocks need mx, my >= 2 and mz >= 1 (mz = 1: a 2-D block)");
   if (d->ngg != 2) return fail(OSTC_ERR_ARG, "ngg must be 2 (MUSCL/NND-2 stencils)");
   if (d->precision != OSTC_FP32 && d->precision != OSTC_FP64) return fail(OSTC_ERR_ARG, "precision 4 or 8");
   auto *z = new (std::nothrow) ostc_zone;
@@ -516,6 +517,11 @@ int ostc_update_physical_properties(ostc_ctx *c, ostc_zone *z, void *s) {
                    physical_properties(z->vd, c->dev, c->opts, S(s)));
 }
 
+int ostc_eliminate_k_gradient(ostc_ctx *c, ostc_zone *z, void *s) {
+  if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
+  return ZONE_CALL(z, eliminate_k_gradient(z->vf, c->opts, S(s)), eliminate_k_gradient(z->vd, c->opts, S(s)));
+}
+
 int ostc_store_last_step(ostc_ctx *c, ostc_zone *z, void *s) {
   if (!c || !z) return fail(OSTC_ERR_ARG, "null argument");
   return ZONE_CALL(z, store_last_step(z->vf, S(s)), store_last_step(z->vd, S(s)));
@@ -637,6 +643,10 @@ int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_fac
     }
     prof_mark(P_HALO, st, false);
   }
+  // 2-D blocks: k ghost layers from the k = 0 plane (src/Driver.cu:259-265)
+  for (int b = 0; b < n; ++b)
+    if (zones[b]->desc.mz == 1)
+      if (int r = ostc_eliminate_k_gradient(c, zones[b], s)) return r;
   prof_mark(P_PROPS, st, true);
   for (int b = 0; b < n; ++b)
     if (int r = ostc_update_physical_properties(c, zones[b], s)) return r;

@@ -466,7 +466,7 @@ int read_inner(ostc_case &c, const std::string &path) {
       std::array<int, 13> raw{a[0], a[1], a[2], a[3], a[4], a[5], t[0], t[1], t[2], t[3], t[4], t[5], t[6]};
       c.blocks[b].inner_raw.push_back(raw);
       ostc_inner_face face{};
-      const int st = ostc_register_inner_face(raw.data(), c.ngg, &face);
+      const int st = ostc_register_inner_face(raw.data(), c.ngg, c.dimension, &face);
       if (st != OSTC_OK) return case_fail(path + ": inner face cannot be registered: " + l1);
       if (face.target_block < 0 || face.target_block >= (int)c.blocks.size())
         return case_fail(path + ": inner face targets a missing block");
@@ -476,22 +476,27 @@ int read_inner(ostc_case &c, const std::string &path) {
   return OSTC_OK;
 }
 
-// Boundary::register_boundary (src/Mesh.cpp:14-32), 3-D
-void register_face(const std::array<int, 7> &rec, int ngg, int rs[3], int re[3], int *face, int *dir) {
+// Boundary::register_boundary (src/Mesh.cpp:14-32); a 2-D face spans the
+// k ghost layers [-ngg, ngg]
+void register_face(const std::array<int, 7> &rec, int ngg, int dim, int rs[3], int re[3], int *face, int *dir) {
   rs[0] = rec[0], re[0] = rec[1], rs[1] = rec[2], re[1] = rec[3], rs[2] = rec[4], re[2] = rec[5];
   *face = 0;
   *dir = 1;
-  for (int q = 0; q < 3; ++q)
+  for (int q = 0; q < dim; ++q)
     if (rs[q] == re[q]) {
       *face = q;
       if (rs[q] == 0) *dir = -1;
       break;
     }
-  for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < dim; ++q) {
     if (q == *face) continue;
     if (re[q] < rs[q]) std::swap(rs[q], re[q]);
     rs[q] -= ngg;
     re[q] += ngg;
+  }
+  if (dim == 2) {
+    rs[2] = -ngg;
+    re[2] = ngg;
   }
 }
 
@@ -582,7 +587,7 @@ int boundary_programs(CaseCtx &cx) {
           if (rec[6] != idx[t][slot]) continue;
           if (n >= OSTC_MAX_FACE) return case_fail("too many boundary faces in one block");
           ostc_face &f = bc.face[n++];
-          register_face(rec, c.ngg, f.range_start, f.range_end, &f.face, &f.direction);
+          register_face(rec, c.ngg, c.dimension, f.range_start, f.range_end, &f.face, &f.direction);
           if (t == T_WALL)
             for (int q = 0; q < 3; ++q) {
               if (q == f.face) continue;
@@ -610,7 +615,7 @@ int boundary_programs(CaseCtx &cx) {
       for (auto &rec : blk.boundary) {
         if (rec[6] != lab) continue;
         int rs[3], re[3], face, dir;
-        register_face(rec, c.ngg, rs, re, &face, &dir);
+        register_face(rec, c.ngg, c.dimension, rs, re, &face, &dir);
         for (int q = 0; q < 3; ++q)
           if (q != face) rs[q] += c.ngg, re[q] -= c.ngg;
         for (int k = rs[2]; k <= re[2]; ++k)
@@ -751,7 +756,7 @@ int read_case(const char *dir_c, ostc_case *c) {
   if (!c->steady) return case_fail("steady = 0: only the steady time loop is on this path");
   if (c->initial != 0 && c->initial != 1)
     return case_fail("initial must be 0 (from the setup's inflows) or 1 (restart from output/field/flowfield.plt)");
-  if (c->dimension != 3) return case_fail("dimension must be 3");
+  if (c->dimension != 2 && c->dimension != 3) return case_fail("dimension must be 2 or 3");
   if (p.inviscid_scheme / 10 == 5) return case_fail("WENO schemes are not on this path");
   if (p.inviscid_scheme != 3) return case_fail("inviscid_scheme must be 3 (AUSM+)");
   if (p.reconstruction < 1 || p.reconstruction > 3) return case_fail("reconstruction must be 1, 2 or 3");
@@ -783,8 +788,13 @@ int read_case(const char *dir_c, ostc_case *c) {
   // mesh
   const std::string bcdir = in + "/boundary_condition";
   if (int e = read_grid(*c, rank_file(in + "/grid", "grid", "grd"))) return e;
-  for (auto &blk : c->blocks)
-    if (blk.mz < 2) return case_fail("two-dimensional blocks (mz = 1) are not on this path");
+  // the grid decides: a first block with mz = 1 makes the mesh 2-D
+  // (src/Mesh.cpp:611-613); every block must then be one k plane
+  if (c->blocks[0].mz == 1) c->dimension = 2;
+  for (auto &blk : c->blocks) {
+    if (c->dimension == 2 && blk.mz != 1) return case_fail("dimension = 2 (a 2-D mesh) needs mz = 1 in every block");
+    if (c->dimension == 3 && blk.mz < 3) return case_fail("3-D blocks need mz >= 3");
+  }
   if (std::abs(c->grid_scale - 1.0) > 1e-15)
     for (auto &blk : c->blocks)
       for (auto *arr : {&blk.x, &blk.y, &blk.z})

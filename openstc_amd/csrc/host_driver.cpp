@@ -144,6 +144,8 @@ int ostc_driver_initialize(ostc_driver *d) {
     for (auto *z : d->zones)
       DRV_TRY(ostc_wall_distance(d->ctx, z, n ? d->wall_xyz.data() : nullptr, n, nullptr), "wall distance");
   }
+  // 2-D blocks: k ghost layers from the k = 0 plane first (src/Driver.cu:59-65)
+  for (auto *z : d->zones) DRV_TRY(ostc_eliminate_k_gradient(d->ctx, z, nullptr), "eliminate_k_gradient");
   for (auto *z : d->zones) DRV_TRY(ostc_apply_bc(d->ctx, z, nullptr), "apply_bc");
   for (auto *z : d->zones) {
     DRV_TRY(ostc_compute_cv_from_bv(d->ctx, z, nullptr), "compute_cv_from_bv");

@@ -49,6 +49,22 @@ void ghost_faces(int mx, int my, int mz, int ngg, double *x, double *y, double *
       for (int j = -1; j >= -ngg - 1; --j) reflect(g, i, j, k, i, 0, k, i, -j, k);
       for (int j = my; j < my + ngg + 1; ++j) reflect(g, i, j, k, i, my - 1, k, i, 2 * (my - 1) - j, k);
     }
+  if (mz == 1) {
+    // a 2-D block: the k ghost layers repeat x, y and step z by one per
+    // layer (src/Mesh.cpp:840-856)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        const int64_t s = g.at(i, j, 0);
+        for (int k = -ngg - 1; k <= ngg + 1; ++k) {
+          if (k == 0) continue;
+          const int64_t d = g.at(i, j, k);
+          g.x[d] = g.x[s];
+          g.y[d] = g.y[s];
+          g.z[d] = g.z[s] + k;
+        }
+      }
+    return;
+  }
   for (int j = 0; j < my; ++j)
     for (int i = 0; i < mx; ++i) {
       for (int k = -1; k >= -ngg - 1; --k) reflect(g, i, j, k, i, j, 0, i, j, -k);
@@ -140,9 +156,12 @@ int jac_metric(int mx, int my, int mz, int ngg, const double *x, const double *y
   Metric M{mx, my, mz, ngg + 1, g, x, y, z, jac, metric,
            (int64_t)(mx + 2 * g) * (my + 2 * g) * (mz + 2 * g)};
   int neg = 0;
+  // a 2-D block (mz = 1) has its one k plane in the interior pass
+  // (src/Mesh.cpp:263-268)
+  const int k_min = mz == 1 ? 0 : 1, k_max = mz == 1 ? 1 : mz - 1;
   for (int i = 1; i < mx - 1; ++i)
     for (int j = 1; j < my - 1; ++j)
-      for (int k = 1; k < mz - 1; ++k)
+      for (int k = k_min; k < k_max; ++k)
         if (!M.point(i, j, k)) return -1;
   // ghost slabs in the reference's order; a bad ghost copies its inner neighbour
   auto slab = [&](int i0, int i1, int di, int j0, int j1, int dj, int k0, int k1, int dk, int si, int sj, int sk,
@@ -172,7 +191,8 @@ int jac_metric(int mx, int my, int mz, int ngg, const double *x, const double *y
 extern "C" int ostc_block_geometry(int mx, int my, int mz, int ngg, const double *x_in, const double *y_in,
                                    const double *z_in, double *x_out, double *y_out, double *z_out, double *jac,
                                    double *metric, int *n_neg_jac) {
-  if (mx < 3 || my < 3 || mz < 3 || ngg < 1 || !x_in || !y_in || !z_in || !jac || !metric) return OSTC_ERR_ARG;
+  if (mx < 3 || my < 3 || (mz < 3 && mz != 1) || ngg < 1 || !x_in || !y_in || !z_in || !jac || !metric)
+    return OSTC_ERR_ARG;
   const int G = ngg + 1;
   const int64_t NX = mx + 2 * G, NY = my + 2 * G, NZ = mz + 2 * G;
   std::vector<double> X(NX * NY * NZ, 0.0), Y(X.size(), 0.0), Z(X.size(), 0.0);
@@ -244,12 +264,17 @@ void inner_ghost(const ostc_inner_face &fc, Grid &B, const Grid &N) {
 
 }  // namespace
 
-// InnerFace::register_boundary (src/Mesh.cpp:44-178), 3-D: the raw record is
-// the reference's inner{rank}.txt pair of lines (1-based ranges; the range
-// with start == end is the face, a negative pair marks the second direction;
-// the target block id is 1-based).
-extern "C" int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_face *out) {
-  if (!raw || !out || ngg < 1) return OSTC_ERR_ARG;
+namespace {
+int finish_inner_face(ostc_inner_face f, int ngg, ostc_inner_face *out);
+}  // namespace
+
+// InnerFace::register_boundary (src/Mesh.cpp:44-178): the raw record is the
+// reference's inner{rank}.txt pair of lines (1-based ranges; the range with
+// start == end is the face; in 3-D a negative pair marks the second
+// direction, in 2-D the tangential direction is the other in-plane one and
+// k is the single plane 0; the target block id is 1-based).
+extern "C" int ostc_register_inner_face(const int raw[13], int ngg, int dim, ostc_inner_face *out) {
+  if (!raw || !out || ngg < 1 || (dim != 2 && dim != 3)) return OSTC_ERR_ARG;
   ostc_inner_face f{};
   for (int i = 0; i < 3; ++i) {
     f.range_start[i] = raw[2 * i];
@@ -262,7 +287,7 @@ extern "C" int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_f
   f.direction = 1;
   f.target_face = -1;
   f.target_direction = 1;
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < dim; ++i)
     if (f.range_start[i] == f.range_end[i]) {
       f.face = i;
       if (f.range_start[i] == 1) f.direction = -1;
@@ -270,7 +295,7 @@ extern "C" int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_f
       f.range_end[i] -= 1;
       break;
     }
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < dim; ++i)
     if (f.target_start[i] == f.target_end[i]) {
       f.target_face = i;
       if (f.target_start[i] == 1) f.target_direction = -1;
@@ -280,6 +305,21 @@ extern "C" int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_f
     }
   if (f.face < 0 || f.target_face < 0) return OSTC_ERR_ARG;
   f.src_tar[f.target_face] = -(f.direction * f.target_direction) * (f.face + 1);
+  if (dim == 2) {
+    const int next_face = 1 - f.face, target_next_face = 1 - f.target_face;
+    f.range_start[next_face] = (f.range_start[next_face] < 0 ? -f.range_start[next_face] : f.range_start[next_face]) - 1;
+    f.range_end[next_face] = (f.range_end[next_face] < 0 ? -f.range_end[next_face] : f.range_end[next_face]) - 1;
+    f.target_start[next_face] =
+        (f.target_start[next_face] < 0 ? -f.target_start[next_face] : f.target_start[next_face]) - 1;
+    f.target_end[next_face] = (f.target_end[next_face] < 0 ? -f.target_end[next_face] : f.target_end[next_face]) - 1;
+    f.src_tar[target_next_face] = sgn((f.target_end[target_next_face] - f.target_start[target_next_face]) *
+                                      (f.range_end[next_face] - f.range_start[next_face])) *
+                                  (next_face + 1);
+    f.src_tar[2] = 3;
+    f.range_start[2] = f.range_end[2] = 0;
+    f.target_start[2] = f.target_end[2] = 0;
+    return finish_inner_face(f, ngg, out);
+  }
   int next_face = -1, target_next_face = -1;
   for (int i = 0; i < 3; ++i)
     if (f.range_start[i] < 0) {
@@ -308,7 +348,13 @@ extern "C" int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_f
   f.src_tar[target_next_face] = sgn((f.target_end[target_next_face] - f.target_start[target_next_face]) *
                                     (f.range_end[next_face] - f.range_start[next_face])) *
                                 (next_face + 1);
-  // loop ranges on the ghost side (corners not communicated)
+  return finish_inner_face(f, ngg, out);
+}
+
+namespace {
+// loop ranges on the ghost side (corners not communicated), loop directions
+// and point counts (src/Mesh.cpp:117-177)
+int finish_inner_face(ostc_inner_face f, int ngg, ostc_inner_face *out) {
   f.range_start[f.face] = f.direction < 0 ? -ngg : f.range_end[f.face];
   f.range_end[f.face] = f.direction < 0 ? 0 : f.range_end[f.face] + ngg;
   const int sgn_tar = sgn(f.src_tar[f.target_face]);
@@ -328,10 +374,12 @@ extern "C" int ostc_register_inner_face(const int raw[13], int ngg, ostc_inner_f
   *out = f;
   return OSTC_OK;
 }
+}  // namespace
 
-// Geometry of a multi-block mesh: ghost grids by reflection, then the ghost
-// layers across inner faces copied from the neighbour blocks, then metrics
-// (src/Mesh.cpp:789-943, 940-985, 259-521, in that order).
+// Geometry of a multi-block mesh: ghost faces by reflection, then the ghost
+// layers across inner faces copied from the neighbour blocks, then edges and
+// corners by the parallelogram rule, then metrics (src/Mesh.cpp:789-862,
+// 945-993, 864-937, 259-521, in that order).
 extern "C" int ostc_mesh_geometry(int n_blocks, int ngg, ostc_block_coords *blocks, const ostc_inner_face *inner,
                                   const int *n_inner) {
   if (n_blocks < 1 || ngg < 1 || !blocks || (!inner && n_inner)) return OSTC_ERR_ARG;
@@ -340,7 +388,8 @@ extern "C" int ostc_mesh_geometry(int n_blocks, int ngg, ostc_block_coords *bloc
   std::vector<Grid> grid(n_blocks);
   for (int b = 0; b < n_blocks; ++b) {
     const ostc_block_coords &c = blocks[b];
-    if (c.mx < 3 || c.my < 3 || c.mz < 3 || !c.x || !c.y || !c.z || !c.jac || !c.metric) return OSTC_ERR_ARG;
+    if (c.mx < 3 || c.my < 3 || (c.mz < 3 && c.mz != 1) || !c.x || !c.y || !c.z || !c.jac || !c.metric)
+      return OSTC_ERR_ARG;
     const int64_t NX = c.mx + 2 * G, NY = c.my + 2 * G, NZ = c.mz + 2 * G;
     X[b].assign(NX * NY * NZ, 0.0);
     Y[b].assign(X[b].size(), 0.0);
@@ -355,8 +404,10 @@ extern "C" int ostc_mesh_geometry(int n_blocks, int ngg, ostc_block_coords *bloc
           Z[b][d] = c.z[s];
         }
     ostc_host::ghost_faces(c.mx, c.my, c.mz, ngg, X[b].data(), Y[b].data(), Z[b].data());
-    ostc_host::ghost_edges_corners(c.mx, c.my, c.mz, ngg, X[b].data(), Y[b].data(), Z[b].data());
   }
+  // the neighbours' points across inner faces come before the edges and
+  // corners, whose parallelogram rule then starts from them
+  // (src/Mesh.cpp:859-862)
   if (n_inner) {
     int off = 0;
     for (int b = 0; b < n_blocks; ++b) {
@@ -368,6 +419,9 @@ extern "C" int ostc_mesh_geometry(int n_blocks, int ngg, ostc_block_coords *bloc
       off += n_inner[b];
     }
   }
+  for (int b = 0; b < n_blocks; ++b)
+    ostc_host::ghost_edges_corners(blocks[b].mx, blocks[b].my, blocks[b].mz, ngg, X[b].data(), Y[b].data(),
+                                   Z[b].data());
   for (int b = 0; b < n_blocks; ++b) {
     ostc_block_coords &c = blocks[b];
     if (ostc_host::jac_metric(c.mx, c.my, c.mz, ngg, X[b].data(), Y[b].data(), Z[b].data(), c.jac, c.metric,

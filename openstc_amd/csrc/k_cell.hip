@@ -776,10 +776,10 @@ __global__ __launch_bounds__(256) void k_update(ZoneView<R> z, const ostc_param 
   const R dtj = z.dt[id] / z.jac[id];
   R cv[NV];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) {
-    cv[l] = z.cv[l * P + id] + dqf[l * P + id] * dtj;
-    z.cv[l * P + id] = cv[l];
-  }
+  for (int l = 0; l < NV; ++l) cv[l] = z.cv[l * P + id] + dqf[l * P + id] * dtj;
+  if (z.mz == 1) cv[3] = 0;  // a 2-D block carries no z momentum
+#pragma unroll
+  for (int l = 0; l < NV; ++l) z.cv[l * P + id] = cv[l];
   const R rho = cv[0];
   const R rinv = (R)1.0 / rho;
   const R u = cv[1] * rinv, v = cv[2] * rinv, w = cv[3] * rinv;
@@ -1045,6 +1045,30 @@ __global__ __launch_bounds__(256) void k_limit_apply(ZoneView<R> z, int slot) {
 }
 
 __global__ void k_zero_int(int *p) { *p = 0; }
+
+// eliminate_k_gradient (src/FieldOperation.cu:42-62): a 2-D block (mz = 1)
+// repeats its k = 0 plane of bv and sv into the k ghost layers, over the
+// padded (i, j) range
+template <typename R>
+__global__ __launch_bounds__(256) void k_eliminate_k_gradient(ZoneView<R> z, int nsc) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - z.g;
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - z.g;
+  if (i >= z.mx + z.g || j >= z.my + z.g) return;
+  const int64_t P = z.plane, src = z.idx(i, j, 0);
+  for (int k = 1; k <= z.g; ++k) {
+    const int64_t up = z.idx(i, j, k), dn = z.idx(i, j, -k);
+    for (int l = 0; l < 6; ++l) {
+      const R v = z.bv[l * P + src];
+      z.bv[l * P + up] = v;
+      z.bv[l * P + dn] = v;
+    }
+    for (int l = 0; l < nsc; ++l) {
+      const R v = z.sv[l * P + src];
+      z.sv[l * P + up] = v;
+      z.sv[l * P + dn] = v;
+    }
+  }
+}
 
 // ------------------------------------------------------------------ residual
 
@@ -1336,6 +1360,15 @@ int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hi
 }
 
 template <typename R>
+int eliminate_k_gradient(const ZoneView<R> &z, const CellOpts &o, hipStream_t st) {
+  if (z.mz != 1) return OSTC_OK;
+  const dim3 block(64, 4, 1);
+  const dim3 grid((unsigned)((z.mx + 2 * z.g + 63) / 64), (unsigned)((z.my + 2 * z.g + 3) / 4), 1);
+  hipLaunchKernelGGL((k_eliminate_k_gradient<R>), grid, block, 0, st, z, o.ns + o.nt);
+  return last_err();
+}
+
+template <typename R>
 int store_last_step(const ZoneView<R> &z, hipStream_t st) {
   const auto g = cell_grid(z.mx, z.my, z.mz);
   hipLaunchKernelGGL((k_store_last<R>), g.grid, g.block, 0, st, z);
@@ -1375,6 +1408,7 @@ int wall_distance(const ZoneView<R> &z, const double *wall_xyz, int n_wall, hipS
                                hipStream_t, int);                                                               \
   template int limit_flow<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t, int);      \
   template int store_last_step<R>(const ZoneView<R> &, hipStream_t);                                            \
+  template int eliminate_k_gradient<R>(const ZoneView<R> &, const CellOpts &, hipStream_t);                      \
   template int residual_partials<R>(const ZoneView<R> &, double *, hipStream_t, int *);                         \
   template int wall_distance<R>(const ZoneView<R> &, const double *, int, hipStream_t);
 OSTC_INST(float)

@@ -65,6 +65,9 @@ template <typename R>
 int limit_flow(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, int slot = -1);
 template <typename R>
 int store_last_step(const ZoneView<R> &z, hipStream_t st);
+// 2-D blocks: the k = 0 plane of bv/sv copied into the k ghost layers (no-op for mz > 1)
+template <typename R>
+int eliminate_k_gradient(const ZoneView<R> &z, const CellOpts &o, hipStream_t st);
 // per-zone partial sums -> partial[nblocks*4]; returns nblocks
 template <typename R>
 int residual_partials(const ZoneView<R> &z, double *partial, hipStream_t st, int *nblocks);

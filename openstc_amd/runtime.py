@@ -108,13 +108,13 @@ def block_geometry(x: np.ndarray, y: np.ndarray, z: np.ndarray, ngg: int = 2):
     return xo, yo, zo, jac, met, nneg.value
 
 
-def register_inner_face(raw, ngg: int = 2) -> L.InnerFace:
+def register_inner_face(raw, ngg: int = 2, dim: int = 3) -> L.InnerFace:
     """One record of the reference's inner{rank}.txt -> registered face
     (libostc; src/Mesh.cpp:44-178)."""
     lib = L.load()
     r = (C.c_int * 13)(*raw)
     out = L.InnerFace()
-    L.check(lib.ostc_register_inner_face(r, ngg, C.byref(out)), "ostc_register_inner_face")
+    L.check(lib.ostc_register_inner_face(r, ngg, dim, C.byref(out)), "ostc_register_inner_face")
     return out
 
 
@@ -186,6 +186,9 @@ class Simulation:
                     L.check(lib.ostc_wall_distance(c, z.h, _dptr(a), a.shape[0], None), "wall_distance")
                 else:
                     z.upload("WALL_DIST", np.full((1,) + z.shape, 1e3))
+        for z in self.zones:
+            if z.mz == 1:  # 2-D blocks (src/Driver.cu:59-65)
+                L.check(lib.ostc_eliminate_k_gradient(c, z.h, None), "eliminate_k_gradient")
         for z in self.zones:
             L.check(lib.ostc_apply_bc(c, z.h, None), "apply_bc")
         for z in self.zones:

@@ -139,6 +139,8 @@ def _bind(lib):
     Pp = C.POINTER(OrcParam)
     sig = {
         "orc_ghost_grid": (None, [I, I, I, I, _DP, _DP, _DP]),
+        "orc_ghost_faces": (None, [I, I, I, I, _DP, _DP, _DP]),
+        "orc_ghost_edges": (None, [I, I, I, I, _DP, _DP, _DP]),
         "orc_jac_metric": (I, [I, I, I, I, _DP, _DP, _DP, _DP, _DP, C.POINTER(I)]),
         "orc_trim_ghost": (None, [I, I, I, I, _DP, _DP]),
         "orc_inner_ghost_grid": (None, [C.POINTER(OrcInnerFace), I, I, I, I, _DP, _DP, _DP, I, I, I, _DP, _DP,
@@ -165,6 +167,7 @@ def _bind(lib):
         "orc_limit_flow": (I, [Z, Pp]),
         "orc_apply_bc": (None, [Z, Pp, C.POINTER(OrcBC)]),
         "orc_inner_communication": (None, [Z, I, C.POINTER(OrcInnerFace), Pp]),
+        "orc_eliminate_k_gradient": (None, [Z, Pp]),
         "orc_store_last_step": (None, [Z]),
         "orc_square_of_dbv": (None, [Z]),
         "orc_sum_dbv": (None, [Z, _DP]),
@@ -225,8 +228,9 @@ class OracleBlock:
         for k in ZONE_ARRAYS:
             setattr(self.s, k, dp(self.arr[k]))
 
-    def ghost_grid(self, x, y, z):
-        """Padded (ngg+1) coordinates with the single-block ghost grid."""
+    def ghost_grid(self, x, y, z, edges=True):
+        """Padded (ngg+1) coordinates with the single-block ghost grid (the
+        face layers only with edges=False)."""
         lib = load()
         mx, my, mz, g = self.mx, self.my, self.mz, self.ngg
         G = g + 1
@@ -235,9 +239,16 @@ class OracleBlock:
         X[G:G + mz, G:G + my, G:G + mx] = x
         Y[G:G + mz, G:G + my, G:G + mx] = y
         Z[G:G + mz, G:G + my, G:G + mx] = z
-        lib.orc_ghost_grid(mx, my, mz, g, dp(X), dp(Y), dp(Z))
+        if edges:
+            lib.orc_ghost_grid(mx, my, mz, g, dp(X), dp(Y), dp(Z))
+        else:
+            lib.orc_ghost_faces(mx, my, mz, g, dp(X), dp(Y), dp(Z))
         self.big = (X, Y, Z)
         return self.big
+
+    def ghost_edges(self):
+        X, Y, Z = self.big
+        load().orc_ghost_edges(self.mx, self.my, self.mz, self.ngg, dp(X), dp(Y), dp(Z))
 
     def metrics(self):
         """Jacobian/metrics from the padded coordinates; trims the ghosts."""
@@ -277,7 +288,7 @@ class OracleSimulation:
         self.blocks = []
         for blk in case.blocks:
             ob = OracleBlock(blk.mx, blk.my, blk.mz, case.ngg, self.param)
-            ob.ghost_grid(blk.x, blk.y, blk.z)
+            ob.ghost_grid(blk.x, blk.y, blk.z, edges=False)
             self.blocks.append(ob)
         # ghost layers across inner faces from the neighbours (src/Mesh.cpp:940-985)
         lib = self.lib
@@ -289,6 +300,9 @@ class OracleSimulation:
                 TX, TY, TZ = t.big
                 lib.orc_inner_ghost_grid(C.byref(copy_struct(OrcInnerFace, f)), ob.mx, ob.my, ob.mz, case.ngg,
                                          dp(X), dp(Y), dp(Z), t.mx, t.my, t.mz, dp(TX), dp(TY), dp(TZ))
+        # edges and corners after the inner-face layers (src/Mesh.cpp:859-937)
+        for ob in self.blocks:
+            ob.ghost_edges()
         for blk, ob in zip(case.blocks, self.blocks):
             ob.metrics()
             ob.arr["bv"][:] = blk.bv0
@@ -314,6 +328,9 @@ class OracleSimulation:
                     lib.orc_wall_distance(C.byref(self.zones[b]), dp(a), a.shape[0])
                 else:
                     self.blocks[b].arr["wall_distance"][:] = 1e3
+        for b in range(len(self.blocks)):
+            if self.blocks[b].mz == 1:  # 2-D blocks (src/Driver.cu:59-65)
+                lib.orc_eliminate_k_gradient(C.byref(self.zones[b]), p)
         for b in range(len(self.blocks)):
             lib.orc_apply_bc(C.byref(self.zones[b]), p, C.byref(self.bcs[b]))
         for b in range(len(self.blocks)):

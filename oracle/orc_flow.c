@@ -1299,6 +1299,25 @@ void orc_inner_communication(orc_zone *zones, int blk, const orc_inner_face *f, 
       }
 }
 
+/* eliminate_k_gradient: src/FieldOperation.cu:42-62.  A 2-D block (mz = 1)
+ * repeats its k = 0 plane into the k ghost layers, so every k derivative
+ * vanishes; i and j cover the ghost layers too. */
+void orc_eliminate_k_gradient(orc_zone *z, const orc_param *p) {
+  const int g = z->ngg;
+  for (int j = -g; j < z->my + g; ++j)
+    for (int i = -g; i < z->mx + g; ++i)
+      for (int k = 1; k <= g; ++k) {
+        for (int l = 0; l < 6; ++l) {
+          F(bv, i, j, k, l) = F(bv, i, j, 0, l);
+          F(bv, i, j, -k, l) = F(bv, i, j, 0, l);
+        }
+        for (int l = 0; l < p->n_scalar; ++l) {
+          F(sv, i, j, k, l) = F(sv, i, j, 0, l);
+          F(sv, i, j, -k, l) = F(sv, i, j, 0, l);
+        }
+      }
+}
+
 /* ---------------------------------------------------------------- residual */
 
 /* store_last_step: src/TimeAdvanceFunc.cu:5-16 (bv_last interior, 4 planes) */
@@ -1384,6 +1403,8 @@ double orc_steady_step(orc_zone *zones, int n_blocks, const orc_param *p, const 
     for (int f = 0; f < n_inner[b]; ++f) orc_inner_communication(zones, b, &inner[off + f], p);
     off += n_inner[b];
   }
+  for (int b = 0; b < n_blocks; ++b)
+    if (zones[b].mz == 1) orc_eliminate_k_gradient(&zones[b], p);
   for (int b = 0; b < n_blocks; ++b) orc_update_physical_properties(&zones[b], p);
   if (newton_fail) *newton_fail = nf;
   if (!do_res) return -1;

@@ -22,8 +22,16 @@ static inline long gidx(int mx, int my, int G, int i, int j, int k) {
 #define Z(i, j, k) z[gidx(mx, my, G, (i), (j), (k))]
 
 /* Mesh::init_ghost_grid, single-block part (src/Mesh.cpp:789-857 faces,
- * :864-937 edges/corners by the parallelogram rule).  3-D only. */
+ * :864-937 edges/corners by the parallelogram rule).  A block with mz = 1 is
+ * a 2-D block: its k ghost layers repeat x, y and step z by one per layer
+ * (src/Mesh.cpp:840-856). */
 void orc_ghost_grid(int mx, int my, int mz, int ngg, double *x, double *y, double *z) {
+  orc_ghost_faces(mx, my, mz, ngg, x, y, z);
+  orc_ghost_edges(mx, my, mz, ngg, x, y, z);
+}
+
+/* the face layers of Mesh::init_ghost_grid (src/Mesh.cpp:789-857) */
+void orc_ghost_faces(int mx, int my, int mz, int ngg, double *x, double *y, double *z) {
   const int G = ngg + 1;
   for (int k = 0; k < mz; ++k)
     for (int j = 0; j < my; ++j) {
@@ -53,6 +61,19 @@ void orc_ghost_grid(int mx, int my, int mz, int ngg, double *x, double *y, doubl
     }
   for (int j = 0; j < my; ++j)
     for (int i = 0; i < mx; ++i) {
+      if (mz == 1) {
+        for (int k = -1; k >= -ngg - 1; --k) {
+          X(i, j, k) = X(i, j, 0);
+          Y(i, j, k) = Y(i, j, 0);
+          Z(i, j, k) = Z(i, j, 0) + k;
+        }
+        for (int k = mz; k < mz + ngg + 1; ++k) {
+          X(i, j, k) = X(i, j, mz - 1);
+          Y(i, j, k) = Y(i, j, mz - 1);
+          Z(i, j, k) = Z(i, j, mz - 1) + k - mz + 1;
+        }
+        continue;
+      }
       for (int k = -1; k >= -ngg - 1; --k) {
         X(i, j, k) = 2 * X(i, j, 0) - X(i, j, -k);
         Y(i, j, k) = 2 * Y(i, j, 0) - Y(i, j, -k);
@@ -64,6 +85,12 @@ void orc_ghost_grid(int mx, int my, int mz, int ngg, double *x, double *y, doubl
         Z(i, j, k) = 2 * Z(i, j, mz - 1) - Z(i, j, 2 * (mz - 1) - k);
       }
     }
+}
+
+/* edges and corners of Mesh::init_ghost_grid by the parallelogram rule
+ * (src/Mesh.cpp:864-937); after the inner-face ghost layers in a mesh */
+void orc_ghost_edges(int mx, int my, int mz, int ngg, double *x, double *y, double *z) {
+  const int G = ngg + 1;
   for (int i = 1; i <= ngg + 1; ++i)
     for (int j = 1; j <= ngg + 1; ++j)
       for (int k = 0; k < mz; ++k) {
@@ -153,15 +180,17 @@ static void copy_metric(int mx, int my, int g, double *jac, double *metric, long
 
 /* Block::compute_jac_metric (src/Mesh.cpp:259-521): interior, then the six
  * ghost slabs in the reference's order; a non-positive ghost Jacobian copies
- * the inner neighbour's J and metric. */
+ * the inner neighbour's J and metric.  With mz = 1 the interior pass covers
+ * k = 0 (src/Mesh.cpp:263-268). */
 int orc_jac_metric(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
                    double *jac, double *metric, int *n_neg) {
   const int G = ngg + 1, g = ngg;
   const long plane = (long)(mx + 2 * g) * (my + 2 * g) * (mz + 2 * g);
   int neg = 0;
+  const int k_min = mz == 1 ? 0 : 1, k_max = mz == 1 ? 1 : mz - 1;
   for (int i = 1; i < mx - 1; ++i)
     for (int j = 1; j < my - 1; ++j)
-      for (int k = 1; k < mz - 1; ++k)
+      for (int k = k_min; k < k_max; ++k)
         if (!metric_point(mx, my, G, g, x, y, z, i, j, k, jac, metric, plane)) return -1;
   for (int i = 1; i < mx - 1; ++i)
     for (int j = 1; j < my - 1; ++j)

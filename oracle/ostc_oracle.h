@@ -110,6 +110,10 @@ void orc_zone_dims(orc_zone *z);
 /* ---- geometry: src/Mesh.cpp ---- */
 /* Interior coordinates in (ngg+1)-padded arrays -> ghost coordinates. */
 void orc_ghost_grid(int mx, int my, int mz, int ngg, double *x, double *y, double *z);
+/* the two halves of orc_ghost_grid: face layers, then edges and corners (a
+ * mesh copies the inner-face ghost layers in between) */
+void orc_ghost_faces(int mx, int my, int mz, int ngg, double *x, double *y, double *z);
+void orc_ghost_edges(int mx, int my, int mz, int ngg, double *x, double *y, double *z);
 /* Returns 0, or -1 on a non-positive interior Jacobian; *n_neg counts ghost
  * points repaired from their inner neighbour. */
 int orc_jac_metric(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
@@ -154,6 +158,8 @@ int orc_update_cv_and_bv(orc_zone *z, const orc_param *p);
 int orc_limit_flow(orc_zone *z, const orc_param *p);
 void orc_apply_bc(orc_zone *z, const orc_param *p, const orc_bc *bc);
 void orc_inner_communication(orc_zone *zones, int blk, const orc_inner_face *f, const orc_param *p);
+/* eliminate_k_gradient (src/FieldOperation.cu:42-62): 2-D blocks only */
+void orc_eliminate_k_gradient(orc_zone *z, const orc_param *p);
 void orc_store_last_step(orc_zone *z);
 void orc_square_of_dbv(orc_zone *z);
 void orc_sum_dbv(const orc_zone *z, double out[4]);

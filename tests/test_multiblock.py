@@ -45,7 +45,7 @@ def test_register_inner_face_known_answers():
     assert list(c.target_loop_dir) == [-1, 1, 1] and list(c.src_tar) == [0, 1, 2]
     # malformed record (no face direction) is rejected
     r = (C.c_int * 13)(*[1, 2, 1, 2, 1, 2, 1, 2, 1, 2, 1, 2, 1])
-    assert L.load().ostc_register_inner_face(r, 2, C.byref(L.InnerFace())) != 0
+    assert L.load().ostc_register_inner_face(r, 2, 3, C.byref(L.InnerFace())) != 0
 
 
 def test_mesh_geometry_matches_oracle_and_neighbours():
@@ -62,6 +62,16 @@ def test_mesh_geometry_matches_oracle_and_neighbours():
     for q in (1, 2):
         assert np.array_equal(b0.xg[g:-g, g:-g, g + b0.mx - 1 + q], b1.x[:, :, q])
         assert np.array_equal(b1.xg[g:-g, g:-g, g - q], b0.x[:, :, b0.mx - 1 - q])
+    # the edge ghosts along the inner face come from the parallelogram rule
+    # applied after the neighbour's points were copied (src/Mesh.cpp:859-937):
+    # x(i2, -1) = x(i2, 0) + x(mx-1, -1) - x(mx-1, 0) with x(i2, 0) block 2's point
+    mx0 = b0.mx
+    for q in (1, 2):
+        for k in range(b0.mz):
+            kk = g + k
+            got = b0.xg[kk, g - 1, g + mx0 - 1 + q]
+            ref = (b1.x[k, 0, q] + b0.xg[kk, g - 1, g + mx0 - 1]) - b0.x[k, 0, mx0 - 1]
+            assert got == ref, (q, k)
     # and differ from the single-block reflection (the warp makes it matter)
     from openstc_amd.runtime import block_geometry
     xs = block_geometry(b0.x, b0.y, b0.z, g)[0]
