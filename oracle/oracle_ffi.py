@@ -165,6 +165,7 @@ def _bind(lib):
         "orc_jac_times_dq_point": (None, [Z, Pp, I, I, I, I, D, _DP]),
         "orc_update_cv_and_bv": (I, [Z, Pp]),
         "orc_limit_flow": (I, [Z, Pp]),
+        "orc_limit_flow_total": (C.c_long, [I]),
         "orc_apply_bc": (None, [Z, Pp, C.POINTER(OrcBC)]),
         "orc_inner_communication": (None, [Z, I, C.POINTER(OrcInnerFace), Pp]),
         "orc_eliminate_k_gradient": (None, [Z, Pp]),

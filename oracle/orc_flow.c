@@ -918,6 +918,10 @@ int orc_update_cv_and_bv(orc_zone *z, const orc_param *p) {
  * everywhere else).  A cell's flow repair precedes its turbulence repair, as in
  * the reference.  The i-neighbour bound uses mz, as the reference does.
  * Returns the number of cells with a flow repair. */
+/* running count of flow repairs over every orc_limit_flow call (the
+ * device counter's counterpart), read and reset by orc_limit_flow_total */
+static long orc_lf_total = 0;
+
 int orc_limit_flow(orc_zone *z, const orc_param *p) {
   const int mx = z->mx, my = z->my, mz = z->mz, ns = p->n_spec, nsc = p->n_scalar, nv = p->n_var;
   const double *ll = p->limit_ll, *ul = p->limit_ul;
@@ -1057,7 +1061,14 @@ int orc_limit_flow(orc_zone *z, const orc_param *p) {
   }
   free(lidx);
   free(lval);
+  orc_lf_total += repaired;
   return repaired;
+}
+
+long orc_limit_flow_total(int reset) {
+  const long t = orc_lf_total;
+  if (reset) orc_lf_total = 0;
+  return t;
 }
 
 /* ---------------------------------------------------------------- BCs */

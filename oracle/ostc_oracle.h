@@ -156,6 +156,8 @@ void orc_jac_times_dq_point(const orc_zone *z, const orc_param *p, int i, int j,
                             double *out);
 int orc_update_cv_and_bv(orc_zone *z, const orc_param *p);
 int orc_limit_flow(orc_zone *z, const orc_param *p);
+/* flow repairs made by every orc_limit_flow call so far; reset if asked */
+long orc_limit_flow_total(int reset);
 void orc_apply_bc(orc_zone *z, const orc_param *p, const orc_bc *bc);
 void orc_inner_communication(orc_zone *zones, int blk, const orc_inner_face *f, const orc_param *p);
 /* eliminate_k_gradient (src/FieldOperation.cu:42-62): 2-D blocks only */

@@ -4,9 +4,9 @@ benchmark configurations, including the FP32-bulk zones as they are timed:
   * the warped Euler box at its stated size (64x64x8, MUSCL + minmod +
     AUSM+, explicit, FP64) for its stated 100 steps, residual history
     included, and the same run with FP32 bulk storage;
-  * the SST flat plate (48x32x12, DPLUR x4, CFL 5) in FP32 bulk, run by the
-    GPU and by the oracle until the residual has dropped four decades
-    (1000 iterations), converged fields compared;
+  * the SST flat plate (48x32x12, DPLUR x4, CFL 2) in FP32 bulk, run by the
+    GPU and by the oracle to a converged state (8000 iterations, the
+    oracle's residual below 5e-8), converged fields compared at 1e-5;
   * the finite-rate mixing layer (28x20x10, 9 species / 19 reactions, EPI,
     DPLUR x4) in FP32 bulk + FP64 chemistry/Newton for 20 iterations;
   * the four-block jet in crossflow (4 x 16x12x10, SST + finite rate,
@@ -87,24 +87,51 @@ def test_warped_box_stated_size_fp32_100_steps():
     pr.close()
 
 
+def converged_errors(pr: Pair):
+    """Converged-field errors: each field scaled by its own maximum, except
+    the velocity components, which are scaled by the largest speed (the
+    spanwise and wall-normal components of a converged plate flow are tiny,
+    and a difference scaled by their own maxima measures nothing physical)."""
+    g = pr.case.ngg
+    a = interior(pr.zg().download("BV"), g)
+    o = interior(pr.oarr("BV"), g)
+    speed = float(np.sqrt(o[1] ** 2 + o[2] ** 2 + o[3] ** 2).max())
+    out = {}
+    for l, name in enumerate(("rho", "u", "v", "w", "p", "T")):
+        if 1 <= l <= 3:
+            out[name] = float(np.abs(a[l] - o[l]).max()) / speed
+        else:
+            out[name] = scaled_err(a[l:l + 1], o[l:l + 1])
+    a = interior(pr.zg().download("SV"), g)
+    o = interior(pr.oarr("SV"), g)
+    out["k"] = scaled_err(a[0:1], o[0:1])
+    out["omega"] = scaled_err(a[1:2], o[1:2])
+    return out
+
+
 def test_sst_plate_fp32_converged():
-    """FP32-bulk SST + DPLUR run until the residual has dropped four decades
-    (1000 iterations of the 48x32x12 plate; the oracle's residual passes
-    1e-4 at that point) and compared with the FP64 oracle there."""
-    case = cases.sst_plate(mx=48, my=32, mz=12, perturb=True)
-    say("sst_plate 48x32x12 FP32 vs FP64 oracle, 1000 iterations")
+    """FP32-bulk SST + DPLUR run to a converged state and compared with the
+    FP64 oracle there.  The 48x32x12 plate with its +-1 % perturbation at CFL 2
+    converges (the oracle's normalised residual falls below 5e-8 by iteration
+    8000); at CFL 5 the same block settles into a residual plateau near 1e-4,
+    where FP32 and FP64 follow different transients and no converged field
+    exists to compare."""
+    case = cases.sst_plate(mx=48, my=32, mz=12, perturb=True, cfl=2.0)
+    steps = 8000
+    say(f"sst_plate 48x32x12 CFL 2, FP32 vs FP64 oracle, {steps} iterations")
     pr = Pair(case, precision=L.FP32)
     pr.init()
     rg = ro = None
-    for s in range(1, 1001):
-        a = pr.gpu.step(output_screen=100)
-        b = pr.orc.step(output_screen=100)
+    for s in range(1, steps + 1):
+        a = pr.gpu.step(output_screen=500)
+        b = pr.orc.step(output_screen=500)
         if a is not None:
             rg, ro = a, b
-            if s % 200 == 0:
+            if s % 1000 == 0:
                 say(f"step {s}: residual gpu {rg:.3e} oracle {ro:.3e}")
-    assert ro <= 1.2e-4 and rg <= 1.2e-4, (rg, ro)
-    check(field_errors(pr), {"*": 1e-4}, "SST plate FP32 converged")
+    assert ro <= 5e-8, ro
+    errs = converged_errors(pr)
+    check(errs, {"*": 1e-5}, "SST plate FP32 converged")
     pr.close()
 
 
