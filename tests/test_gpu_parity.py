@@ -206,7 +206,7 @@ def test_warped_box_fp32_vs_fp64_trajectory():
         pr.orc.step()
     e = pr.err("BV")
     say("bv scaled err FP32 after 20 steps", e)
-    assert e < 1e-4
+    assert e < 1e-5  # measured 1.1e-6
     pr.close()
 
 
@@ -224,5 +224,5 @@ def test_sst_plate_fp32_vs_fp64_trajectory():
     for f in ("BV", "SV"):
         e = pr.err(f)
         say("scaled err FP32 after 5 steps", f, e)
-        assert e < 1e-4, (f, e)
+        assert e < 2e-5, (f, e)  # measured 7.2e-6 (BV), 9.2e-6 (SV)
     pr.close()
