@@ -331,6 +331,17 @@ int ostc_store_last_step(ostc_ctx *ctx, ostc_zone *zone, void *stream);
  * 18-32, src/Driver.cu:309-328): sums[4] of the squared changes of rho, |V|,
  * p, T over the interior of the given zones (synchronises the stream). */
 int ostc_residual_sums(ostc_ctx *ctx, ostc_zone **zones, int n, double sums[4], void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the sums of
+ * ostc_residual_sums (same kernels, same summation order, so the same bits),
+ * written to the device array d_sums[4] on `stream` without synchronising and
+ * without any allocation.  The partial sums go to a library-owned device
+ * array of 2^20 doubles (8 MiB, one per GPU, held until the process ends):
+ * 4 doubles per 64x4 cell tile of every zone, i.e. the sum over zones of
+ * ceil(mx/64)*ceil(my/4)*mz must not exceed 262144, else OSTC_ERR_ARG and
+ * nothing is launched.  Calls that overlap on one GPU (other streams, other
+ * threads) share that array and race. */
+int ostc_residual_sums_noalloc(ostc_ctx *ctx, ostc_zone **zones, int n, double *d_sums, void *stream);
 /* Device-side counters since the last reset: [0] Newton iterations that hit
  * the 1000 cap, [1] cells repaired by limit_flow, [2] surrogate fallbacks
  * (synchronises the stream). */

@@ -15,6 +15,7 @@ MAX_FACE = 32
 MAX_BCDATA = 8
 
 OK = 0
+ERR_ARG = 1
 AIR, MIXTURE, FR = 0, 1, 2
 LAMINAR, RANS = 0, 1
 FP32, FP64 = 4, 8
@@ -170,6 +171,7 @@ SIGNATURES = {
     "ostc_update_physical_properties": (C.c_int, [_P, _P, _P]),
     "ostc_store_last_step": (C.c_int, [_P, _P, _P]),
     "ostc_residual_sums": (C.c_int, [_P, C.POINTER(_P), C.c_int, _D, _P]),
+    "ostc_residual_sums_noalloc": (C.c_int, [_P, C.POINTER(_P), C.c_int, _P, _P]),
     "ostc_eliminate_k_gradient": (C.c_int, [_P, _P, _P]),
     "ostc_zone_counters": (C.c_int, [_P, C.POINTER(C.c_longlong), C.c_int, _P]),
     "ostc_steady_step": (C.c_int, [_P, C.POINTER(_P), C.c_int, C.POINTER(InnerFace), _I, C.c_int, C.c_int, _D,

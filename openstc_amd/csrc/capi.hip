@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -54,6 +55,34 @@ struct Profiler {
   }
 };
 Profiler *g_prof = nullptr;
+
+// Partial sums of ostc_residual_sums_noalloc: a library-owned device array,
+// one copy per GPU, held until the process ends, so that entry point
+// allocates nothing.  In host code `scratch` is only a stand-in for the
+// symbol: kernels and copies get the per-device address that
+// hipGetSymbolAddress returns (g_scratch_ptr), never the name itself.
+constexpr long kScratchDoubles = 1L << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Look the scratch array's address up on the current device (once per
+// device; ostc_ctx_create calls this with the context's device current).
+int scratch_setup(double **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(OSTC_ERR_ARG, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(OSTC_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  if (out) *out = g_scratch_ptr[dev];
+  return OSTC_OK;
+}
 
 }  // namespace
 
@@ -226,6 +255,10 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   if (e != hipSuccess) {
     ostc_ctx_destroy(c);
     return hip_fail(e, "ctx setup");
+  }
+  if (int r = scratch_setup(nullptr)) {
+    ostc_ctx_destroy(c);
+    return r;
   }
   CellOpts &o = c->opts;
   o.ns = param->n_spec;
@@ -555,6 +588,31 @@ int ostc_residual_sums(ostc_ctx *c, ostc_zone **zones, int n, double sums[4], vo
   HIP_TRY(hipStreamSynchronize(S(s)));
   for (int l = 0; l < 4; ++l) sums[l] = c->host4[l];
   return OSTC_OK;
+}
+
+int ostc_residual_sums_noalloc(ostc_ctx *c, ostc_zone **zones, int n, double *d_sums, void *s) {
+  if (!c || !zones || n <= 0 || !d_sums) return fail(OSTC_ERR_ARG, "null argument");
+  long need = 0;
+  for (int b = 0; b < n; ++b) {
+    if (!zones[b]) return fail(OSTC_ERR_ARG, "null zone");
+    const auto g = cell_grid(zones[b]->desc.mx, zones[b]->desc.my, zones[b]->desc.mz);
+    need += (long)g.grid.x * g.grid.y * g.grid.z * 4;
+    if (need > kScratchDoubles)
+      return fail(OSTC_ERR_ARG, "residual partial sums exceed the 8 MiB scratch array (2^20 doubles: "
+                                "4 per 64x4 cell tile of every zone)");
+  }
+  double *ptr = nullptr;
+  if (int r = scratch_setup(&ptr)) return r;
+  int off = 0;
+  for (int b = 0; b < n; ++b) {
+    int nb = 0;
+    ostc_zone *z = zones[b];
+    int r = ZONE_CALL(z, residual_partials(z->vf, ptr + (size_t)off * 4, S(s), &nb),
+                      residual_partials(z->vd, ptr + (size_t)off * 4, S(s), &nb));
+    if (r) return r;
+    off += nb;
+  }
+  return residual_finalize(ptr, off, d_sums, S(s));
 }
 
 int ostc_zone_counters(ostc_zone *z, long long out[4], int reset, void *s) {
