@@ -723,7 +723,7 @@ int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_fac
 static const char *kStageNames[P_COUNT] = {"source", "flux_i", "flux_j", "flux_k", "local_dt", "dplur_dq0",
                                            "dplur_sweep", "explicit_implicit", "update", "limit_flow", "bc",
                                            "inner_halo", "properties", "store_last", "residual", "flux_ij",
-                                           "dplur_sweep_first"};
+                                           "dplur_sweep_first", "dplur_chem"};
 
 int ostc_profile(ostc_ctx *c, int enable) {
   (void)c;

@@ -422,18 +422,9 @@ __device__ __forceinline__ void dplur_finish(const ZoneView<R> &z, const ostc_pa
     const int meth = p->chem_src_method;
     const double dtd = (double)dt, dg = (double)diag;
     if (meth == 1) {
-      double a[NS * NS], b[NS];
-      int piv[NS];
+      // the species rows' off-diagonal sums; k_dplur_chem solves the block
 #pragma unroll
-      for (int q = 0; q < NS * NS; ++q) a[q] = z.chem_lu[q * P + id];
-#pragma unroll
-      for (int q = 0; q < NS; ++q) {
-        piv[q] = z.chem_piv[q * P + id];
-        b[q] = (double)acc[5 + q] * dtd;
-      }
-      lu_solve<NS>(a, piv, b, NS);
-#pragma unroll
-      for (int q = 0; q < NS; ++q) dout[(5 + q) * P + id] = (R)((double)z.dq0[(5 + q) * P + id] + b[q]);
+      for (int q = 0; q < NS; ++q) dout[(5 + q) * P + id] = acc[5 + q];
     } else if (meth == 2) {
 #pragma unroll
       for (int q = 0; q < NS; ++q)
@@ -455,6 +446,33 @@ __device__ __forceinline__ void dplur_finish(const ZoneView<R> &z, const ostc_pa
       dout[(NS + 6) * P + id] = q0(NS + 6) + dt * acc[6 + NS] / diag;
     }
   }
+}
+
+// Chemistry block of one DPLUR sweep of a reacting zone with EPI
+// (EPI_for_dqk, src/FiniteRateChem.cu:255-284): the sweep kernel leaves the
+// species rows' off-diagonal sums in dout; each cell solves its block with
+// the LU factors k_dq0 kept, dout_s = dq0_s + (diag I - dt J)^-1 (dt acc_s),
+// in place.  A kernel of its own: one streaming pass over the 9x9 FP64
+// factors with every load issued at once, instead of loads that the sweep's
+// neighbour exchange would leave exposed at the end of each plane.
+template <typename R, int NS>
+__global__ __launch_bounds__(256) void k_dplur_chem(ZoneView<R> z, R *__restrict__ dout) {
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const double dtd = (double)z.dt[id];
+  double a[NS * NS], b[NS];
+  int piv[NS];
+#pragma unroll
+  for (int q = 0; q < NS * NS; ++q) a[q] = z.chem_lu[q * P + id];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    piv[q] = z.chem_piv[q * P + id];
+    b[q] = (double)dout[(5 + q) * P + id] * dtd;
+  }
+  lu_solve<NS>(a, piv, b, NS);
+#pragma unroll
+  for (int q = 0; q < NS; ++q) dout[(5 + q) * P + id] = (R)((double)z.dq0[(5 + q) * P + id] + b[q]);
 }
 
 // Per-cell inputs of the convective-Jacobian products of one DPLUR sweep
@@ -1317,6 +1335,13 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
     }
     prof_mark(stage, st, false);
     if (last_err()) return OSTC_ERR_HIP;
+    if (chem && o.chem == 1 && o.ns > 0) {
+      prof_mark(P_DPLUR_CHEM, st, true);
+      if (o.ns != 9) return OSTC_ERR_ARG;
+      hipLaunchKernelGGL((k_dplur_chem<R, 9>), g.grid, g.block, 0, st, z, out);
+      prof_mark(P_DPLUR_CHEM, st, false);
+      if (last_err()) return OSTC_ERR_HIP;
+    }
     in = out;
     out = (out == z.dqk) ? z.dq : z.dqk;
   }

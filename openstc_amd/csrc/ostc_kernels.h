@@ -9,7 +9,7 @@ namespace ostc {
 // ostc_profile); prof_mark is a no-op when profiling is off.
 enum ProfStage {
   P_SOURCE = 0, P_FLUX_I, P_FLUX_J, P_FLUX_K, P_DT, P_DQ0, P_DPLUR, P_EXPL_IMPL, P_UPDATE, P_LIMIT, P_BC,
-  P_HALO, P_PROPS, P_STORE, P_RESID, P_FLUX_IJ, P_DPLUR_FIRST, P_COUNT
+  P_HALO, P_PROPS, P_STORE, P_RESID, P_FLUX_IJ, P_DPLUR_FIRST, P_DPLUR_CHEM, P_COUNT
 };
 void prof_mark(int stage, hipStream_t st, bool begin);
 

@@ -55,7 +55,7 @@ def main(argv=None):
     ap.add_argument("--activation", choices=sorted(S.ACTIVATIONS), default="hswish")
     a = ap.parse_args(argv)
     jet = a.workload == "jet"
-    defaults = dict(steps=60, every=3, per_snapshot=25000, val_per_snapshot=5000, cfl=1.0, grid=[256, 192, 128]) \
+    defaults = dict(steps=60, every=3, per_snapshot=25000, val_per_snapshot=5000, cfl=0.2, grid=[256, 192, 128]) \
         if jet else dict(steps=400, every=20, per_snapshot=100000, val_per_snapshot=20000, cfl=0.2, grid=[192, 96, 96])
     for k, v in defaults.items():
         if getattr(a, k) is None:
