@@ -86,7 +86,7 @@ def test_warped_box_stated_size_fp32_100_steps():
     for s in range(100):
         pr.gpu.step()
         pr.orc.step()
-    check(field_errors(pr), {"*": 1e-4}, "64x64x8 FP32 after 100 steps")
+    check(field_errors(pr), {"*": 1e-5}, "64x64x8 FP32 after 100 steps")  # measured 2.8e-6
     pr.close()
 
 
@@ -163,7 +163,7 @@ def test_mixing_layer_fp32_20_iterations():
             errs = field_errors(pr)
             say(f"iteration {s}", {k: f"{v:.2e}" for k, v in errs.items()})
     no_repairs(pr)
-    check(field_errors(pr), {"*": 1e-4}, "mixing layer FP32 after 20 iterations")
+    check(field_errors(pr), {"*": 2e-5}, "mixing layer FP32 after 20 iterations")  # measured 8.5e-6
     pr.close()
 
 
@@ -180,5 +180,6 @@ def test_jet_crossflow_fp32_20_iterations():
         pr.orc.step()
     no_repairs(pr)
     for b in range(4):
-        check(field_errors(pr, b=b), {"*": 1e-4}, f"jet block {b} FP32 after 20 iterations")
+        # measured 1.25e-5 (w, scaled by its own small maximum), the rest <= 6.2e-6
+        check(field_errors(pr, b=b), {"*": 3e-5}, f"jet block {b} FP32 after 20 iterations")
     pr.close()
