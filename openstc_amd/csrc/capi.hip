@@ -11,6 +11,7 @@
 
 #include "ostc_kernels.h"
 #include "ostc_surrogate.h"
+#include "ostc_thermo.h"
 
 using namespace ostc;
 
@@ -222,6 +223,50 @@ int implicit_and_update(ostc_ctx *c, const ZoneView<R> &v, hipStream_t st, bool 
 }
 }  // namespace
 
+namespace ostc {
+void make_thermo32(const ostc_param &p, ThermoF32 &out) {
+  std::memset(&out, 0, sizeof(out));
+  for (int i = 0; i < p.n_spec && i < kMaxSpec; ++i) {
+    const double rw = p.mw[i] > 0 ? kRu / p.mw[i] : 0.0;
+    out.t_low[i] = (float)p.t_low[i];
+    out.t_mid[i] = (float)p.t_mid[i];
+    for (int r = 0; r < 2; ++r) {
+      const double *a = r == 0 ? &p.low_temp_coeff[i * 7] : &p.high_temp_coeff[i * 7];
+      const double hc[6] = {a[5], a[0], a[1] / 2, a[2] / 3, a[3] / 4, a[4] / 5};
+      for (int q = 0; q < 6; ++q) out.h[r][i][q] = (float)(rw * hc[q]);
+      for (int q = 0; q < 5; ++q) out.cp[r][i][q] = (float)(rw * a[q]);
+    }
+    const double tl = p.t_low[i], *a = &p.low_temp_coeff[i * 7];
+    const double h_tl = a[0] * tl + a[1] / 2 * tl * tl + a[2] / 3 * tl * tl * tl + a[3] / 4 * tl * tl * tl * tl +
+                        a[4] / 5 * tl * tl * tl * tl * tl + a[5];
+    const double cp_tl = a[0] + a[1] * tl + a[2] * tl * tl + a[3] * tl * tl * tl + a[4] * tl * tl * tl * tl;
+    out.h_tl[i] = (float)(rw * h_tl);
+    out.cp_tl[i] = (float)(rw * cp_tl);
+    out.r_over_w[i] = (float)rw;
+    out.inv_w[i] = p.mw[i] > 0 ? (float)(1.0 / p.mw[i]) : 0.0f;
+    out.cp_lam[i] = (float)(1.25 * rw);
+    out.lj_inv[i] = (float)p.LJ_potent_inv[i];
+    out.vis_coeff[i] = (float)p.vis_coeff[i];
+  }
+  for (int q = 0; q < kMaxSpec * kMaxSpec; ++q) {
+    out.wj_wi4[q] = (float)p.WjDivWi_to_One4th[q];
+    out.sqrt_wi_wj8[q] = (float)p.sqrt_WiDivWjPl1Mul8[q];
+  }
+}
+void make_chem64(const ostc_param &p, ChemF64 &out) {
+  std::memset(&out, 0, sizeof(out));
+  for (int i = 0; i < p.n_spec && i < kMaxSpec; ++i) out.inv_mw[i] = p.mw[i] > 0 ? 1.0 / p.mw[i] : 0.0;
+  for (int r = 0; r < p.n_reac && r < kMaxReac; ++r)
+    for (int l = 0; l < p.n_spec && l < kMaxSpec; ++l) {
+      const int f = p.stoi_f[r * kMaxSpec + l], b = p.stoi_b[r * kMaxSpec + l];
+      out.nu_f[r * kMaxSpec + l] = f;
+      out.nu_b[r * kMaxSpec + l] = b;
+      if (f != 0 || b != 0) out.part[r] |= 1u << l;
+      if (f != b) out.net[r] |= 1u << l;
+    }
+}
+}  // namespace ostc
+
 extern "C" {
 
 const char *ostc_last_error(void) { return g_err.c_str(); }
@@ -238,18 +283,31 @@ int ostc_device_sync(void) {
   return OSTC_OK;
 }
 
+
 int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   if (!param || !out) return fail(OSTC_ERR_ARG, "null argument");
   if (int r = check_param(*param)) return r;
   auto *c = new (std::nothrow) ostc_ctx;
   if (!c) return fail(OSTC_ERR_ALLOC, "host allocation");
   c->host = *param;
-  hipError_t e = hipMalloc(&c->dev, sizeof(ostc_param));
+  // the parameter block, followed by its FP32 thermodynamic and FP64
+  // chemistry tables
+  hipError_t e = hipMalloc(&c->dev, kParamBlockBytes);
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "hipMalloc(param)");
   }
   e = hipMemcpy(c->dev, param, sizeof(ostc_param), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    ThermoF32 tf;
+    make_thermo32(*param, tf);
+    e = hipMemcpy(reinterpret_cast<char *>(c->dev) + kThermoF32Offset, &tf, sizeof(tf), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) {
+    ChemF64 cf;
+    make_chem64(*param, cf);
+    e = hipMemcpy(reinterpret_cast<char *>(c->dev) + kChemF64Offset, &cf, sizeof(cf), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess) e = hipMalloc(&c->out4, 4 * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(&c->host4, 4 * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) {
@@ -263,6 +321,7 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   CellOpts &o = c->opts;
   o.ns = param->n_spec;
   o.nt = param->turb_method == OSTC_RANS ? 2 : 0;
+  o.n_reac = param->n_reac;
   o.mix_model = param->mix_model;
   o.chem = param->chem_src_method;
   o.rans_model = param->rans_model;

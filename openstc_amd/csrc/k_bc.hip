@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void k_wall(ZoneView<R> z, FaceLaunch f, const
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       yw[l] = z.sv[l * P + in];
-      mw += yw[l] * (R)(1.0 / p->mw[l]);
+      mw += yw[l] * inv_mw<R>(p, l);
     }
     mw = (R)1 / mw;
   }
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void k_wall(ZoneView<R> z, FaceLaunch f, const
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       ys[l] = z.sv[l * P + si];
-      mg += ys[l] * (R)(1.0 / p->mw[l]);
+      mg += ys[l] * inv_mw<R>(p, l);
     }
     mg = (R)1 / mg;
   }

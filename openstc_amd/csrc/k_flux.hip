@@ -74,7 +74,7 @@ __device__ __forceinline__ void inviscid_face(const ZoneView<R> &z, const ostc_p
     R mwl = 0, mwr = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      const R inv = (R)(1.0 / p->mw[l]);
+      const R inv = inv_mw<R>(p, l);
       mwl += pl[5 + l] * inv;
       mwr += pr[5 + l] * inv;
     }
@@ -85,7 +85,7 @@ __device__ __forceinline__ void inviscid_face(const ZoneView<R> &z, const ostc_p
     R cpL = 0, cpR = 0, cvL = 0, cvR = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      const R rw = (R)(kRu / p->mw[l]);
+      const R rw = r_over_mw<R>(p, l);
       cpL += cpl[l] * pl[5 + l];
       cpR += cpr[l] * pr[5 + l];
       cvL += pl[5 + l] * (cpl[l] - rw);

@@ -32,6 +32,7 @@ int flux_sweeps(const ZoneView<R> &z, const ostc_param *dp, int ns, int nt, bool
 // k_cell.hip
 struct CellOpts {
   int ns, nt;      // species count, turbulence equations (0 or 2)
+  int n_reac;      // reactions of the mechanism
   int mix_model;   // OSTC_AIR / OSTC_MIXTURE / OSTC_FR
   int chem;        // chem_src_method
   int rans_model, turb_implicit, implicit_method, dplur_steps;
