@@ -390,7 +390,7 @@ def surrogate_at_scale(sim, case, reps: int, weights: str | None):
     from openstc_amd import surrogate as S
     from openstc_amd import _lib as L
 
-    weights = weights or (S.JET_WEIGHTS if os.path.exists(S.JET_WEIGHTS) else S.DEFAULT_WEIGHTS)
+    weights = weights or S.DEFAULT_WEIGHTS
     if not os.path.exists(weights):
         return None
     lib = sim.ctx.lib
@@ -479,51 +479,55 @@ def leg_mixing_layer(a):
         sim.step(output_screen=case.output_screen)
         done += 1
     sim.sync()
-    weights = a.surrogate or S.DEFAULT_WEIGHTS
-    sur = S.Surrogate.load(sim.ctx, weights) if os.path.exists(weights) else None
     z = sim.zones[0]
     dy_p = S.DeviceArray(nbytes=cells * 9 * 8)
     dy_s = S.DeviceArray(nbytes=cells * 9 * 8)
-    stats = (C.c_longlong * 4)()
 
-    def run(mode, buf, st):
-        L.check(lib.ostc_chem_zone_step(sim.ctx.h, z.h, sur.h if sur else None, mode, buf.p, st, None),
-                "zone chemistry")
-
-    modes = (0, 1) if sur else (0,)
-    for m, buf in zip(modes, (dy_p, dy_s)):
-        run(m, buf, None)
-    times = {}
-    for mode, buf in zip(modes, (dy_p, dy_s)):
+    def timed(fn):
+        fn()
         L.check(lib.ostc_device_sync(), "sync")
         t0 = time.perf_counter()
         for _ in range(a.chem_reps):
-            run(mode, buf, None)
+            fn()
         L.check(lib.ostc_device_sync(), "sync")
-        times[mode] = (time.perf_counter() - t0) / a.chem_reps
+        return (time.perf_counter() - t0) / a.chem_reps
+
+    def zone_step(sur, mode, buf, st=None):
+        L.check(lib.ostc_chem_zone_step(sim.ctx.h, z.h, sur.h if sur else None, mode, buf.p, st, None),
+                "zone chemistry")
+
+    t_plain = timed(lambda: zone_step(None, 0, dy_p))
     flop = plain_chem_flop_per_cell(case.param)
     rec["plain_chemistry"] = {
-        "eval_iteration": a.chem_eval_iter, "ms": round(times[0] * 1e3, 4),
-        "Mcell_s": round(cells / times[0] / 1e6, 2), "flop_per_cell": flop,
-        "fp64_TFLOPs": round(flop * cells / times[0] / 1e12, 3), "fp64_peak_TFLOPs": FP64_PEAK_TFLOPS}
-    if sur:
-        run(1, dy_s, stats)
-        ref = dy_p.get(np.float64, (cells, 9))
-        got = dy_s.get(np.float64, (cells, 9))
-        rep = S.error_report(got, ref, list(stats))
-        rec["surrogate"] = {
-            "weights": os.path.basename(weights),
-            "plain_ms": round(times[0] * 1e3, 4), "surrogate_ms": round(times[1] * 1e3, 4),
-            "speedup": round(times[0] / times[1], 3),
-            "surrogate_Mcell_s": round(cells / times[1] / 1e6, 2),
-            "surrogate_TFLOPs_incl_fallback": round(SUR_FLOP_PER_CELL * cells / times[1] / 1e12, 2),
-            "mfma_peak_TFLOPs_bf16": BF16_PEAK_TFLOPS,
-            **{k: (round(v, 6) if isinstance(v, float) else v) for k, v in rep.items()},
-            "error_definition": "rel_l2 = ||dY_sur - dY_plain||_2 / ||dY_plain||_2 over all cells and species "
-                                "(dY/dt is the effective production rate); cell_* = max_k|err_k| / "
-                                "max_k|dY_plain_k| over cells with max|dY_plain| > 1e-8",
-        }
+        "eval_iteration": a.chem_eval_iter, "ms": round(t_plain * 1e3, 4),
+        "Mcell_s": round(cells / t_plain / 1e6, 2), "flop_per_cell": flop,
+        "fp64_TFLOPs": round(flop * cells / t_plain / 1e12, 3), "fp64_peak_TFLOPs": FP64_PEAK_TFLOPS}
+    ref = dy_p.get(np.float64, (cells, 9))
+
+    def compare(weights):
+        """the surrogate with fallback against the plain step on the same cells"""
+        sur = S.Surrogate.load(sim.ctx, weights)
+        t_sur = timed(lambda: zone_step(sur, 1, dy_s))
+        stats = (C.c_longlong * 4)()
+        zone_step(sur, 1, dy_s, stats)
+        rep = S.error_report(dy_s.get(np.float64, (cells, 9)), ref, list(stats))
+        act = "hard-swish" if sur.desc.activation == S.ACT_HSWISH else "GELU"
         sur.close()
+        return {"weights": os.path.basename(weights), "activation": act,
+                "plain_ms": round(t_plain * 1e3, 4), "surrogate_ms": round(t_sur * 1e3, 4),
+                "speedup": round(t_plain / t_sur, 3), "surrogate_Mcell_s": round(cells / t_sur / 1e6, 2),
+                "surrogate_TFLOPs_incl_fallback": round(SUR_FLOP_PER_CELL * cells / t_sur / 1e12, 2),
+                "mfma_peak_TFLOPs_bf16": BF16_PEAK_TFLOPS,
+                **{k: (round(v, 6) if isinstance(v, float) else v) for k, v in rep.items()}}
+
+    weights = a.surrogate or S.DEFAULT_WEIGHTS
+    if os.path.exists(weights):
+        rec["surrogate"] = compare(weights)
+        rec["surrogate"]["error_definition"] = (
+            "rel_l2 = ||dY_sur - dY_plain||_2 / ||dY_plain||_2 over all cells and species (dY/dt is the effective "
+            "production rate); cell_* = max_k|err_k| / max_k|dY_plain_k| over cells with max|dY_plain| > 1e-8")
+    if not a.surrogate and os.path.exists(S.HSWISH_WEIGHTS):
+        rec["surrogate_hswish_mixing_only"] = compare(S.HSWISH_WEIGHTS)
     for b in (dy_p, dy_s):
         b.free()
     sim.close()

@@ -165,8 +165,59 @@ __device__ __forceinline__ void face_der(const R *__restrict__ f, int64_t id0, c
   }
 }
 
-// Viscous face flux (src/ViscousScheme.cuh:22-219 and the gv/hv copies).
+// Species part of the viscous face flux of a mixture (src/ViscousScheme.cuh:
+// 158-215): the diffusion fluxes with the correction velocity and the
+// enthalpy they carry.  fv[0] receives the energy term, fv[1 + l] species l.
 template <typename R, int NS, int NT, int DIR>
+__device__ __forceinline__ void viscous_face_species(const ZoneView<R> &z, const ostc_param *__restrict__ p,
+                                                     int64_t id0, const int64_t s[3], R *__restrict__ fv) {
+  const int64_t id1 = id0 + s[DIR];
+  const int64_t P = z.plane;
+  R mm[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) mm[q] = (R)0.5 * (z.metric[q * P + id0] + z.metric[q * P + id1]);
+  const R xi_x = mm[0], xi_y = mm[1], xi_z = mm[2];
+  const R eta_x = mm[3], eta_y = mm[4], eta_z = mm[5];
+  const R zeta_x = mm[6], zeta_y = mm[7], zeta_z = mm[8];
+  const int64_t mo = (int64_t)(DIR * 3) * P;
+  const R ja = z.jac[id0], jb = z.jac[id1];
+  const R kx = (R)0.5 * (z.metric[mo + id0] * ja + z.metric[mo + id1] * jb);
+  const R ky = (R)0.5 * (z.metric[mo + P + id0] * ja + z.metric[mo + P + id1] * jb);
+  const R kz = (R)0.5 * (z.metric[mo + 2 * P + id0] * ja + z.metric[mo + 2 * P + id1] * jb);
+  const R *svp = z.sv;
+  R turb_d = 0;
+  if constexpr (NT > 0) turb_d = fdiv<R>((R)0.5 * (z.mut[id0] + z.mut[id1]), (R)p->Sct);
+  R h[NS], dif[NS], yx[NS], yy[NS], yz[NS];
+  const R tm = (R)0.5 * (z.bv[5 * P + id0] + z.bv[5 * P + id1]);
+  enthalpy<R>(tm, h, p, NS);
+  R rho_uc = 0, rho_vc = 0, rho_wc = 0, e = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    dif[l] = (R)0.5 * (z.rho_d[l * P + id0] + z.rho_d[l * P + id1]) + turb_d;
+    R dy[3];
+    face_der<R, DIR>(svp + l * P, id0, s, dy);
+    yx[l] = dy[0] * xi_x + dy[1] * eta_x + dy[2] * zeta_x;
+    yy[l] = dy[0] * xi_y + dy[1] * eta_y + dy[2] * zeta_y;
+    yz[l] = dy[0] * xi_z + dy[1] * eta_z + dy[2] * zeta_z;
+    const R dh = dif[l] * h[l];
+    e += kx * dh * yx[l];
+    e += ky * dh * yy[l];
+    e += kz * dh * yz[l];
+    rho_uc += dif[l] * yx[l];
+    rho_vc += dif[l] * yy[l];
+    rho_wc += dif[l] * yz[l];
+  }
+  fv[0] = e;
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+    fv[1 + l] = dif[l] * (kx * yx[l] + ky * yy[l] + kz * yz[l]) -
+                (R)0.5 * (svp[l * P + id0] + svp[l * P + id1]) * (kx * rho_uc + ky * rho_vc + kz * rho_wc);
+}
+
+// Viscous face flux (src/ViscousScheme.cuh:22-219 and the gv/hv copies).
+// SPECIES = false leaves the mixture's species terms to viscous_face_species
+// (fv[5 .. 5+NS) = 0 and no enthalpy diffusion in fv[4]).
+template <typename R, int NS, int NT, int DIR, bool SPECIES = true>
 __device__ __forceinline__ void viscous_face(const ZoneView<R> &z, const ostc_param *__restrict__ p, int64_t id0,
                                              const int64_t s[3], R *__restrict__ fv) {
   constexpr int NV = 5 + NS + NT;
@@ -226,7 +277,10 @@ __device__ __forceinline__ void viscous_face(const ZoneView<R> &z, const ostc_pa
   R cond = (R)0.5 * (z.lambda[id0] + z.lambda[id1]);
   if constexpr (NT > 0) cond += (R)0.5 * (z.lambda_t[id0] + z.lambda_t[id1]);
   fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + cond * (kx * t_x + ky * t_y + kz * t_z);
-  if constexpr (NS > 0) {
+  if constexpr (NS > 0 && !SPECIES) {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) fv[5 + l] = 0;
+  } else if constexpr (NS > 0) {
     R turb_d = 0;
     if constexpr (NT > 0) turb_d = fdiv<R>(mut, (R)p->Sct);
     R h[NS], dif[NS], yx[NS], yy[NS], yz[NS];
@@ -334,12 +388,38 @@ __global__ __launch_bounds__(256) void sweep_kernel(ZoneView<R> z, const ostc_pa
     }
     if (VISC) {
       R v[NV];
-      viscous_face<R, NS, NT, DIR>(z, p, id0, s, v);
+      // mixtures: the species terms in a second pass over the faces below,
+      // so that the two halves of the viscous flux never hold registers at
+      // the same time
+      viscous_face<R, NS, NT, DIR, (NS == 0)>(z, p, id0, s, v);
 #pragma unroll
       for (int l = 0; l < NV; ++l) g[l] -= v[l];
     }
 #pragma unroll
     for (int l = 0; l < NV; ++l) fc[l * nfaces + q] = g[l];
+  }
+  if constexpr (VISC && NS > 0) {
+    // same face -> thread mapping as above: each thread updates its own faces
+    for (int q = threadIdx.x; q < nfaces; q += blockDim.x) {
+      int a, f;
+      if (DIR == 0) {
+        a = q / (L + 1);
+        f = q - a * (L + 1) - 1;
+      } else {
+        f = q / TW - 1;
+        a = q - (f + 1) * TW;
+      }
+      int c[3] = {o[0], o[1], o[2]};
+      c[A] += a;
+      c[DIR] += f;
+      if (c[A] >= ext[A] || c[DIR] >= ext[DIR]) continue;
+      const int64_t id0 = z.idx(c[0], c[1], c[2]);
+      R v[1 + NS];
+      viscous_face_species<R, NS, NT, DIR>(z, p, id0, s, v);
+      fc[4 * nfaces + q] -= v[0];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) fc[(5 + l) * nfaces + q] -= v[1 + l];
+    }
   }
   __syncthreads();
   const int ncells = TW * L;

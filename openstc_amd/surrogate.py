@@ -3,8 +3,8 @@
 No counterpart in the reference.  The surrogate stands in for the plain
 point-implicit chemistry step (``ostc_chem_plain_step``: (I - dt J) d(rhoY) =
 dt omega at fixed T, p) of the 9-species H2-air mechanism.  States are
-sampled from the solver's own trajectories of the finite-rate mixing-layer
-workload, labelled by that step on the GPU, and the network (12-128x4-n_out,
+sampled from the solver's own trajectories of the finite-rate workloads (and
+widened over the T/p/dt box), labelled by that step on the GPU, and the network (12-128x4-n_out,
 GELU or hard-swish hidden activation) is trained on the GPU by the library's forward/backward kernels with
 Adam on FP32 master weights.  Inference runs on BF16 MFMA with per-cell
 fallback to the plain step.
@@ -29,8 +29,12 @@ ACT_GELU, ACT_HSWISH = 0, 1
 ACTIVATIONS = {"gelu": ACT_GELU, "hswish": ACT_HSWISH}
 N_SPEC = 9
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
-DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2air9.npz")        # mixing-layer states
-JET_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2jet9.npz")            # four-block H2-jet states
+# GELU, trained on the mixing-layer and four-block jet states plus states
+# widened to T 300-2800 K, p 0.5-3 atm
+DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2air9.npz")
+# hard-swish, trained on the mixing-layer states only (faster activation,
+# narrower trusted box)
+HSWISH_WEIGHTS = os.path.join(DATA_DIR, "surrogate_h2air9_mixing_hswish.npz")
 ATM = 101325.0
 
 
