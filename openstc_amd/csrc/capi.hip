@@ -1,6 +1,7 @@
 // C-ABI of the residual-and-update path: contexts, zones, field I/O and the
 // per-stage entry points declared in include/ostc.h.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -264,6 +265,16 @@ void make_chem64(const ostc_param &p, ChemF64 &out) {
       if (f != 0 || b != 0) out.part[r] |= 1u << l;
       if (f != b) out.net[r] |= 1u << l;
     }
+  for (int i = 0; i < p.n_spec && i < kMaxSpec; ++i) {
+    for (int r = 0; r < 2; ++r) {
+      const double *c = r == 0 ? &p.low_temp_coeff[i * 7] : &p.high_temp_coeff[i * 7];
+      const double g[7] = {c[0] - c[6], -c[0], -0.5 * c[1], -c[2] / 6.0, -c[3] / 12.0, -c[4] * 0.05, c[5]};
+      for (int q = 0; q < 7; ++q) out.g[r][i][q] = g[q];
+    }
+    const double tt = p.t_low[i], *c = &p.low_temp_coeff[i * 7];
+    out.g_low[i] = c[0] * (1.0 - std::log(tt)) - 0.5 * c[1] * tt - c[2] * tt * tt / 6.0 -
+                   c[3] * tt * tt * tt / 12.0 - c[4] * tt * tt * tt * tt * 0.05 + c[5] / tt - c[6];
+  }
 }
 }  // namespace ostc
 

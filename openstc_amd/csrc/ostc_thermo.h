@@ -39,6 +39,11 @@ struct ChemF64 {
   double inv_mw[kMaxSpec];
   double nu_f[kMaxReac * kMaxSpec], nu_b[kMaxReac * kMaxSpec];
   unsigned part[kMaxReac], net[kMaxReac];
+  // g_k/(R T) of the NASA-7 fit per range [low/high]: (a0 - a6, -a0, -a1/2,
+  // -a2/6, -a3/12, -a4/20, a5): g = c0 + c1 ln T + T(c2 + T(c3 + T(c4 + T c5)))
+  // + c6 / T; below T_low the value at T_low
+  double g[2][kMaxSpec][7];
+  double g_low[kMaxSpec];
 };
 constexpr size_t kChemF64Offset = (kThermoF32Offset + sizeof(ThermoF32) + 255) / 256 * 256;
 constexpr size_t kParamBlockBytes = kChemF64Offset + sizeof(ChemF64);
@@ -222,13 +227,20 @@ __device__ __forceinline__ void mixture_transport<float>(const ostc_param *__res
     const float ci = 1.147f * powf(t_dl, -0.145f) + 1.0f / ((t_dl + 0.5f) * (t_dl + 0.5f));
     vis[l] = tf->vis_coeff[l] * sqrt_t / ci;
   }
+  // sqrt(mu_m / mu_n) as sqrt(mu_m) * (1 / sqrt(mu_n)): one square root and
+  // one reciprocal per species instead of one of each per pair
+  float sq[kMaxSpec], isq[kMaxSpec];
+  for (int l = 0; l < ns; ++l) {
+    sq[l] = sqrtf(vis[l]);
+    isq[l] = 1.0f / sq[l];
+  }
   float viscosity = 0, conductivity = 0;
   for (int m = 0; m < ns; ++m) {
     float vis_temp = 0;
     for (int n = 0; n < ns; ++n) {
       float pf = 1;
       if (m != n) {
-        const float num = 1.0f + sqrtf(vis[m] / vis[n]) * tf->wj_wi4[m * kMaxSpec + n];
+        const float num = 1.0f + sq[m] * isq[n] * tf->wj_wi4[m * kMaxSpec + n];
         pf = num * num * tf->sqrt_wi_wj8[m * kMaxSpec + n];
       }
       vis_temp += pf * x[n];
@@ -306,23 +318,20 @@ __device__ __forceinline__ double arrhenius(double t_inv, double log_t, double A
   return b == 0.0 ? A * exp(-Ea * t_inv) : A * exp(b * log_t - Ea * t_inv);
 }
 
-// g_k/(R T) (src/Thermo.cu:63-79, evaluated at T_low below T_low)
+// g_k/(R T) (src/Thermo.cu:63-79, evaluated at T_low below T_low), from the
+// derived coefficients (no divisions; ln T and 1/T from the caller)
 template <int NS>
-__device__ __forceinline__ void gibbs_div_rt(double t, const ostc_param *__restrict__ p, double *g) {
-  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t_inv = 1 / t, log_t = log(t);
+__device__ __forceinline__ void gibbs_div_rt(double t, double log_t, double t_inv, const ostc_param *__restrict__ p,
+                                             double *g) {
+  const ChemF64 *ct = chem64(p);
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    if (t < p->t_low[i]) {
-      const double tt = p->t_low[i];
-      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt_inv = 1 / tt, log_tt = log(tt);
-      const double *c = &p->low_temp_coeff[i * 7];
-      g[i] = c[0] * (1.0 - log_tt) - 0.5 * c[1] * tt - c[2] * tt2 / 6.0 - c[3] * tt3 / 12.0 - c[4] * tt4 * 0.05 +
-             c[5] * tt_inv - c[6];
-    } else {
-      const double *c = t < p->t_mid[i] ? &p->low_temp_coeff[i * 7] : &p->high_temp_coeff[i * 7];
-      g[i] = c[0] * (1.0 - log_t) - 0.5 * c[1] * t - c[2] * t2 / 6.0 - c[3] * t3 / 12.0 - c[4] * t4 * 0.05 +
-             c[5] * t_inv - c[6];
-    }
+    const bool hi = t >= p->t_mid[i];
+    double c[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) c[q] = hi ? ct->g[1][i][q] : ct->g[0][i][q];
+    const double v = c[0] + c[1] * log_t + t * (c[2] + t * (c[3] + t * (c[4] + t * c[5]))) + c[6] * t_inv;
+    g[i] = t < p->t_low[i] ? ct->g_low[i] : v;
   }
 }
 
@@ -417,8 +426,8 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
   double c[NS], g[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) c[l] = rho * y[l] * ct->inv_mw[l] * 1e-3;
-  gibbs_div_rt<NS>(t, p, g);
   const double log_t = log(t), t_inv = 1.0 / t;
+  gibbs_div_rt<NS>(t, log_t, t_inv, p, g);
   const double temp_t = kPatm / kRu * 1e-3 * t_inv, temp_t_inv = 1.0 / temp_t;
   // the Jacobian sums accumulate in the caller's array and are scaled in place
   double creation[NS], od[NS];
@@ -495,8 +504,8 @@ __device__ __forceinline__ void chem_source_rows(const ostc_param *__restrict__ 
     double c[NS], g[NS];
 #pragma unroll
     for (int l = 0; l < NS; ++l) c[l] = rho * y[l] * ct->inv_mw[l] * 1e-3;
-    gibbs_div_rt<NS>(t, p, g);
     const double log_t = log(t), t_inv = 1.0 / t;
+    gibbs_div_rt<NS>(t, log_t, t_inv, p, g);
     const double temp_t = kPatm / kRu * 1e-3 * t_inv, temp_t_inv = 1.0 / temp_t;
     // fully unrolled (q1/q2 at compile-time indices stay in registers)
 #pragma clang loop unroll(full)
