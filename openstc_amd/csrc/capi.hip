@@ -111,6 +111,12 @@ struct ostc_ctx {
   size_t partial_cap = 0;      // in doubles
   double *out4 = nullptr;      // device [4]
   double *host4 = nullptr;     // pinned host [4]
+  // ostc_steady_step on several zones: one stream per zone (default
+  // priority) forked from and joined back to the caller's stream, so that
+  // the kernels of independent zones can share the GPU
+  std::vector<hipStream_t> zstream;
+  std::vector<hipEvent_t> zdone;
+  hipEvent_t fork = nullptr;
 };
 
 struct ostc_zone {
@@ -349,6 +355,9 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
 
 int ostc_ctx_destroy(ostc_ctx *ctx) {
   if (!ctx) return OSTC_OK;
+  for (auto e : ctx->zdone) (void)hipEventDestroy(e);
+  for (auto q : ctx->zstream) (void)hipStreamDestroy(q);
+  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
   if (ctx->dev) (void)hipFree(ctx->dev);
   if (ctx->partial) (void)hipFree(ctx->partial);
   if (ctx->out4) (void)hipFree(ctx->out4);
@@ -695,71 +704,107 @@ int ostc_zone_counters(ostc_zone *z, long long out[4], int reset, void *s) {
   return OSTC_OK;
 }
 
+// Zone streams for ostc_steady_step (created on first use, n of them)
+static int zone_streams(ostc_ctx *c, int n) {
+  if (!c->fork) HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+  while ((int)c->zstream.size() < n) {
+    hipStream_t q = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    c->zstream.push_back(q);
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->zdone.push_back(e);
+  }
+  return OSTC_OK;
+}
+
+// Everything of one iteration that touches only zone z (src/Driver.cu:
+// 224-257): source, fluxes, local dt, implicit treatment, update,
+// limit_flow, boundary conditions.
+static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen, hipStream_t st) {
+  const CellOpts &o = c->opts;
+  const bool visc = c->host.viscous_scheme != 0;
+  void *s = (void *)st;
+  int r = OSTC_OK;
+  if (step % output_screen == 0) {
+    prof_mark(P_STORE, st, true);
+    r = ostc_store_last_step(c, z, s);
+    prof_mark(P_STORE, st, false);
+  }
+  if (r) return r;
+  bool wrote = false;
+  // SST zones without chemistry: the source kernel also forms the local
+  // time step (it holds the metric, density, viscosity and mut)
+  const bool dt_in_source = source_has_dt(o);
+  prof_mark(P_SOURCE, st, true);
+  r = ZONE_CALL(z, source_terms(z->vf, c->dev, o, st, false, &wrote, dt_in_source),
+                source_terms(z->vd, c->dev, o, st, false, &wrote, dt_in_source));
+  prof_mark(P_SOURCE, st, false);
+  if (r) return r;
+  r = ZONE_CALL(z, flux_sweeps(z->vf, c->dev, o.ns, o.nt, true, visc, !wrote, st),
+                flux_sweeps(z->vd, c->dev, o.ns, o.nt, true, visc, !wrote, st));
+  if (r) return r;
+  if (!dt_in_source) {
+    prof_mark(P_DT, st, true);
+    r = ZONE_CALL(z, local_time_step(z->vf, c->dev, o, st), local_time_step(z->vd, c->dev, o, st));
+    prof_mark(P_DT, st, false);
+    if (r) return r;
+  }
+  if (o.implicit_method == 0) prof_mark(P_EXPL_IMPL, st, true);
+  {
+    // implicit treatment, then the update with the T-from-e solve
+    const bool dplur = o.implicit_method == 1;
+    if (fp64(z)) {
+      double *res = nullptr;
+      r = implicit_treatment(z->vd, c->dev, o, st, &res);
+      if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
+      if (r) return r;
+      prof_mark(P_UPDATE, st, true);
+      r = update_cv_bv(z->vd, c->dev, o, res, st, z->lf_slot);
+    } else {
+      float *res = nullptr;
+      r = implicit_treatment(z->vf, c->dev, o, st, &res);
+      if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
+      if (r) return r;
+      prof_mark(P_UPDATE, st, true);
+      r = update_cv_bv(z->vf, c->dev, o, res, st, z->lf_slot);
+    }
+    prof_mark(P_UPDATE, st, false);
+  }
+  if (r) return r;
+  prof_mark(P_LIMIT, st, true);
+  r = ZONE_CALL(z, limit_flow(z->vf, c->dev, o, st, z->lf_slot), limit_flow(z->vd, c->dev, o, st, z->lf_slot));
+  prof_mark(P_LIMIT, st, false);
+  if (r) return r;
+  // the update cleared the other slot for the next step's list
+  z->lf_slot ^= 1;
+  prof_mark(P_BC, st, true);
+  r = ostc_apply_bc(c, z, s);
+  prof_mark(P_BC, st, false);
+  return r;
+}
+
 int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_face *inner, const int *n_inner,
                      int step, int output_screen, double res_sums[4], int *res_computed, void *s) {
   if (!c || !zones || n <= 0 || output_screen <= 0) return fail(OSTC_ERR_ARG, "bad arguments");
   const hipStream_t st = S(s);
-  const CellOpts &o = c->opts;
-  const bool visc = c->host.viscous_scheme != 0;
-  for (int b = 0; b < n; ++b) {
-    ostc_zone *z = zones[b];
-    int r = OSTC_OK;
-    if (step % output_screen == 0) {
-      prof_mark(P_STORE, st, true);
-      r = ostc_store_last_step(c, z, s);
-      prof_mark(P_STORE, st, false);
+  // Several zones: each zone's chain on its own stream, forked from the
+  // caller's stream and joined back before the inner-face exchange (the
+  // zones share nothing until then).  With the stage profiler on, the zones
+  // run one after another on the caller's stream, so that the stage events
+  // bracket one kernel at a time.
+  if (n > 1 && !g_prof) {
+    if (int r = zone_streams(c, n)) return r;
+    HIP_TRY(hipEventRecord(c->fork, st));
+    for (int b = 0; b < n; ++b) {
+      HIP_TRY(hipStreamWaitEvent(c->zstream[b], c->fork, 0));
+      if (int r = zone_iteration(c, zones[b], step, output_screen, c->zstream[b])) return r;
+      HIP_TRY(hipEventRecord(c->zdone[b], c->zstream[b]));
     }
-    if (r) return r;
-    bool wrote = false;
-    // SST zones without chemistry: the source kernel also forms the local
-    // time step (it holds the metric, density, viscosity and mut)
-    const bool dt_in_source = source_has_dt(o);
-    prof_mark(P_SOURCE, st, true);
-    r = ZONE_CALL(z, source_terms(z->vf, c->dev, o, st, false, &wrote, dt_in_source),
-                  source_terms(z->vd, c->dev, o, st, false, &wrote, dt_in_source));
-    prof_mark(P_SOURCE, st, false);
-    if (r) return r;
-    r = ZONE_CALL(z, flux_sweeps(z->vf, c->dev, o.ns, o.nt, true, visc, !wrote, st),
-                  flux_sweeps(z->vd, c->dev, o.ns, o.nt, true, visc, !wrote, st));
-    if (r) return r;
-    if (!dt_in_source) {
-      prof_mark(P_DT, st, true);
-      r = ZONE_CALL(z, local_time_step(z->vf, c->dev, o, st), local_time_step(z->vd, c->dev, o, st));
-      prof_mark(P_DT, st, false);
-      if (r) return r;
-    }
-    if (o.implicit_method == 0) prof_mark(P_EXPL_IMPL, st, true);
-    {
-      // implicit treatment, then the update with the T-from-e solve
-      const bool dplur = o.implicit_method == 1;
-      if (fp64(z)) {
-        double *res = nullptr;
-        r = implicit_treatment(z->vd, c->dev, o, st, &res);
-        if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
-        if (r) return r;
-        prof_mark(P_UPDATE, st, true);
-        r = update_cv_bv(z->vd, c->dev, o, res, st, z->lf_slot);
-      } else {
-        float *res = nullptr;
-        r = implicit_treatment(z->vf, c->dev, o, st, &res);
-        if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
-        if (r) return r;
-        prof_mark(P_UPDATE, st, true);
-        r = update_cv_bv(z->vf, c->dev, o, res, st, z->lf_slot);
-      }
-      prof_mark(P_UPDATE, st, false);
-    }
-    if (r) return r;
-    prof_mark(P_LIMIT, st, true);
-    r = ZONE_CALL(z, limit_flow(z->vf, c->dev, o, st, z->lf_slot), limit_flow(z->vd, c->dev, o, st, z->lf_slot));
-    prof_mark(P_LIMIT, st, false);
-    if (r) return r;
-    // the update cleared the other slot for the next step's list
-    z->lf_slot ^= 1;
-    prof_mark(P_BC, st, true);
-    r = ostc_apply_bc(c, z, s);
-    prof_mark(P_BC, st, false);
-    if (r) return r;
+    for (int b = 0; b < n; ++b) HIP_TRY(hipStreamWaitEvent(st, c->zdone[b], 0));
+  } else {
+    for (int b = 0; b < n; ++b)
+      if (int r = zone_iteration(c, zones[b], step, output_screen, st)) return r;
   }
   if (inner && n_inner) {
     prof_mark(P_HALO, st, true);
