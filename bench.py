@@ -44,15 +44,34 @@ METRIC = "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem
 # HBM bytes per launch of a stage's kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE on these workloads (FETCH_SIZE scaled x2.0 and
 # WRITE_SIZE x1.0, the factors a 512 MiB copy kernel gives).  Keyed by
-# (workload, stage); the summaries with the commands are under profiles/.
+# (workload, stage); the summary with the commands is
+# profiles/kernels_fourblock_jet_4x256x192x128_plate_mixing_layer.md.
 # Re-measured whenever the kernel changes; a stage without an entry reports
 # null.
 MEASURED_TRAFFIC = {
-    ("sst_flat_plate_dplur", "dplur_sweep"): 478.8e6,
-    ("sst_flat_plate_dplur", "dplur_sweep_first"): 494.3e6,
-    ("sst_flat_plate_dplur", "flux_ij"): 402.7e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2339.8e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 5095.8e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 9737.0e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 8562.8e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3912.8e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4760.6e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3504.5e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.6e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "local_dt"): 537.3e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 543.6e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1441.5e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 2715.1e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "source"): 2190.3e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 701.7e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 824.0e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 744.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "local_dt"): 144.9e6,
+    ("sst_flat_plate_dplur", "dplur_sweep"): 480.5e6,
+    ("sst_flat_plate_dplur", "dplur_sweep_first"): 491.4e6,
+    ("sst_flat_plate_dplur", "flux_ij"): 402.4e6,
     ("sst_flat_plate_dplur", "flux_k"): 402.3e6,
-    ("sst_flat_plate_dplur", "source"): 354.7e6,
+    ("sst_flat_plate_dplur", "source"): 354.0e6,
     ("sst_flat_plate_dplur", "update"): 273.5e6,
 }
 
