@@ -559,7 +559,7 @@ def mixture_param(mech: L.Param, rans: bool, implicit: bool, dplur_steps: int, c
 
 
 def mixing_layer(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=4, cfl=0.2, chem_method=1,
-            lx=0.2, ly=0.1, lz=0.1, seed=3) -> Case:
+            lx=0.2, ly=0.1, lz=0.1, seed=3, mech_dir: str = DATA) -> Case:
     """Finite-rate chemistry: 9-species/19-reaction H2-air, Cartesian box with
     a seeded mixing layer: phi = (1+tanh((y-y0)/(2*dy)))/2 between H2 (Y=1,
     1000 K, Ma 1.2, below y0) and air (O2 0.233 / N2 0.767, 1500 K, Ma 2.0,
@@ -569,9 +569,10 @@ def mixing_layer(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=
     CFL 0.2: the layer ignites (T up to ~2800 K) and burns stably through
     500 iterations; at CFL 1 the local time step overshoots the heat release
     at ignition within three iterations and limit_flow clamps the whole field
-    (the oracle does the same: the reference's algorithm, not a GPU effect)."""
+    (the oracle does the same: the reference's algorithm, not a GPU effect).
+    mech_dir: another CHEMKIN mechanism with H2, O2 and N2 among its species."""
     ngg = 2
-    mech, info = load_mechanism(True)
+    mech, info = load_mechanism(True, mech_dir)
     names = species_names(info)
     ns = mech.n_spec
     y_air = np.zeros(ns)
@@ -632,7 +633,7 @@ def mixing_layer(mx=192, my=96, mz=96, steps=500, precision=L.FP32, dplur_steps=
 
 
 def jet_crossflow(mx=256, my=192, mz=128, nblocks=4, steps=200, precision=L.FP32, dplur_steps=4, cfl=0.2, chem_method=1,
-            lx=0.4, ly=0.1, lz=0.05, seed=5) -> Case:
+            lx=0.4, ly=0.1, lz=0.05, seed=5, mech_dir: str = DATA) -> Case:
     """Full path at scale: SST RANS + 9-species finite-rate chemistry on
     `nblocks` blocks of mx x my x mz joined along i by inner faces (shared node
     planes, the reference's inner{rank}.txt records).  Vitiated-air crossflow
@@ -648,7 +649,7 @@ def jet_crossflow(mx=256, my=192, mz=128, nblocks=4, steps=200, precision=L.FP32
     ignition overshoots within 20-30 iterations and limit_flow clamps the
     field (the oracle does the same)."""
     ngg = 2
-    mech, info = load_mechanism(True)
+    mech, info = load_mechanism(True, mech_dir)
     names = species_names(info)
     ns = mech.n_spec
     y_air = np.zeros(ns)

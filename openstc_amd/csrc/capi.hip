@@ -104,7 +104,11 @@ void prof_mark(int stage, hipStream_t st, bool begin) {
 }  // namespace ostc
 
 struct ostc_ctx {
+  // host copy of the parameter block the kernels use: a mechanism of fewer
+  // than kSpeciesBucket species is padded to that many (see pad_species)
   ostc_param host{};
+  int user_ns = 0;  // species of the caller's mechanism
+  int pad = 0;      // inert species added after them
   ostc_param *dev = nullptr;
   CellOpts opts{};
   double *partial = nullptr;  // residual partial sums
@@ -196,11 +200,15 @@ ZoneView<R> make_view(ostc_zone *z) {
 bool fp64(const ostc_zone *z) { return z->desc.precision == OSTC_FP64; }
 hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
 
+// species count the mixture kernels are instantiated for
+constexpr int kSpeciesBucket = 9;
+
 int check_param(const ostc_param &p) {
   if (p.mix_model < 0 || p.mix_model > 2) return fail(OSTC_ERR_ARG, "mix_model must be 0..2");
   if (p.turb_method == OSTC_RANS && p.rans_model != 2) return fail(OSTC_ERR_ARG, "only the k-omega SST RANS model");
-  if (p.mix_model != OSTC_AIR && p.n_spec != 9)
-    return fail(OSTC_ERR_ARG, "mixtures are instantiated for 9 species");
+  if (p.mix_model != OSTC_AIR && (p.n_spec < 1 || p.n_spec > kSpeciesBucket))
+    return fail(OSTC_ERR_ARG, "mixtures of 1 to 9 species (the kernels are instantiated for 9; smaller "
+                              "mechanisms are padded with inert species)");
   if (p.mix_model == OSTC_AIR && p.n_spec != 0) return fail(OSTC_ERR_ARG, "air has no species");
   const int nt = p.turb_method == OSTC_RANS ? 2 : 0;
   if (p.n_scalar != p.n_spec + nt || p.n_var != 5 + p.n_spec + nt)
@@ -260,6 +268,47 @@ void make_thermo32(const ostc_param &p, ThermoF32 &out) {
     out.sqrt_wi_wj8[q] = (float)p.sqrt_WiDivWjPl1Mul8[q];
   }
 }
+// A mechanism of ns < 9 species runs on the 9-species kernels with inert
+// species ns..8 appended: zero mass fraction everywhere, no reactions, zero
+// NASA-7 coefficients, W = 1 and a positive viscosity coefficient.  Every
+// term they enter is an exact zero (a product with their mass fraction,
+// concentration or gradient, or a zero stoichiometric coefficient), so the
+// real species see exactly the mechanism they were given.  The caller's
+// freestream scalars [Y_0..Y_ns-1, k, omega] move to [Y..., 0.., k, omega].
+ostc_param pad_species(const ostc_param &u, int bucket) {
+  ostc_param p = u;
+  const int ns = u.n_spec, nt = u.n_scalar - u.n_spec;
+  for (int l = ns; l < bucket; ++l) {
+    p.mw[l] = 1.0;
+    p.t_low[l] = 200.0;
+    p.t_mid[l] = 1000.0;
+    p.t_high[l] = 6000.0;
+    for (int q = 0; q < 7; ++q) p.low_temp_coeff[l * 7 + q] = p.high_temp_coeff[l * 7 + q] = 0.0;
+    p.LJ_potent_inv[l] = 0.01;
+    p.vis_coeff[l] = 1e-6;
+  }
+  for (int m = 0; m < bucket; ++m)
+    for (int n = 0; n < bucket; ++n) {
+      if (m < ns && n < ns) continue;
+      // the Wilke factors of pairs with an inert species (finite; they only
+      // ever multiply its zero mole fraction)
+      p.WjDivWi_to_One4th[m * kMaxSpec + n] = std::pow(p.mw[n] / p.mw[m], 0.25);
+      p.sqrt_WiDivWjPl1Mul8[m * kMaxSpec + n] = 1.0 / std::sqrt(8.0 * (1.0 + p.mw[m] / p.mw[n]));
+    }
+  for (int r = 0; r < u.n_reac && r < kMaxReac; ++r)
+    for (int l = ns; l < bucket; ++l) {
+      p.stoi_f[r * kMaxSpec + l] = p.stoi_b[r * kMaxSpec + l] = 0;
+      p.third_body_coeff[r * kMaxSpec + l] = 0.0;
+    }
+  for (int l = 0; l < kMaxSpec + 2; ++l) p.sv_inf[l] = 0.0;
+  for (int l = 0; l < ns; ++l) p.sv_inf[l] = u.sv_inf[l];
+  for (int l = 0; l < nt; ++l) p.sv_inf[bucket + l] = u.sv_inf[ns + l];
+  p.n_spec = bucket;
+  p.n_scalar = bucket + nt;
+  p.n_var = 5 + bucket + nt;
+  return p;
+}
+
 void make_chem64(const ostc_param &p, ChemF64 &out) {
   std::memset(&out, 0, sizeof(out));
   for (int i = 0; i < p.n_spec && i < kMaxSpec; ++i) out.inv_mw[i] = p.mw[i] > 0 ? 1.0 / p.mw[i] : 0.0;
@@ -306,7 +355,10 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   if (int r = check_param(*param)) return r;
   auto *c = new (std::nothrow) ostc_ctx;
   if (!c) return fail(OSTC_ERR_ALLOC, "host allocation");
-  c->host = *param;
+  c->user_ns = param->n_spec;
+  c->pad = (param->mix_model != OSTC_AIR && param->n_spec < kSpeciesBucket) ? kSpeciesBucket - param->n_spec : 0;
+  c->host = c->pad ? pad_species(*param, kSpeciesBucket) : *param;
+  param = &c->host;
   // the parameter block, followed by its FP32 thermodynamic and FP64
   // chemistry tables
   hipError_t e = hipMalloc(&c->dev, kParamBlockBytes);
@@ -460,9 +512,52 @@ int ostc_zone_destroy(ostc_zone *z) {
 
 long ostc_zone_plane(const ostc_zone *z) { return z ? (long)z->plane : 0; }
 
+// Planes of field f as the caller sees them: with a padded mechanism the
+// species-indexed fields hold the caller's species count.
+static int user_planes(const ostc_zone *z, int f) {
+  const int pad = z->ctx->pad, n = z->planes[f];
+  if (!pad || n == 0) return n;
+  const int b = z->ctx->host.n_spec, u = z->ctx->user_ns;
+  switch (f) {
+    case OSTC_F_SV:
+    case OSTC_F_CV:
+    case OSTC_F_DQ:
+    case OSTC_F_DQ0:
+    case OSTC_F_DQK:
+    case OSTC_F_RHO_D:
+      return n - pad;
+    case OSTC_F_CHEM_JAC:
+      return n == b * b ? u * u : u;
+    default:
+      return n;
+  }
+}
+
+// The device plane of the caller's plane l of field f: species planes keep
+// their index, the planes after them (turbulence) move past the inert ones,
+// and the EPI Jacobian's (m, n) entry moves from m*ns + n to m*9 + n.
+static int device_plane(const ostc_zone *z, int f, int l) {
+  const int pad = z->ctx->pad;
+  if (!pad) return l;
+  const int b = z->ctx->host.n_spec, u = z->ctx->user_ns;
+  switch (f) {
+    case OSTC_F_SV:
+      return l < u ? l : l + pad;
+    case OSTC_F_CV:
+    case OSTC_F_DQ:
+    case OSTC_F_DQ0:
+    case OSTC_F_DQK:
+      return l < 5 + u ? l : l + pad;
+    case OSTC_F_CHEM_JAC:
+      return z->planes[f] == b * b ? (l / u) * b + (l % u) : l;
+    default:
+      return l;
+  }
+}
+
 int ostc_zone_field_planes(const ostc_zone *z, int f, int *n) {
   if (!z || f < 0 || f >= OSTC_F_COUNT || !n) return fail(OSTC_ERR_ARG, "bad field");
-  *n = z->planes[f];
+  *n = user_planes(z, f);
   return OSTC_OK;
 }
 
@@ -475,31 +570,39 @@ int ostc_zone_field_ptr(const ostc_zone *z, int f, void **dev, int *es) {
 
 int ostc_zone_upload(ostc_zone *z, int f, const double *host, int n_planes) {
   if (!z || f < 0 || f >= OSTC_F_COUNT || !host) return fail(OSTC_ERR_ARG, "bad field");
-  if (n_planes > z->planes[f] || n_planes < 0) return fail(OSTC_ERR_ARG, "too many planes for this field");
-  const size_t n = (size_t)z->plane * n_planes;
+  if (n_planes > user_planes(z, f) || n_planes < 0) return fail(OSTC_ERR_ARG, "too many planes for this field");
+  const size_t pn = (size_t)z->plane, n = pn * n_planes;
   if (n == 0) return OSTC_OK;
-  if (z->fsize[f] == 8) {
-    HIP_TRY(hipMemcpy(z->ptr[f], host, n * 8, hipMemcpyHostToDevice));
-  } else {
-    std::vector<float> tmp(n);
-    for (size_t q = 0; q < n; ++q) tmp[q] = (float)host[q];
-    HIP_TRY(hipMemcpy(z->ptr[f], tmp.data(), n * 4, hipMemcpyHostToDevice));
+  std::vector<float> tmp(z->fsize[f] == 8 ? 0 : pn);
+  for (int l = 0; l < n_planes; ++l) {
+    char *dst = static_cast<char *>(z->ptr[f]) + (size_t)device_plane(z, f, l) * pn * z->fsize[f];
+    const double *src = host + (size_t)l * pn;
+    if (z->fsize[f] == 8) {
+      HIP_TRY(hipMemcpy(dst, src, pn * 8, hipMemcpyHostToDevice));
+    } else {
+      for (size_t q = 0; q < pn; ++q) tmp[q] = (float)src[q];
+      HIP_TRY(hipMemcpy(dst, tmp.data(), pn * 4, hipMemcpyHostToDevice));
+    }
   }
   return OSTC_OK;
 }
 
 int ostc_zone_download(const ostc_zone *z, int f, double *host, int n_planes) {
   if (!z || f < 0 || f >= OSTC_F_COUNT || !host) return fail(OSTC_ERR_ARG, "bad field");
-  if (n_planes > z->planes[f] || n_planes < 0) return fail(OSTC_ERR_ARG, "too many planes for this field");
-  const size_t n = (size_t)z->plane * n_planes;
+  if (n_planes > user_planes(z, f) || n_planes < 0) return fail(OSTC_ERR_ARG, "too many planes for this field");
+  const size_t pn = (size_t)z->plane, n = pn * n_planes;
   if (n == 0) return OSTC_OK;
   HIP_TRY(hipDeviceSynchronize());
-  if (z->fsize[f] == 8) {
-    HIP_TRY(hipMemcpy(host, z->ptr[f], n * 8, hipMemcpyDeviceToHost));
-  } else {
-    std::vector<float> tmp(n);
-    HIP_TRY(hipMemcpy(tmp.data(), z->ptr[f], n * 4, hipMemcpyDeviceToHost));
-    for (size_t q = 0; q < n; ++q) host[q] = tmp[q];
+  std::vector<float> tmp(z->fsize[f] == 8 ? 0 : pn);
+  for (int l = 0; l < n_planes; ++l) {
+    const char *src = static_cast<const char *>(z->ptr[f]) + (size_t)device_plane(z, f, l) * pn * z->fsize[f];
+    double *dst = host + (size_t)l * pn;
+    if (z->fsize[f] == 8) {
+      HIP_TRY(hipMemcpy(dst, src, pn * 8, hipMemcpyDeviceToHost));
+    } else {
+      HIP_TRY(hipMemcpy(tmp.data(), src, pn * 4, hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < pn; ++q) dst[q] = tmp[q];
+    }
   }
   return OSTC_OK;
 }
@@ -525,8 +628,18 @@ int ostc_zone_set_bc(ostc_zone *z, const ostc_bc *bc) {
     if (b.kind != 2 && b.kind != 3 && b.kind != 5 && b.kind != 6) return fail(OSTC_ERR_ARG, "unknown BC kind");
   }
   z->bc = *bc;
+  if (z->ctx->pad) {
+    // inflow scalars [Y_0..Y_ns-1, k, omega] -> the padded layout
+    const int u = z->ctx->user_ns, b = z->ctx->host.n_spec, nt = z->ctx->host.n_scalar - b;
+    for (auto &in : z->bc.inflow) {
+      double sv[OSTC_MAX_SPEC + 2] = {};
+      for (int l = 0; l < u; ++l) sv[l] = in.sv[l];
+      for (int l = 0; l < nt; ++l) sv[b + l] = in.sv[u + l];
+      std::memcpy(in.sv, sv, sizeof(sv));
+    }
+  }
   z->has_bc = true;
-  HIP_TRY(hipMemcpy(z->d_inflow, bc->inflow, sizeof(bc->inflow), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(z->d_inflow, z->bc.inflow, sizeof(z->bc.inflow), hipMemcpyHostToDevice));
   return OSTC_OK;
 }
 
@@ -886,14 +999,14 @@ int ostc_profile_stage_count(void) { return P_COUNT; }
 int ostc_chem_plain_step(ostc_ctx *c, long n, const double *t, const double *p, const double *y, const double *dt,
                          double *dy, void *s) {
   if (!c || n < 0 || (n > 0 && (!t || !p || !y || !dt || !dy))) return fail(OSTC_ERR_ARG, "bad arguments");
-  if (c->host.n_spec != 9 || c->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
+  if (c->user_ns != 9 || c->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
   return chem_plain_step(c->dev, c->host.n_spec, c->host.n_reac, n, t, p, y, dt, dy, S(s));
 }
 
 int ostc_chem_source_batch(ostc_ctx *c, long n, const double *rho, const double *t, const double *y, double *omega,
                            double *jac, void *s) {
   if (!c || n < 0 || (n > 0 && (!rho || !t || !y || !omega))) return fail(OSTC_ERR_ARG, "bad arguments");
-  if (c->host.n_spec != 9) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
+  if (c->user_ns != 9) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
   return chem_source_batch(c->dev, c->host.n_spec, c->host.n_reac, n, rho, t, y, omega, jac, S(s));
 }
 
@@ -996,7 +1109,7 @@ int ostc_surrogate_create(ostc_ctx *ctx, const ostc_surrogate_desc *desc, const 
                           ostc_surrogate **out) {
   if (!ctx || !desc || !weights || !out || max_batch < 0 || max_batch % 16)
     return fail(OSTC_ERR_ARG, "bad arguments");
-  if (ctx->host.n_spec != 9 || ctx->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
+  if (ctx->user_ns != 9 || ctx->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
   int st = sur_check_desc(*desc);
   if (st) return st;
   auto *s = new (std::nothrow) ostc_surrogate;
@@ -1134,7 +1247,7 @@ int ostc_surrogate_train_step(ostc_surrogate *s, long nb, const double *t, const
 int ostc_chem_zone_step(ostc_ctx *c, ostc_zone *z, ostc_surrogate *s, int mode, double *dy, long long stats[4],
                         void *strm) {
   if (!c || !z || !dy || (mode != 0 && mode != 1) || (mode == 1 && !s)) return fail(OSTC_ERR_ARG, "bad arguments");
-  if (c->host.n_spec != 9 || c->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
+  if (c->user_ns != 9 || c->host.n_reac <= 0) return fail(OSTC_ERR_ARG, "needs the 9-species mechanism");
   const long n = (long)z->desc.mx * z->desc.my * z->desc.mz;
   int st;
   if (mode == 0) {
