@@ -338,7 +338,7 @@ extern "C" {
 const char *ostc_last_error(void) { return g_err.c_str(); }
 
 const char *ostc_build_info(void) {
-  return "openstc_amd libostc: HIP gfx950, FP32/FP64 zones, species instantiations {0, 9}";
+  return "openstc_amd libostc: HIP gfx950, FP32/FP64 zones, air and mixtures of 1-9 species (9-species kernels)";
 }
 
 int ostc_sizeof_param(void) { return (int)sizeof(ostc_param); }
