@@ -397,9 +397,9 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   o.turb_implicit = param->turb_implicit;
   o.implicit_method = param->implicit_method;
   o.dplur_steps = param->dplur_inner_step;
-  // launch tuning of the DPLUR sweep: the number of workgroups its k chunks
-  // are sized for (environment override)
-  o.dplur_wg = 1024;
+  // launch tuning of the DPLUR sweep: 0 sizes its k chunks from the device's
+  // resident workgroup capacity; the environment can fix a workgroup target
+  o.dplur_wg = 0;
   if (const char *e = std::getenv("OSTC_DPLUR_WG")) o.dplur_wg = std::max(64, std::atoi(e));
   *out = c;
   return OSTC_OK;

@@ -1228,6 +1228,50 @@ __global__ __launch_bounds__(256) void k_wall_dist(ZoneView<R> z, const double *
 
 // ------------------------------------------------------------------ launchers
 
+// Workgroups of a DPLUR sweep kernel resident on the whole current device at
+// once (registers and LDS decide it), cached per device.
+template <typename R, int NS, int NT, bool CHEM, int TJ, bool FIRST>
+int dplur_resident() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (cached[dev] > 0) return cached[dev];
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dplur<R, NS, NT, CHEM, TJ, FIRST>, 64 * TJ, 0) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  cached[dev] = per_cu * cus;
+  return cached[dev];
+}
+
+// Planes per k chunk of a DPLUR sweep over `tiles` (i, j) tiles.  The chunks
+// run in rounds of `resident` workgroups and a round lasts one chunk's march,
+// so the sweep takes about rounds x planes-per-chunk plane steps: the count
+// minimising that is taken (ties: fewer chunks, fewer boundary planes to
+// reload), with at least 4 planes per chunk.  resident <= 0 (unknown) falls
+// back to about 1024 workgroups.
+inline int dplur_chunk_planes(int tiles, int mz, int resident) {
+  const int nmax = std::max(1, (mz + 3) / 4);
+  if (resident <= 0) {
+    const int n = std::max(1, std::min((1024 + tiles - 1) / tiles, nmax));
+    return (mz + n - 1) / n;
+  }
+  int best = mz;
+  long long best_cost = -1;
+  for (int n = 1; n <= nmax; ++n) {
+    const int kc = (mz + n - 1) / n;
+    const long long chunks = (mz + kc - 1) / kc;
+    const long long rounds = ((long long)tiles * chunks + resident - 1) / resident;
+    const long long cost = rounds * kc;
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = kc;
+    }
+  }
+  return best;
+}
+
 #define OSTC_DISPATCH_NSNT(o, CALL)              \
   do {                                           \
     if ((o).ns == 0 && (o).nt == 0) {            \
@@ -1345,13 +1389,23 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
     prof_mark(P_DQ0, st, false);
     if (last_err()) return OSTC_ERR_HIP;
   }
-  // k-marching sweeps: (64 x 4) tiles, k split into chunks so that about
-  // `target` workgroups are in flight; at least 4 planes per chunk
+  // k-marching sweeps: (64 x 4) tiles, k split into chunks sized from the
+  // resident capacity of the kernel that runs most sweeps (or so that about
+  // o.dplur_wg workgroups are in flight, when set); at least 4 planes per chunk
   constexpr int tj = 4;
-  const int target = o.dplur_wg > 0 ? o.dplur_wg : 1024;
   const int tiles = ((z.mx + 63) / 64) * ((z.my + tj - 1) / tj);
-  const int nkc = std::max(1, std::min((target + tiles - 1) / tiles, (z.mz + 3) / 4));
-  const int kc = (z.mz + nkc - 1) / nkc;
+  int kc;
+  if (o.dplur_wg > 0) {
+    const int nkc = std::max(1, std::min((o.dplur_wg + tiles - 1) / tiles, (z.mz + 3) / 4));
+    kc = (z.mz + nkc - 1) / nkc;
+  } else {
+    int resident = 0;
+    if (chem)
+      OSTC_DISPATCH_NSNT(o, resident = (dplur_resident<R, NS_, NT_, true, tj, false>()));
+    else
+      OSTC_DISPATCH_NSNT(o, resident = (dplur_resident<R, NS_, NT_, false, tj, false>()));
+    kc = dplur_chunk_planes(tiles, z.mz, resident);
+  }
   const dim3 sgrid((unsigned)((z.mx + 63) / 64), (unsigned)((z.my + tj - 1) / tj), (unsigned)((z.mz + kc - 1) / kc));
   const dim3 sblock(64, tj, 1);
   R *in = fuse_dq0 ? z.dq : z.dq0, *out = z.dqk;

@@ -36,7 +36,7 @@ struct CellOpts {
   int mix_model;   // OSTC_AIR / OSTC_MIXTURE / OSTC_FR
   int chem;        // chem_src_method
   int rans_model, turb_implicit, implicit_method, dplur_steps;
-  int dplur_wg;            // DPLUR sweep workgroup target (launch tuning)
+  int dplur_wg;            // DPLUR sweep workgroup target (launch tuning; 0 = from the resident capacity)
 };
 template <typename R>
 int cv_from_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
