@@ -267,6 +267,16 @@ void make_thermo32(const ostc_param &p, ThermoF32 &out) {
     out.wj_wi4[q] = (float)p.WjDivWi_to_One4th[q];
     out.sqrt_wi_wj8[q] = (float)p.sqrt_WiDivWjPl1Mul8[q];
   }
+  // range bounds over the species for the wave-uniform range test (with no
+  // species the test never passes)
+  out.t_mid_min = 3.0e38f;
+  out.t_mid_max = out.t_low_max = -3.0e38f;
+  for (int i = 0; i < p.n_spec && i < kMaxSpec; ++i) {
+    out.t_mid_min = std::min(out.t_mid_min, out.t_mid[i]);
+    out.t_mid_max = std::max(out.t_mid_max, out.t_mid[i]);
+    out.t_low_max = std::max(out.t_low_max, out.t_low[i]);
+  }
+  if (p.n_spec <= 0) out.t_mid_max = out.t_low_max = 3.0e38f;
 }
 // A mechanism of ns < 9 species runs on the 9-species kernels with inert
 // species ns..8 appended: zero mass fraction everywhere, no reactions, zero

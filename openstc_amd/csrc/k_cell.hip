@@ -617,8 +617,8 @@ __device__ __forceinline__ void jd_dir(const ZoneView<R> &z, int64_t q, int dir,
 // wave = one i row), along j through LDS, along k in registers from the
 // previous plane.  Tile-edge neighbours outside the tile: the rows above and
 // below by the edge rows themselves (whole-wave loads); the cells left and
-// right of every row by 2*TJ lanes of the first wave in one pass, handed over
-// through LDS (single-lane halo loads per row would double the tile's vector
+// right of every row by 2*TJ lanes of an inner row's wave in one pass, handed
+// over through LDS (single-lane halo loads per row would double the tile's vector
 // memory instructions, which bound this kernel).  Off-diagonal sums are
 // accumulated in the reference's order (i-, j-, k-, i+, j+, k+).  FIRST
 // (first sweep of a non-reacting zone): din is the right-hand side dq, scaled
@@ -678,9 +678,11 @@ __global__ __launch_bounds__(64 * TJ) void k_dplur(ZoneView<R> z, const ostc_par
       ip[l] = __shfl_up(pi[l], 1);
       im[l] = __shfl_down(mi[l], 1);
     }
-    if (ty == 0 && tx < 2 * TJ) {
+    if (ty == (TJ > 2 ? 1 : 0) && tx < 2 * TJ) {
       // lane 2r: the cell left of row r's segment (its P_i); lane 2r+1: the
-      // cell right of it (its M_i)
+      // cell right of it (its M_i).  An inner row does this (rows 0 and TJ-1
+      // also evaluate the rows outside the tile), so that no wave carries
+      // more than two cells' loads per plane.
       const int r = tx >> 1, side = tx & 1;
       const int ih = side ? (int)bk.x * 64 + 64 : (int)bk.x * 64 - 1;
       const int jr = min((int)bk.y * TJ + r, z.my - 1);

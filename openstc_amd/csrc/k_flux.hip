@@ -934,6 +934,11 @@ static int launch_march(const ZoneView<R> &z, const ostc_param *dp, bool inv, bo
   return OSTC_OK;
 }
 
+// LDS budget of one sweep workgroup: 160 KB per CU holds three of them, the
+// residency the kernels' registers allow (a larger tile would cost a
+// workgroup per CU)
+constexpr int kSweepLds = 53248;
+
 template <typename R, int NS, int NT>
 static int launch_sweeps(const ZoneView<R> &z, const ostc_param *dp, bool inv, bool visc, bool first_writer,
                          hipStream_t st) {
@@ -946,15 +951,31 @@ static int launch_sweeps(const ZoneView<R> &z, const ostc_param *dp, bool inv, b
     if (dir == 0) {
       L = ext[0] < 256 ? ext[0] : 256;
       TW = 1024 / (L + 1);
-      const int lds_cap = 65536 / ((L + 1) * NV * (int)sizeof(R));
+      const int lds_cap = kSweepLds / ((L + 1) * NV * (int)sizeof(R));
       if (TW > lds_cap) TW = lds_cap;
       if (TW < 1) TW = 1;
       if (TW > ext[1]) TW = ext[1];
       grid = dim3((ext[0] + L - 1) / L, (ext[1] + TW - 1) / TW, ext[2]);
     } else {
+      // cells along the sweep per tile (launch tuning; environment override)
+      static const int l_env = [] {
+        const char *e = std::getenv("OSTC_SWEEP_L");
+        return e ? std::atoi(e) : 0;
+      }();
       TW = ext[0] < 64 ? ext[0] : 64;
-      L = 16;
-      while (L > 1 && TW * (L + 1) * NV * (int)sizeof(R) > 65536) L /= 2;
+      const auto lds_ok = [&](int l) { return TW * (l + 1) * NV * (int)sizeof(R) <= kSweepLds; };
+      if (l_env > 0) {
+        L = l_env;
+      } else {
+        // the longest tile (<= 16 cells) whose TW * (L + 1) faces fill whole
+        // rounds of the 256 threads (a partly filled last round costs as much
+        // as a full one), else the longest that fits
+        L = 0;
+        for (int c = 16; c >= 2 && L == 0; --c)
+          if ((TW * (c + 1)) % 256 == 0 && lds_ok(c)) L = c;
+        if (L == 0) L = 16;
+      }
+      while (L > 1 && !lds_ok(L)) L /= 2;
       if (L > ext[dir]) L = ext[dir];
       if (dir == 1)
         grid = dim3((ext[0] + TW - 1) / TW, (ext[1] + L - 1) / L, ext[2]);

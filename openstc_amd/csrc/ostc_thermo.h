@@ -21,6 +21,7 @@ struct ThermoF32 {
   float r_over_w[kMaxSpec], inv_w[kMaxSpec], cp_lam[kMaxSpec];  // R/W, 1/W, 1.25 R/W
   float lj_inv[kMaxSpec], vis_coeff[kMaxSpec];
   float wj_wi4[kMaxSpec * kMaxSpec], sqrt_wi_wj8[kMaxSpec * kMaxSpec];
+  float t_mid_min, t_mid_max, t_low_max;  // over the species: the wave-uniform range test
 };
 constexpr size_t kThermoF32Offset = (sizeof(ostc_param) + 255) / 256 * 256;
 __host__ __device__ __forceinline__ const ThermoF32 *thermo32(const ostc_param *p) {
@@ -82,9 +83,28 @@ __device__ __forceinline__ void pick_coeffs(const float *lo, const float *hi, bo
   for (int q = 0; q < N; ++q) c[q] = use_hi ? hi[q] : lo[q];
 }
 
+// 1 if every active lane of the wave is in the high range of every species,
+// 0 if every one is in the low range of every species and above every T_low,
+// -1 otherwise.  In the first two cases the coefficients are wave-uniform
+// (scalar operands), and the per-lane selects and the T_low test drop out:
+// the same polynomial on the same coefficients, so the same values.
+__device__ __forceinline__ int uniform_range(float t, const ThermoF32 *tf) {
+  if (__all(t >= tf->t_mid_max && t >= tf->t_low_max)) return 1;
+  if (__all(t < tf->t_mid_min && t >= tf->t_low_max)) return 0;
+  return -1;
+}
+
 template <>
 __device__ __forceinline__ void enthalpy<float>(float t, float *h, const ostc_param *__restrict__ p, int ns) {
   const ThermoF32 *tf = thermo32(p);
+  const int r = uniform_range(t, tf);
+  if (r >= 0) {
+    for (int i = 0; i < ns; ++i) {
+      const float *c = tf->h[r][i];
+      h[i] = c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * (c[4] + t * c[5]))));
+    }
+    return;
+  }
   for (int i = 0; i < ns; ++i) {
     float c[6];
     pick_coeffs<6>(tf->h[0][i], tf->h[1][i], t >= tf->t_mid[i], c);
@@ -125,6 +145,15 @@ template <>
 __device__ __forceinline__ void enthalpy_cp<float>(float t, float *h, float *cp, const ostc_param *__restrict__ p,
                                                    int ns) {
   const ThermoF32 *tf = thermo32(p);
+  const int r = uniform_range(t, tf);
+  if (r >= 0) {
+    for (int i = 0; i < ns; ++i) {
+      const float *c = tf->h[r][i], *d = tf->cp[r][i];
+      h[i] = c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * (c[4] + t * c[5]))));
+      cp[i] = d[0] + t * (d[1] + t * (d[2] + t * (d[3] + t * d[4])));
+    }
+    return;
+  }
   for (int i = 0; i < ns; ++i) {
     const bool hi = t >= tf->t_mid[i];
     float c[6], d[5];
@@ -173,6 +202,14 @@ __device__ __forceinline__ void heat_capacity(T t, T *cp, const ostc_param *__re
 template <>
 __device__ __forceinline__ void heat_capacity<float>(float t, float *cp, const ostc_param *__restrict__ p, int ns) {
   const ThermoF32 *tf = thermo32(p);
+  const int r = uniform_range(t, tf);
+  if (r >= 0) {
+    for (int i = 0; i < ns; ++i) {
+      const float *d = tf->cp[r][i];
+      cp[i] = d[0] + t * (d[1] + t * (d[2] + t * (d[3] + t * d[4])));
+    }
+    return;
+  }
   for (int i = 0; i < ns; ++i) {
     float d[5];
     pick_coeffs<5>(tf->cp[0][i], tf->cp[1][i], t >= tf->t_mid[i], d);
