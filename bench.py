@@ -141,6 +141,11 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
     chem = p.mix_model == L.FR and p.chem_src_method in (1, 2)
     epi = p.mix_model == L.FR and p.chem_src_method == 1
     njac = ns * ns if epi else (ns if chem else 0)   # FP64 Jacobian entries per cell
+    # the EPI block of DPLUR is factored in the space of the species that take
+    # part in a reaction (na of them): na^2 LU entries, pivots packed 8 to a word
+    na = sum(1 for l in range(ns) if any(p.stoi_f[r * L.MAX_SPEC + l] or p.stoi_b[r * L.MAX_SPEC + l]
+                                          for r in range(p.n_reac)))
+    npw = (na + 7) // 8
     out = {}
 
     def put(stage, n_bulk, n_f64=0, n_i32=0):
@@ -168,18 +173,24 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
     # local time step: metric 9, u,v,w, rho, c [mu] [mu_t] [gamma] -> spectral radii 3, dt
     put("local_dt", 9 + 3 + 2 + (1 if visc else 0) + (1 if rans else 0) + (1 if ns else 0) + 4)
     # DQ0 (reacting zones; non-reacting zones form dq0 in the first sweep):
-    # dq, dt, spectral radii 3, [J_t 2] -> dq0; Jacobian read, LU + pivots written
-    put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, njac + (ns * ns if epi else 0), ns if epi else 0)
+    # dq, dt, spectral radii 3, [J_t 2] -> dq0; the reacting species' block of
+    # the Jacobian read, its LU factors + packed pivots written
+    if epi:
+        put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, 2 * na * na, npw)
+    else:
+        put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, njac)
     # one sweep: din, sv, u,v,w,|V|, (air p, rho | mixture T, gamma), metric 9,
     # spectral radii 3, dt, dq0, [J_t 2] -> dout.  With EPI the species rows'
     # block solve is its own kernel (dplur_chem): the sweep reads the flow and
     # turbulence rows of dq0 and leaves the species rows' sums in dout, which
     # dplur_chem reads back with dt, the species rows of dq0, the LU factors
-    # and pivots, and overwrites with the solution.
+    # and pivots (and the spectral radii 3 when a species takes part in no
+    # reaction: its block row is the diagonal), and overwrites with the
+    # solution.
     sweep = nv + nsc + 4 + 2 + 9 + 3 + 1 + nv + (2 if rans else 0) + nv
     if epi:
         put("dplur_sweep", sweep - ns)
-        put("dplur_chem", 2 * ns + 1 + ns, ns * ns, ns)
+        put("dplur_chem", 2 * ns + 1 + ns + (3 if na < ns else 0), na * na, npw)
     else:
         put("dplur_sweep", sweep, ns if chem else 0)
     put("dplur_sweep_first", sweep)   # reads dq in place of dq0 and writes dq0 in place of reading it

@@ -330,6 +330,14 @@ void make_chem64(const ostc_param &p, ChemF64 &out) {
       if (f != 0 || b != 0) out.part[r] |= 1u << l;
       if (f != b) out.net[r] |= 1u << l;
     }
+  unsigned any = 0;
+  for (int r = 0; r < p.n_reac && r < kMaxReac; ++r) any |= out.part[r];
+  for (int l = 0; l < kMaxSpec; ++l) out.rank[l] = -1;
+  for (int l = 0; l < p.n_spec && l < kMaxSpec; ++l)
+    if (any & (1u << l)) {
+      out.act[out.n_act] = l;
+      out.rank[l] = out.n_act++;
+    }
   for (int i = 0; i < p.n_spec && i < kMaxSpec; ++i) {
     for (int r = 0; r < 2; ++r) {
       const double *c = r == 0 ? &p.low_temp_coeff[i * 7] : &p.high_temp_coeff[i * 7];
@@ -382,10 +390,16 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
     make_thermo32(*param, tf);
     e = hipMemcpy(reinterpret_cast<char *>(c->dev) + kThermoF32Offset, &tf, sizeof(tf), hipMemcpyHostToDevice);
   }
+  int n_act = 0;
   if (e == hipSuccess) {
     ChemF64 cf;
     make_chem64(*param, cf);
+    n_act = cf.n_act;
     e = hipMemcpy(reinterpret_cast<char *>(c->dev) + kChemF64Offset, &cf, sizeof(cf), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && n_act == 0 && param->mix_model == OSTC_FR && param->chem_src_method == 1) {
+    ostc_ctx_destroy(c);
+    return fail(OSTC_ERR_ARG, "finite-rate chemistry with EPI: no species takes part in a reaction");
   }
   if (e == hipSuccess) e = hipMalloc(&c->out4, 4 * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(&c->host4, 4 * sizeof(double), hipHostMallocDefault);
@@ -407,6 +421,7 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   o.turb_implicit = param->turb_implicit;
   o.implicit_method = param->implicit_method;
   o.dplur_steps = param->dplur_inner_step;
+  o.n_act = n_act;
   // launch tuning of the DPLUR sweep: 0 sizes its k chunks from the device's
   // resident workgroup capacity; the environment can fix a workgroup target
   o.dplur_wg = 0;

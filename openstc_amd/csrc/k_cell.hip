@@ -373,10 +373,35 @@ __global__ __launch_bounds__(256) void k_explicit_implicit(ZoneView<R> z, const 
   }
 }
 
+// The chemistry block of DPLUR, (diag I - dt J), in the space of the species
+// that react.  A species that takes part in no reaction (N2 of H2-air, the
+// inert species appended to smaller mechanisms) has a zero row and column in
+// J: a factorisation of the whole block never picks its row as a pivot (its
+// column entries are zero), its L and U entries stay zero and its U diagonal
+// stays diag, every update it enters subtracts an exact zero, and the solve
+// gives it b / diag.  So k_dq0 factors only the NA x NA block of the NA
+// species that react (ChemF64::act, in their order) and the sweeps solve that
+// block: the same values as the whole 9 x 9 factors (the sign of a zero
+// aside), with NA^2 doubles per cell instead of 81.  Pivots are kept as
+// positions in the block, four bits each, eight to a 32-bit plane of
+// chem_piv.
+template <int NA>
+__device__ __forceinline__ void piv_pack(const int *piv, unsigned *w) {
+#pragma unroll
+  for (int q = 0; q < (NA + 7) / 8; ++q) w[q] = 0;
+#pragma unroll
+  for (int r = 0; r < NA; ++r) w[r >> 3] |= (unsigned)piv[r] << (4 * (r & 7));
+}
+template <int NA>
+__device__ __forceinline__ void piv_unpack(const unsigned *w, int *piv) {
+#pragma unroll
+  for (int r = 0; r < NA; ++r) piv[r] = (int)((w[r >> 3] >> (4 * (r & 7))) & 15u);
+}
+
 // compute_DQ_0 (src/DPLURHPP.cuh:13-76): dq0 = dq / diag with the chemistry
 // (EPI_for_dq0, src/FiniteRateChem.cu:236-253, LU factors kept for the sweeps)
 // and SST (src/SST.cu:196-204) blocks treated point-implicitly.
-template <typename R, int NS, int NT, bool CHEM>
+template <typename R, int NS, int NT, bool CHEM, int NA = NS>
 __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__restrict__ p) {
   const auto c = cell_index(z, 0);
   if (!c.ok) return;
@@ -389,23 +414,33 @@ __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__
     const double dtd = (double)dt, dg = (double)diag;
     const int meth = p->chem_src_method;
     if (meth == 1) {
-      double a[NS * NS], b[NS];
-      int piv[NS];
+      // the block of the species that react (NA of them, n_act at run time)
+      const ChemF64 *ct = chem64(p);
+      double a[NA * NA], b[NA];
+      int piv[NA];
 #pragma unroll
-      for (int q = 0; q < NS * NS; ++q) a[q] = -dtd * z.chem_jac[q * P + id];
+      for (int r = 0; r < NA; ++r) {
+        const int m = ct->act[r];
 #pragma unroll
-      for (int q = 0; q < NS; ++q) {
-        a[q * NS + q] = dg - dtd * z.chem_jac[(q * NS + q) * P + id];
-        b[q] = (double)z.dq[(5 + q) * P + id];
+        for (int c2 = 0; c2 < NA; ++c2) a[r * NA + c2] = -dtd * z.chem_jac[(m * NS + ct->act[c2]) * P + id];
+        a[r * NA + r] = dg - dtd * z.chem_jac[(m * NS + m) * P + id];
+        b[r] = (double)z.dq[(5 + m) * P + id];
       }
-      lu_factor<NS>(a, piv, NS);
+      lu_factor<NA>(a, piv, NA);
 #pragma unroll
-      for (int q = 0; q < NS * NS; ++q) z.chem_lu[q * P + id] = a[q];
+      for (int q = 0; q < NA * NA; ++q) z.chem_lu[q * P + id] = a[q];
+      unsigned w[(NA + 7) / 8];
+      piv_pack<NA>(piv, w);
 #pragma unroll
-      for (int q = 0; q < NS; ++q) z.chem_piv[q * P + id] = piv[q];
-      lu_solve<NS>(a, piv, b, NS);
+      for (int q = 0; q < (NA + 7) / 8; ++q) z.chem_piv[q * P + id] = (int)w[q];
+      lu_solve<NA>(a, piv, b, NA);
 #pragma unroll
-      for (int q = 0; q < NS; ++q) z.dq0[(5 + q) * P + id] = (R)b[q];
+      for (int r = 0; r < NA; ++r) z.dq0[(5 + ct->act[r]) * P + id] = (R)b[r];
+      if constexpr (NA < NS) {
+        // species that react in no reaction: b / diag
+        for (int m = 0; m < NS; ++m)
+          if (ct->rank[m] < 0) z.dq0[(5 + m) * P + id] = (R)((double)z.dq[(5 + m) * P + id] / dg);
+      }
     } else if (meth == 2) {
 #pragma unroll
       for (int q = 0; q < NS; ++q)
@@ -483,24 +518,39 @@ __device__ __forceinline__ void dplur_finish(const ZoneView<R> &z, const ostc_pa
 // in place.  A kernel of its own: one streaming pass over the 9x9 FP64
 // factors with every load issued at once, instead of loads that the sweep's
 // neighbour exchange would leave exposed at the end of each plane.
-template <typename R, int NS>
-__global__ __launch_bounds__(256) void k_dplur_chem(ZoneView<R> z, R *__restrict__ dout) {
+template <typename R, int NS, int NA>
+__global__ __launch_bounds__(256) void k_dplur_chem(ZoneView<R> z, const ostc_param *__restrict__ p,
+                                                    R *__restrict__ dout) {
   const auto c = cell_index(z, 0);
   if (!c.ok) return;
   const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
-  const double dtd = (double)z.dt[id];
-  double a[NS * NS], b[NS];
-  int piv[NS];
+  const ChemF64 *ct = chem64(p);
+  const R dt = z.dt[id];
+  const double dtd = (double)dt;
+  double a[NA * NA], b[NA];
+  int piv[NA];
+  unsigned w[(NA + 7) / 8];
 #pragma unroll
-  for (int q = 0; q < NS * NS; ++q) a[q] = z.chem_lu[q * P + id];
+  for (int q = 0; q < NA * NA; ++q) a[q] = z.chem_lu[q * P + id];
 #pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    piv[q] = z.chem_piv[q * P + id];
-    b[q] = (double)dout[(5 + q) * P + id] * dtd;
+  for (int q = 0; q < (NA + 7) / 8; ++q) w[q] = (unsigned)z.chem_piv[q * P + id];
+  piv_unpack<NA>(w, piv);
+#pragma unroll
+  for (int r = 0; r < NA; ++r) b[r] = (double)dout[(5 + ct->act[r]) * P + id] * dtd;
+  lu_solve<NA>(a, piv, b, NA);
+#pragma unroll
+  for (int r = 0; r < NA; ++r) {
+    const int m = ct->act[r];
+    dout[(5 + m) * P + id] = (R)((double)z.dq0[(5 + m) * P + id] + b[r]);
   }
-  lu_solve<NS>(a, piv, b, NS);
-#pragma unroll
-  for (int q = 0; q < NS; ++q) dout[(5 + q) * P + id] = (R)((double)z.dq0[(5 + q) * P + id] + b[q]);
+  if constexpr (NA < NS) {
+    // species that react in no reaction: their block row is diag, as k_dq0
+    // formed it
+    const double dg = (double)((R)1 + dt * (z.spectr[id] + z.spectr[P + id] + z.spectr[2 * P + id]));
+    for (int m = 0; m < NS; ++m)
+      if (ct->rank[m] < 0)
+        dout[(5 + m) * P + id] = (R)((double)z.dq0[(5 + m) * P + id] + (double)dout[(5 + m) * P + id] * dtd / dg);
+  }
 }
 
 // Per-cell inputs of the convective-Jacobian products of one DPLUR sweep
@@ -1274,6 +1324,23 @@ inline int dplur_chunk_planes(int tiles, int mz, int resident) {
   return best;
 }
 
+// species of the mechanism that react (the EPI block of DPLUR), 1..9
+#define OSTC_DISPATCH_NA(o, CALL)                                  \
+  do {                                                             \
+    switch ((o).n_act) {                                           \
+      case 1: { constexpr int NA_ = 1; CALL; } break;              \
+      case 2: { constexpr int NA_ = 2; CALL; } break;              \
+      case 3: { constexpr int NA_ = 3; CALL; } break;              \
+      case 4: { constexpr int NA_ = 4; CALL; } break;              \
+      case 5: { constexpr int NA_ = 5; CALL; } break;              \
+      case 6: { constexpr int NA_ = 6; CALL; } break;              \
+      case 7: { constexpr int NA_ = 7; CALL; } break;              \
+      case 8: { constexpr int NA_ = 8; CALL; } break;              \
+      case 9: { constexpr int NA_ = 9; CALL; } break;              \
+      default: return OSTC_ERR_ARG;                                \
+    }                                                              \
+  } while (0)
+
 #define OSTC_DISPATCH_NSNT(o, CALL)              \
   do {                                           \
     if ((o).ns == 0 && (o).nt == 0) {            \
@@ -1384,7 +1451,15 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
   if (!fuse_dq0) {
     prof_mark(P_DQ0, st, true);
     if (chem) {
-      OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp));
+      if (o.ns == 9 && o.chem == 1) {
+        // EPI block: factored in the space of the species that react
+        if (o.nt == 2)
+          OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 2, true, NA_>), g.grid, g.block, 0, st, z, dp));
+        else
+          OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 0, true, NA_>), g.grid, g.block, 0, st, z, dp));
+      } else {
+        OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp));
+      }
     } else {
       OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, false>), g.grid, g.block, 0, st, z, dp));
     }
@@ -1429,7 +1504,7 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
     if (chem && o.chem == 1 && o.ns > 0) {
       prof_mark(P_DPLUR_CHEM, st, true);
       if (o.ns != 9) return OSTC_ERR_ARG;
-      hipLaunchKernelGGL((k_dplur_chem<R, 9>), g.grid, g.block, 0, st, z, out);
+      OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dplur_chem<R, 9, NA_>), g.grid, g.block, 0, st, z, dp, out));
       prof_mark(P_DPLUR_CHEM, st, false);
       if (last_err()) return OSTC_ERR_HIP;
     }

@@ -37,6 +37,7 @@ struct CellOpts {
   int chem;        // chem_src_method
   int rans_model, turb_implicit, implicit_method, dplur_steps;
   int dplur_wg;            // DPLUR sweep workgroup target (launch tuning; 0 = from the resident capacity)
+  int n_act;               // species that take part in a reaction (the EPI block's size)
 };
 template <typename R>
 int cv_from_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);

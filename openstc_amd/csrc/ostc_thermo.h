@@ -45,6 +45,11 @@ struct ChemF64 {
   // + c6 / T; below T_low the value at T_low
   double g[2][kMaxSpec][7];
   double g_low[kMaxSpec];
+  // species that take part in at least one reaction, in order: act[r] is the
+  // r-th of them, rank[k] species k's position among them (-1: none -- its
+  // row and column of the EPI Jacobian are zero), n_act their count
+  int n_act;
+  int act[kMaxSpec], rank[kMaxSpec];
 };
 constexpr size_t kChemF64Offset = (kThermoF32Offset + sizeof(ThermoF32) + 255) / 256 * 256;
 constexpr size_t kParamBlockBytes = kChemF64Offset + sizeof(ChemF64);
@@ -87,7 +92,8 @@ __device__ __forceinline__ void pick_coeffs(const float *lo, const float *hi, bo
 // 0 if every one is in the low range of every species and above every T_low,
 // -1 otherwise.  In the first two cases the coefficients are wave-uniform
 // (scalar operands), and the per-lane selects and the T_low test drop out:
-// the same polynomial on the same coefficients, so the same values.
+// the same Horner polynomial on the same coefficients, in another
+// instruction sequence (FP32 results can differ in the last bit).
 __device__ __forceinline__ int uniform_range(float t, const ThermoF32 *tf) {
   if (__all(t >= tf->t_mid_max && t >= tf->t_low_max)) return 1;
   if (__all(t < tf->t_mid_min && t >= tf->t_low_max)) return 0;
