@@ -207,14 +207,17 @@ __global__ __launch_bounds__(256) void k_local_dt(ZoneView<R> z, const ostc_para
 // density, viscosity and mut this kernel holds -- local_time_step reads
 // nothing the flux stages between them write.
 // CR: 0 no chemistry; > 0 the two-pass chemistry for mechanisms of at most
-// CR reactions; < 0 the one-pass form (any reaction count).  Three waves per
-// SIMD are asked for: the two-pass chemistry then spills ~230 B/lane, and
-// still runs 1.15x faster than at its unspilled two waves per SIMD.
+// CR reactions (rates of progress in 2 CR doubles of LDS per thread: 80 KB
+// per workgroup at CR = 20, two workgroups per CU); < 0 the one-pass form (any
+// reaction count).  Three waves per SIMD are asked for (the kernels without
+// the LDS reach them).
 template <typename R, int NS, int NT, int CR, bool DT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_source(ZoneView<R> z,
                                                                                         const ostc_param *__restrict__ p) {
   constexpr bool CHEM = CR != 0;
   static_assert(!DT || NT > 0, "the fused time step needs the SST branch's metric and mut");
+  // CR > 0: the rates of progress of the 256 cells of the workgroup
+  __shared__ double qs_mem[CR > 0 && NS > 0 ? 2 * CR * 256 : 1];
   constexpr int NV = 5 + NS + NT;
   const auto c = cell_index(z, 0);
   if (!c.ok) return;
@@ -299,19 +302,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
       for (int l = 5 + NS; l < NV; ++l) z.dq[l * P + id] = dq[l];
       double *jac = z.chem_jac;
+      // rates of progress in LDS, this thread's column of qs_mem
+      double *qs = qs_mem + threadIdx.x + 64 * threadIdx.y;
+      const int nr = min(p->n_reac, CR);
+      const auto om = [&](int l, double v) { z.dq[(5 + l) * P + id] = (R)((double)J * v); };
       if (meth == 1) {
-        chem_source_rows<NS, CR, 1>(p, p->n_reac, rho, t, y, omega, [&](int m, const double *row) {
+        chem_source_lds<NS, 1>(p, nr, CR, rho, t, y, qs, 256, om, [&](int m, const double *row) {
 #pragma unroll
           for (int n = 0; n < NS; ++n) jac[(m * NS + n) * P + id] = row[n];
         });
       } else if (meth == 2) {
-        chem_source_rows<NS, CR, 2>(p, p->n_reac, rho, t, y, omega,
-                                    [&](int m, const double *d) { jac[m * P + id] = *d; });
+        chem_source_lds<NS, 2>(p, nr, CR, rho, t, y, qs, 256, om,
+                               [&](int m, const double *d) { jac[m * P + id] = *d; });
       } else {
-        chem_source_rows<NS, CR, 0>(p, p->n_reac, rho, t, y, omega, [&](int, const double *) {});
+        chem_source_lds<NS, 0>(p, nr, CR, rho, t, y, qs, 256, om, [&](int, const double *) {});
       }
-#pragma unroll
-      for (int l = 0; l < NS; ++l) z.dq[(5 + l) * P + id] = (R)((double)J * omega[l]);
+      (void)omega;
       return;
     } else if (meth == 1) {
       double jac[NS * NS];

@@ -529,20 +529,24 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
   }
 }
 
-// The same production rates and Jacobian in two passes, for mechanisms of at
-// most NR reactions (NR a compile-time bound, nr <= NR at run time): first
-// every reaction's rates of progress into registers (q1, q2 at compile-time
-// indices), then omega and the Jacobian row by row.  Each finished row goes
-// to row_out(m, row) at once, so only one row of the 9x9 matrix is live --
-// against the whole matrix for the one-pass form, which holds the kernel to
-// one wave per SIMD.  Sums run over the reactions in the same order as the
-// one-pass form (same terms, same rounding).  JAC: 1 EPI rows, 2 DA diagonal
-// (row_out(m, &d) with the diagonal entry), 0 none.
-template <int NS, int NR, int JAC, class RowOut>
-__device__ __forceinline__ void chem_source_rows(const ostc_param *__restrict__ p, int nr, double rho, double t,
-                                                 const double *y, double *omega, RowOut row_out) {
+// The same production rates and Jacobian in two passes: first every
+// reaction's rates of progress, kept in LDS (q1 of reaction r at
+// qs[r * stride], q2 at qs[(nr_max + r) * stride]; stride = the workgroup's
+// thread count, qs already offset to this thread), then omega and the
+// Jacobian row by row.  Each finished row goes to row_out(m, row) at once, so
+// only one row of the 9x9 matrix is live -- against the whole matrix for the
+// one-pass form, which holds a kernel to one wave per SIMD -- and every loop
+// over the reactions is a real loop with one copy of the rate law (rates of
+// progress in registers need the reaction loops unrolled: 20 copies of the
+// rate law, ~0.5 MB of code through a 64 KB instruction cache).  Sums run over
+// the reactions in the same order as the one-pass form (same terms, same
+// rounding).  JAC: 1 EPI rows, 2 DA diagonal (row_out(m, &d) with the diagonal
+// entry), 0 none; omega_out(k, omega_k) receives the production rates.
+template <int NS, int JAC, class RowOut, class OmegaOut>
+__device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p, int nr, int nr_max, double rho,
+                                                double t, const double *y, double *qs, int stride,
+                                                OmegaOut omega_out, RowOut row_out) {
   const ChemF64 *ct = chem64(p);
-  double q1[NR], q2[NR];
   {
     double c[NS], g[NS];
 #pragma unroll
@@ -550,58 +554,65 @@ __device__ __forceinline__ void chem_source_rows(const ostc_param *__restrict__ 
     const double log_t = log(t), t_inv = 1.0 / t;
     gibbs_div_rt<NS>(t, log_t, t_inv, p, g);
     const double temp_t = kPatm / kRu * 1e-3 * t_inv, temp_t_inv = 1.0 / temp_t;
-    // fully unrolled (q1/q2 at compile-time indices stay in registers)
-#pragma clang loop unroll(full)
-    for (int r = 0; r < NR; ++r) {
-      q1[r] = q2[r] = 0;
-      if (r < nr) {
-        double nf[NS], nb[NS];
+#pragma unroll 1
+    for (int r = 0; r < nr; ++r) {
+      double nf[NS], nb[NS];
 #pragma unroll
-        for (int l = 0; l < NS; ++l) {
-          nf[l] = ct->nu_f[r * kMaxSpec + l];
-          nb[l] = ct->nu_b[r * kMaxSpec + l];
-        }
-        reaction_rates<NS>(p, r, t, t_inv, log_t, temp_t, temp_t_inv, c, g, nf, nb, ct->part[r], q1[r], q2[r]);
+      for (int l = 0; l < NS; ++l) {
+        nf[l] = ct->nu_f[r * kMaxSpec + l];
+        nb[l] = ct->nu_b[r * kMaxSpec + l];
       }
+      double q1, q2;
+      reaction_rates<NS>(p, r, t, t_inv, log_t, temp_t, temp_t_inv, c, g, nf, nb, ct->part[r], q1, q2);
+      qs[r * stride] = q1;
+      qs[(nr_max + r) * stride] = q2;
     }
   }
+  {
+    double cr[NS], d[NS];
 #pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    double cr = 0, d = 0;
-#pragma clang loop unroll(full)
-    for (int r = 0; r < NR; ++r) {
-      if (r < nr && (ct->part[r] & (1u << i))) {
-        const double nf = ct->nu_f[r * kMaxSpec + i], nb = ct->nu_b[r * kMaxSpec + i];
-        cr += q2[r] * nf + q1[r] * nb;
-        d += q1[r] * nf + q2[r] * nb;
+    for (int i = 0; i < NS; ++i) cr[i] = d[i] = 0;
+#pragma unroll 1
+    for (int r = 0; r < nr; ++r) {
+      const unsigned part = ct->part[r];
+      const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        if (part & (1u << i)) {
+          const double nf = ct->nu_f[r * kMaxSpec + i], nb = ct->nu_b[r * kMaxSpec + i];
+          cr[i] += q2 * nf + q1 * nb;
+          d[i] += q1 * nf + q2 * nb;
+        }
       }
     }
-    cr *= p->mw[i] * 1e+3;
-    d *= p->mw[i] * 1e+3;
-    omega[i] = cr - d;
-    if constexpr (JAC == 2) {
-      const double jd = y[i] > 1e-25 ? -d / (y[i] * rho) : 0.0;
-      row_out(i, &jd);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const double crv = cr[i] * (p->mw[i] * 1e+3), dv = d[i] * (p->mw[i] * 1e+3);
+      omega_out(i, crv - dv);
+      if constexpr (JAC == 2) {
+        const double jd = y[i] > 1e-25 ? -dv / (y[i] * rho) : 0.0;
+        row_out(i, &jd);
+      }
     }
   }
   if constexpr (JAC == 1) {
     double rry[NS];
 #pragma unroll
     for (int n = 0; n < NS; ++n) rry[n] = y[n] > 1e-25 ? 1.0 / (rho * y[n]) : 0.0;
-#pragma unroll
+#pragma unroll 1
     for (int m = 0; m < NS; ++m) {
       double row[NS];
 #pragma unroll
       for (int n = 0; n < NS; ++n) row[n] = 0;
-#pragma clang loop unroll(full)
-      for (int r = 0; r < NR; ++r) {
-        if (r < nr && (ct->net[r] & (1u << m))) {
+#pragma unroll 1
+      for (int r = 0; r < nr; ++r) {
+        if (ct->net[r] & (1u << m)) {
           const unsigned part = ct->part[r];
           const double dm = ct->nu_b[r * kMaxSpec + m] - ct->nu_f[r * kMaxSpec + m];
+          const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
 #pragma unroll
           for (int n = 0; n < NS; ++n)
-            if (part & (1u << n))
-              row[n] += dm * (ct->nu_f[r * kMaxSpec + n] * q1[r] - ct->nu_b[r * kMaxSpec + n] * q2[r]);
+            if (part & (1u << n)) row[n] += dm * (ct->nu_f[r * kMaxSpec + n] * q1 - ct->nu_b[r * kMaxSpec + n] * q2);
         }
       }
 #pragma unroll
