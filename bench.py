@@ -49,29 +49,29 @@ METRIC = "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem
 # Re-measured whenever the kernel changes; a stage without an entry reports
 # null.
 MEASURED_TRAFFIC = {
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2339.8e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 5095.8e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 9737.0e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 8562.8e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3912.8e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4760.6e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3504.5e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.6e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2305.7e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4178.5e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 7700.3e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 5663.0e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3635.6e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4034.1e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3071.2e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.5e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "local_dt"): 537.3e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 543.6e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1441.5e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 2715.1e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "source"): 2190.3e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 701.7e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 824.0e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 744.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "local_dt"): 144.9e6,
-    ("sst_flat_plate_dplur", "dplur_sweep"): 480.5e6,
-    ("sst_flat_plate_dplur", "dplur_sweep_first"): 491.4e6,
-    ("sst_flat_plate_dplur", "flux_ij"): 402.4e6,
-    ("sst_flat_plate_dplur", "flux_k"): 402.3e6,
-    ("sst_flat_plate_dplur", "source"): 354.0e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 514.7e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1174.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 2137.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "source"): 1363.2e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 739.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 840.6e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 743.9e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.6e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "local_dt"): 144.8e6,
+    ("sst_flat_plate_dplur", "dplur_sweep"): 478.6e6,
+    ("sst_flat_plate_dplur", "dplur_sweep_first"): 506.3e6,
+    ("sst_flat_plate_dplur", "flux_ij"): 402.8e6,
+    ("sst_flat_plate_dplur", "flux_k"): 402.2e6,
+    ("sst_flat_plate_dplur", "source"): 358.1e6,
     ("sst_flat_plate_dplur", "update"): 273.5e6,
 }
 
