@@ -50,6 +50,10 @@ struct ChemF64 {
   // row and column of the EPI Jacobian are zero), n_act their count
   int n_act;
   int act[kMaxSpec], rank[kMaxSpec];
+  // per species m, the reactions whose net coefficient of m is nonzero (the
+  // terms of row m of the EPI Jacobian), in order
+  int jn[kMaxSpec];
+  unsigned char jr[kMaxSpec][kMaxReac];
 };
 constexpr size_t kChemF64Offset = (kThermoF32Offset + sizeof(ThermoF32) + 255) / 256 * 256;
 constexpr size_t kParamBlockBytes = kChemF64Offset + sizeof(ChemF64);
@@ -539,8 +543,9 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
 // over the reactions is a real loop with one copy of the rate law (rates of
 // progress in registers need the reaction loops unrolled: 20 copies of the
 // rate law, ~0.5 MB of code through a 64 KB instruction cache).  Sums run over
-// the reactions in the same order as the one-pass form (same terms, same
-// rounding).  JAC: 1 EPI rows, 2 DA diagonal (row_out(m, &d) with the diagonal
+// the reactions in the same order as the one-pass form and, as it does, over
+// all species (one that takes no part in a reaction adds exact zeros); row m
+// visits only the reactions that change species m (ChemF64::jr).  JAC: 1 EPI rows, 2 DA diagonal (row_out(m, &d) with the diagonal
 // entry), 0 none; omega_out(k, omega_k) receives the production rates.
 template <int NS, int JAC, class RowOut, class OmegaOut>
 __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p, int nr, int nr_max, double rho,
@@ -569,20 +574,19 @@ __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p
     }
   }
   {
+    // dense over the species, as the one-pass form: a species that takes no
+    // part in reaction r adds exact zeros (no per-species branches)
     double cr[NS], d[NS];
 #pragma unroll
     for (int i = 0; i < NS; ++i) cr[i] = d[i] = 0;
 #pragma unroll 1
     for (int r = 0; r < nr; ++r) {
-      const unsigned part = ct->part[r];
       const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
 #pragma unroll
       for (int i = 0; i < NS; ++i) {
-        if (part & (1u << i)) {
-          const double nf = ct->nu_f[r * kMaxSpec + i], nb = ct->nu_b[r * kMaxSpec + i];
-          cr[i] += q2 * nf + q1 * nb;
-          d[i] += q1 * nf + q2 * nb;
-        }
+        const double nf = ct->nu_f[r * kMaxSpec + i], nb = ct->nu_b[r * kMaxSpec + i];
+        cr[i] += q2 * nf + q1 * nb;
+        d[i] += q1 * nf + q2 * nb;
       }
     }
 #pragma unroll
@@ -604,16 +608,15 @@ __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p
       double row[NS];
 #pragma unroll
       for (int n = 0; n < NS; ++n) row[n] = 0;
+      // the reactions that change species m, dense over the columns
+      const int jn = ct->jn[m];
 #pragma unroll 1
-      for (int r = 0; r < nr; ++r) {
-        if (ct->net[r] & (1u << m)) {
-          const unsigned part = ct->part[r];
-          const double dm = ct->nu_b[r * kMaxSpec + m] - ct->nu_f[r * kMaxSpec + m];
-          const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
+      for (int q = 0; q < jn; ++q) {
+        const int r = ct->jr[m][q];
+        const double dm = ct->nu_b[r * kMaxSpec + m] - ct->nu_f[r * kMaxSpec + m];
+        const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
 #pragma unroll
-          for (int n = 0; n < NS; ++n)
-            if (part & (1u << n)) row[n] += dm * (ct->nu_f[r * kMaxSpec + n] * q1 - ct->nu_b[r * kMaxSpec + n] * q2);
-        }
+        for (int n = 0; n < NS; ++n) row[n] += dm * (ct->nu_f[r * kMaxSpec + n] * q1 - ct->nu_b[r * kMaxSpec + n] * q2);
       }
 #pragma unroll
       for (int n = 0; n < NS; ++n) row[n] = y[n] > 1e-25 ? p->mw[m] * (row[n] * rry[n]) * 1e+3 : 0.0;
