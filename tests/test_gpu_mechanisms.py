@@ -27,8 +27,9 @@ def say(*a):
     print("[gpu-mech]", *a, flush=True)
 
 
-def write_subset(tmp):
-    """The shipped mechanism cut to SPECIES and its first eight reactions."""
+def write_subset(tmp, species=SPECIES, name="chemkin7"):
+    """The shipped mechanism cut to `species` (in that order) and its first
+    eight reactions."""
     src = open(os.path.join(cases.DATA, "combustion_mechanism.inp")).read().splitlines()
     out, in_species, in_reactions = [], False, False
     for line in src:
@@ -36,7 +37,7 @@ def write_subset(tmp):
         if u.startswith("SPECIES"):
             in_species = True
             out.append(line)
-            out.append(" " + "  ".join(SPECIES))
+            out.append(" " + "  ".join(species))
             continue
         if in_species:
             if u.startswith("END"):
@@ -49,7 +50,7 @@ def write_subset(tmp):
             out.append("END")
             break
         out.append(line)
-    d = os.path.join(tmp, "chemkin7")
+    d = os.path.join(tmp, name)
     os.makedirs(d, exist_ok=True)
     with open(os.path.join(d, "combustion_mechanism.inp"), "w") as f:
         f.write("\n".join(out) + "\n")
@@ -62,6 +63,16 @@ def write_subset(tmp):
 @pytest.fixture(scope="module")
 def mech_dir(tmp_path_factory):
     return write_subset(str(tmp_path_factory.mktemp("mech")))
+
+
+# the same subset with N2, which takes part in no reaction, second in the
+# species list: the EPI block of the species that react has a gap
+SPECIES_N2_INSIDE = ["H2", "N2", "H", "O2", "O", "OH", "H2O"]
+
+
+@pytest.fixture(scope="module")
+def mech_dir_n2_inside(tmp_path_factory):
+    return write_subset(str(tmp_path_factory.mktemp("mech_n2")), SPECIES_N2_INSIDE, "chemkin7_n2")
 
 
 def compare(pr: Pair, b, tol_scaled, tol_y, label):
@@ -146,4 +157,22 @@ def test_jet_crossflow_7_species_sst_fp64(mech_dir):
         pr.orc.step()
     for b in range(2):
         compare(pr, b, 1e-9, 1e-12, f"jet block {b} after 5 iterations")
+    pr.close()
+
+
+def test_mixing_layer_7_species_n2_inside_fp64(mech_dir_n2_inside):
+    """N2 second of seven species: the DPLUR chemistry block is factored and
+    solved over the six species that react (species 0, 2..6 of the caller,
+    with the two appended inert species also outside it), and N2's rows are
+    divided by the diagonal; the oracle factors the whole 7x7 block."""
+    mech, info = cases.load_mechanism(True, mech_dir_n2_inside)
+    assert cases.species_names(info) == SPECIES_N2_INSIDE
+    case = cases.mixing_layer(mx=24, my=16, mz=8, cfl=0.2, mech_dir=mech_dir_n2_inside)
+    say("mixing_layer 24x16x8, 7 species with N2 second, FP64, 5 iterations")
+    pr = Pair(case, precision=L.FP64)
+    pr.init()
+    for _ in range(5):
+        pr.gpu.step()
+        pr.orc.step()
+    compare(pr, 0, 1e-9, 1e-12, "7 species (N2 second) FP64 after 5 iterations")
     pr.close()
