@@ -40,12 +40,13 @@ enum { OSTC_LAMINAR = 0, OSTC_RANS = 1 };
  * (src/DParameter.h:11-58, filled by src/DParameter.cu:4-101) plus the model
  * switches the reference reads from Parameter (implicit_method,
  * DPLUR_inner_step).  Species matrices use the fixed stride OSTC_MAX_SPEC;
- * NASA-7 coefficient rows use stride 7.  Mixtures take 1 to 9 species: the
- * kernels are instantiated for 9, and a smaller mechanism runs on them with
- * inert species appended (zero mass fraction, no reactions), which leaves
- * every result of its own species unchanged.  Zone fields are exchanged in
- * the caller's layout (its species count); ostc_zone_field_ptr exposes the
- * device layout, with 9 species planes. */
+ * NASA-7 coefficient rows use stride 7.  Mixtures take 1 to 20 species: the
+ * kernels are instantiated for 9 and for 20, and a mechanism runs on the
+ * smaller of the two that holds it, with inert species appended (zero mass
+ * fraction, no reactions), which leaves every result of its own species
+ * unchanged.  Zone fields are exchanged in the caller's layout (its species
+ * count); ostc_zone_field_ptr exposes the device layout, with 9 or 20
+ * species planes. */
 typedef struct ostc_param {
   int mix_model, turb_method;
   int inviscid_scheme, reconstruction, limiter, viscous_scheme;

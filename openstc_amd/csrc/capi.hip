@@ -105,7 +105,7 @@ void prof_mark(int stage, hipStream_t st, bool begin) {
 
 struct ostc_ctx {
   // host copy of the parameter block the kernels use: a mechanism of fewer
-  // than kSpeciesBucket species is padded to that many (see pad_species)
+  // than its species bucket (9 or 20) is padded to that many (see pad_species)
   ostc_param host{};
   int user_ns = 0;  // species of the caller's mechanism
   int pad = 0;      // inert species added after them
@@ -200,15 +200,22 @@ ZoneView<R> make_view(ostc_zone *z) {
 bool fp64(const ostc_zone *z) { return z->desc.precision == OSTC_FP64; }
 hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
 
-// species count the mixture kernels are instantiated for
-constexpr int kSpeciesBucket = 9;
+// species counts the mixture kernels are instantiated for: a mechanism of
+// n species runs on the smallest bucket that holds it, padded with inert
+// species (pad_species)
+constexpr int kSpeciesBuckets[] = {9, 20};
+int species_bucket(int ns) {
+  for (int b : kSpeciesBuckets)
+    if (ns <= b) return b;
+  return 0;
+}
 
 int check_param(const ostc_param &p) {
   if (p.mix_model < 0 || p.mix_model > 2) return fail(OSTC_ERR_ARG, "mix_model must be 0..2");
   if (p.turb_method == OSTC_RANS && p.rans_model != 2) return fail(OSTC_ERR_ARG, "only the k-omega SST RANS model");
-  if (p.mix_model != OSTC_AIR && (p.n_spec < 1 || p.n_spec > kSpeciesBucket))
-    return fail(OSTC_ERR_ARG, "mixtures of 1 to 9 species (the kernels are instantiated for 9; smaller "
-                              "mechanisms are padded with inert species)");
+  if (p.mix_model != OSTC_AIR && (p.n_spec < 1 || p.n_spec > OSTC_MAX_SPEC))
+    return fail(OSTC_ERR_ARG, "mixtures of 1 to 20 species (the kernels are instantiated for 9 and 20; "
+                              "other mechanisms are padded with inert species)");
   if (p.mix_model == OSTC_AIR && p.n_spec != 0) return fail(OSTC_ERR_ARG, "air has no species");
   const int nt = p.turb_method == OSTC_RANS ? 2 : 0;
   if (p.n_scalar != p.n_spec + nt || p.n_var != 5 + p.n_spec + nt)
@@ -359,7 +366,7 @@ extern "C" {
 const char *ostc_last_error(void) { return g_err.c_str(); }
 
 const char *ostc_build_info(void) {
-  return "openstc_amd libostc: HIP gfx950, FP32/FP64 zones, air and mixtures of 1-9 species (9-species kernels)";
+  return "openstc_amd libostc: HIP gfx950, FP32/FP64 zones, air and mixtures of 1-20 species (9- and 20-species kernels)";
 }
 
 int ostc_sizeof_param(void) { return (int)sizeof(ostc_param); }
@@ -377,8 +384,9 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
   auto *c = new (std::nothrow) ostc_ctx;
   if (!c) return fail(OSTC_ERR_ALLOC, "host allocation");
   c->user_ns = param->n_spec;
-  c->pad = (param->mix_model != OSTC_AIR && param->n_spec < kSpeciesBucket) ? kSpeciesBucket - param->n_spec : 0;
-  c->host = c->pad ? pad_species(*param, kSpeciesBucket) : *param;
+  const int bucket = param->mix_model != OSTC_AIR ? species_bucket(param->n_spec) : 0;
+  c->pad = (param->mix_model != OSTC_AIR && param->n_spec < bucket) ? bucket - param->n_spec : 0;
+  c->host = c->pad ? pad_species(*param, bucket) : *param;
   param = &c->host;
   // the parameter block, followed by its FP32 thermodynamic and FP64
   // chemistry tables

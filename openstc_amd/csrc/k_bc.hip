@@ -414,6 +414,8 @@ int apply_bc(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, cons
     else if (nsc == 2 && rans) hipLaunchKernelGGL((KERNEL<R, 2, true>), grid, block, 0, st, __VA_ARGS__); \
     else if (nsc == 9 && !rans) hipLaunchKernelGGL((KERNEL<R, 9, false>), grid, block, 0, st, __VA_ARGS__); \
     else if (nsc == 11 && rans) hipLaunchKernelGGL((KERNEL<R, 11, true>), grid, block, 0, st, __VA_ARGS__); \
+    else if (nsc == 20 && !rans) hipLaunchKernelGGL((KERNEL<R, 20, false>), grid, block, 0, st, __VA_ARGS__); \
+    else if (nsc == 22 && rans) hipLaunchKernelGGL((KERNEL<R, 22, true>), grid, block, 0, st, __VA_ARGS__); \
     else return OSTC_ERR_ARG;                                                                \
   } while (0)
     switch (bf.kind) {
@@ -430,6 +432,10 @@ int apply_bc(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, cons
           hipLaunchKernelGGL((k_wall<R, 9, false>), grid, block, 0, st, z, f, dp, w.isothermal, w.temperature);
         else if (o.ns == 9 && rans)
           hipLaunchKernelGGL((k_wall<R, 9, true>), grid, block, 0, st, z, f, dp, w.isothermal, w.temperature);
+        else if (o.ns == 20 && !rans)
+          hipLaunchKernelGGL((k_wall<R, 20, false>), grid, block, 0, st, z, f, dp, w.isothermal, w.temperature);
+        else if (o.ns == 20 && rans)
+          hipLaunchKernelGGL((k_wall<R, 20, true>), grid, block, 0, st, z, f, dp, w.isothermal, w.temperature);
         else
           return OSTC_ERR_ARG;
         break;
@@ -459,6 +465,10 @@ int inner_halo(const ZoneView<R> &z, const ZoneView<R> &t, const ostc_param *dp,
     hipLaunchKernelGGL((k_inner<R, 9, 0>), grid, block, 0, st, z, t, dp, f);
   else if (o.ns == 9 && o.nt == 2)
     hipLaunchKernelGGL((k_inner<R, 9, 2>), grid, block, 0, st, z, t, dp, f);
+  else if (o.ns == 20 && o.nt == 0)
+    hipLaunchKernelGGL((k_inner<R, 20, 0>), grid, block, 0, st, z, t, dp, f);
+  else if (o.ns == 20 && o.nt == 2)
+    hipLaunchKernelGGL((k_inner<R, 20, 2>), grid, block, 0, st, z, t, dp, f);
   else
     return OSTC_ERR_ARG;
   return hipGetLastError() == hipSuccess ? OSTC_OK : OSTC_ERR_HIP;

@@ -361,8 +361,8 @@ __global__ __launch_bounds__(256) void k_explicit_implicit(ZoneView<R> z, const 
         a[q * NS + q] = 1.0 - dtd * z.chem_jac[(q * NS + q) * P + id];
         b[q] = (double)z.dq[(5 + q) * P + id];
       }
-      lu_factor<NS>(a, piv, NS);
-      lu_solve<NS>(a, piv, b, NS);
+      lu_factor_any<NS>(a, piv);
+      lu_solve_any<NS>(a, piv, b);
 #pragma unroll
       for (int q = 0; q < NS; ++q) z.dq[(5 + q) * P + id] = (R)b[q];
     } else if (p->chem_src_method == 2) {
@@ -388,20 +388,36 @@ __global__ __launch_bounds__(256) void k_explicit_implicit(ZoneView<R> z, const 
 // gives it b / diag.  So k_dq0 factors only the NA x NA block of the NA
 // species that react (ChemF64::act, in their order) and the sweeps solve that
 // block: the same values as the whole 9 x 9 factors (the sign of a zero
-// aside), with NA^2 doubles per cell instead of 81.  Pivots are kept as
-// positions in the block, four bits each, eight to a 32-bit plane of
-// chem_piv.
+// aside), with NA^2 doubles per cell instead of 81.  NA == NS takes the whole
+// block in the species' own order (the 20-species bucket, whose appended
+// inert species stay in it as zero rows and columns).  Pivots are kept as
+// positions in the block, four bits each (eight bits above 16 species),
+// packed into 32-bit planes of chem_piv.
+template <int NA>
+struct PivPack {
+  static constexpr int bits = NA > 16 ? 8 : 4, per = 32 / bits, words = (NA + per - 1) / per;
+};
 template <int NA>
 __device__ __forceinline__ void piv_pack(const int *piv, unsigned *w) {
+  using PP = PivPack<NA>;
 #pragma unroll
-  for (int q = 0; q < (NA + 7) / 8; ++q) w[q] = 0;
+  for (int q = 0; q < PP::words; ++q) w[q] = 0;
 #pragma unroll
-  for (int r = 0; r < NA; ++r) w[r >> 3] |= (unsigned)piv[r] << (4 * (r & 7));
+  for (int r = 0; r < NA; ++r) w[r / PP::per] |= (unsigned)piv[r] << (PP::bits * (r % PP::per));
 }
 template <int NA>
 __device__ __forceinline__ void piv_unpack(const unsigned *w, int *piv) {
+  using PP = PivPack<NA>;
 #pragma unroll
-  for (int r = 0; r < NA; ++r) piv[r] = (int)((w[r >> 3] >> (4 * (r & 7))) & 15u);
+  for (int r = 0; r < NA; ++r) piv[r] = (int)((w[r / PP::per] >> (PP::bits * (r % PP::per))) & ((1u << PP::bits) - 1));
+}
+// the r-th species of the block
+template <int NS, int NA>
+__device__ __forceinline__ int block_species(const ChemF64 *ct, int r) {
+  if constexpr (NA == NS)
+    return r;
+  else
+    return ct->act[r];
 }
 
 // compute_DQ_0 (src/DPLURHPP.cuh:13-76): dq0 = dq / diag with the chemistry
@@ -426,22 +442,23 @@ __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__
       int piv[NA];
 #pragma unroll
       for (int r = 0; r < NA; ++r) {
-        const int m = ct->act[r];
+        const int m = block_species<NS, NA>(ct, r);
 #pragma unroll
-        for (int c2 = 0; c2 < NA; ++c2) a[r * NA + c2] = -dtd * z.chem_jac[(m * NS + ct->act[c2]) * P + id];
+        for (int c2 = 0; c2 < NA; ++c2)
+          a[r * NA + c2] = -dtd * z.chem_jac[(m * NS + block_species<NS, NA>(ct, c2)) * P + id];
         a[r * NA + r] = dg - dtd * z.chem_jac[(m * NS + m) * P + id];
         b[r] = (double)z.dq[(5 + m) * P + id];
       }
-      lu_factor<NA>(a, piv, NA);
+      lu_factor_any<NA>(a, piv);
 #pragma unroll
       for (int q = 0; q < NA * NA; ++q) z.chem_lu[q * P + id] = a[q];
-      unsigned w[(NA + 7) / 8];
+      unsigned w[PivPack<NA>::words];
       piv_pack<NA>(piv, w);
 #pragma unroll
-      for (int q = 0; q < (NA + 7) / 8; ++q) z.chem_piv[q * P + id] = (int)w[q];
-      lu_solve<NA>(a, piv, b, NA);
+      for (int q = 0; q < PivPack<NA>::words; ++q) z.chem_piv[q * P + id] = (int)w[q];
+      lu_solve_any<NA>(a, piv, b);
 #pragma unroll
-      for (int r = 0; r < NA; ++r) z.dq0[(5 + ct->act[r]) * P + id] = (R)b[r];
+      for (int r = 0; r < NA; ++r) z.dq0[(5 + block_species<NS, NA>(ct, r)) * P + id] = (R)b[r];
       if constexpr (NA < NS) {
         // species that react in no reaction: b / diag
         for (int m = 0; m < NS; ++m)
@@ -535,18 +552,18 @@ __global__ __launch_bounds__(256) void k_dplur_chem(ZoneView<R> z, const ostc_pa
   const double dtd = (double)dt;
   double a[NA * NA], b[NA];
   int piv[NA];
-  unsigned w[(NA + 7) / 8];
+  unsigned w[PivPack<NA>::words];
 #pragma unroll
   for (int q = 0; q < NA * NA; ++q) a[q] = z.chem_lu[q * P + id];
 #pragma unroll
-  for (int q = 0; q < (NA + 7) / 8; ++q) w[q] = (unsigned)z.chem_piv[q * P + id];
+  for (int q = 0; q < PivPack<NA>::words; ++q) w[q] = (unsigned)z.chem_piv[q * P + id];
   piv_unpack<NA>(w, piv);
 #pragma unroll
-  for (int r = 0; r < NA; ++r) b[r] = (double)dout[(5 + ct->act[r]) * P + id] * dtd;
-  lu_solve<NA>(a, piv, b, NA);
+  for (int r = 0; r < NA; ++r) b[r] = (double)dout[(5 + block_species<NS, NA>(ct, r)) * P + id] * dtd;
+  lu_solve_any<NA>(a, piv, b);
 #pragma unroll
   for (int r = 0; r < NA; ++r) {
-    const int m = ct->act[r];
+    const int m = block_species<NS, NA>(ct, r);
     dout[(5 + m) * P + id] = (R)((double)z.dq0[(5 + m) * P + id] + b[r]);
   }
   if constexpr (NA < NS) {
@@ -1361,6 +1378,12 @@ inline int dplur_chunk_planes(int tiles, int mz, int resident) {
     } else if ((o).ns == 9 && (o).nt == 2) {     \
       constexpr int NS_ = 9, NT_ = 2;            \
       CALL;                                      \
+    } else if ((o).ns == 20 && (o).nt == 0) {    \
+      constexpr int NS_ = 20, NT_ = 0;           \
+      CALL;                                      \
+    } else if ((o).ns == 20 && (o).nt == 2) {    \
+      constexpr int NS_ = 20, NT_ = 2;           \
+      CALL;                                      \
     } else {                                     \
       return OSTC_ERR_ARG;                       \
     }                                            \
@@ -1382,6 +1405,8 @@ int physical_properties(const ZoneView<R> &z, const ostc_param *dp, const CellOp
     hipLaunchKernelGGL((k_props<R, 0>), g.grid, g.block, 0, st, z, dp);
   else if (o.ns == 9)
     hipLaunchKernelGGL((k_props<R, 9>), g.grid, g.block, 0, st, z, dp);
+  else if (o.ns == 20)
+    hipLaunchKernelGGL((k_props<R, 20>), g.grid, g.block, 0, st, z, dp);
   else
     return OSTC_ERR_ARG;
   return last_err();
@@ -1395,6 +1420,8 @@ int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o
     hipLaunchKernelGGL((k_init_mut<R, 0>), g.grid, g.block, 0, st, z, dp);
   else if (o.ns == 9)
     hipLaunchKernelGGL((k_init_mut<R, 9>), g.grid, g.block, 0, st, z, dp);
+  else if (o.ns == 20)
+    hipLaunchKernelGGL((k_init_mut<R, 20>), g.grid, g.block, 0, st, z, dp);
   else
     return OSTC_ERR_ARG;
   return last_err();
@@ -1417,8 +1444,10 @@ int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, 
   } else if (with_dt) {
     if (o.ns == 0)
       hipLaunchKernelGGL((k_source<R, 0, 2, 0, true>), g.grid, g.block, 0, st, z, dp);
-    else
+    else if (o.ns == 9)
       hipLaunchKernelGGL((k_source<R, 9, 2, 0, true>), g.grid, g.block, 0, st, z, dp);
+    else
+      hipLaunchKernelGGL((k_source<R, 20, 2, 0, true>), g.grid, g.block, 0, st, z, dp);
   } else {
     OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, 0, false>), g.grid, g.block, 0, st, z, dp));
   }
@@ -1463,6 +1492,13 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
           OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 2, true, NA_>), g.grid, g.block, 0, st, z, dp));
         else
           OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 0, true, NA_>), g.grid, g.block, 0, st, z, dp));
+      } else if (o.ns == 20 && o.chem == 1) {
+        // the whole 20 x 20 block (non-reacting species as zero rows and
+        // columns), factored with run-time loops
+        if (o.nt == 2)
+          hipLaunchKernelGGL((k_dq0<R, 20, 2, true, 20>), g.grid, g.block, 0, st, z, dp);
+        else
+          hipLaunchKernelGGL((k_dq0<R, 20, 0, true, 20>), g.grid, g.block, 0, st, z, dp);
       } else {
         OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_dq0<R, NS_, NT_, true>), g.grid, g.block, 0, st, z, dp));
       }
@@ -1509,8 +1545,12 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
     if (last_err()) return OSTC_ERR_HIP;
     if (chem && o.chem == 1 && o.ns > 0) {
       prof_mark(P_DPLUR_CHEM, st, true);
-      if (o.ns != 9) return OSTC_ERR_ARG;
-      OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dplur_chem<R, 9, NA_>), g.grid, g.block, 0, st, z, dp, out));
+      if (o.ns == 9)
+        OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dplur_chem<R, 9, NA_>), g.grid, g.block, 0, st, z, dp, out));
+      else if (o.ns == 20)
+        hipLaunchKernelGGL((k_dplur_chem<R, 20, 20>), g.grid, g.block, 0, st, z, dp, out);
+      else
+        return OSTC_ERR_ARG;
       prof_mark(P_DPLUR_CHEM, st, false);
       if (last_err()) return OSTC_ERR_HIP;
     }

@@ -1018,6 +1018,8 @@ int flux_sweeps(const ZoneView<R> &z, const ostc_param *dp, int ns, int nt, bool
   if (ns == 0 && nt == 2) return launch_march<R, 2>(z, dp, inv, visc, first_writer, st);
   if (ns == 9 && nt == 0) return launch_sweeps<R, 9, 0>(z, dp, inv, visc, first_writer, st);
   if (ns == 9 && nt == 2) return launch_sweeps<R, 9, 2>(z, dp, inv, visc, first_writer, st);
+  if (ns == 20 && nt == 0) return launch_sweeps<R, 20, 0>(z, dp, inv, visc, first_writer, st);
+  if (ns == 20 && nt == 2) return launch_sweeps<R, 20, 2>(z, dp, inv, visc, first_writer, st);
   return OSTC_ERR_ARG;
 }
 

@@ -701,6 +701,74 @@ __device__ __forceinline__ void lu_solve(const double *a, const int *piv, double
   }
 }
 
+// The same factorisation and solve with run-time loops over an array of
+// stride n in private memory, for blocks too large to unroll into registers
+// (the 20-species bucket): the same operations in the same order.
+__device__ inline void lu_factor_rt(double *a, int *piv, int n) {
+  for (int k = 0; k < n; ++k) {
+    int ik = k;
+    double best = 0;
+    for (int m = k; m < n; ++m) {
+      double v = a[m * n + k];
+      for (int t = 0; t < k; ++t) v -= a[m * n + t] * a[t * n + k];
+      a[m * n + k] = v;
+      if (m == k) {
+        best = fabs(v);
+      } else if (fabs(v) > best) {
+        best = fabs(v);
+        ik = m;
+      }
+    }
+    piv[k] = ik;
+    if (ik != k)
+      for (int t = 0; t < n; ++t) {
+        const double x = a[ik * n + t];
+        a[ik * n + t] = a[k * n + t];
+        a[k * n + t] = x;
+      }
+    for (int q = k + 1; q < n; ++q) {
+      double v = a[k * n + q];
+      for (int t = 0; t < k; ++t) v -= a[k * n + t] * a[t * n + q];
+      a[k * n + q] = v;
+    }
+    for (int m = k + 1; m < n; ++m) a[m * n + k] /= a[k * n + k];
+  }
+}
+__device__ inline void lu_solve_rt(const double *a, const int *piv, double *b, int n) {
+  for (int m = 0; m < n; ++m) {
+    const int t = piv[m];
+    if (t != m) {
+      const double x = b[t];
+      b[t] = b[m];
+      b[m] = x;
+    }
+  }
+  for (int m = 1; m < n; ++m)
+    for (int t = 0; t < m; ++t) b[m] -= a[m * n + t] * b[t];
+  b[n - 1] /= a[n * n - 1];
+  for (int m = n - 2; m >= 0; --m) {
+    for (int t = m + 1; t < n; ++t) b[m] -= a[m * n + t] * b[t];
+    b[m] /= a[m * n + m];
+  }
+}
+
+// LU of an NS x NS block held in a[NS * NS]: unrolled into registers up to 9
+// species, run-time loops beyond
+template <int NS>
+__device__ __forceinline__ void lu_factor_any(double *a, int *piv) {
+  if constexpr (NS > 9)
+    lu_factor_rt(a, piv, NS);
+  else
+    lu_factor<NS>(a, piv, NS);
+}
+template <int NS>
+__device__ __forceinline__ void lu_solve_any(const double *a, const int *piv, double *b) {
+  if constexpr (NS > 9)
+    lu_solve_rt(a, piv, b, NS);
+  else
+    lu_solve<NS>(a, piv, b, NS);
+}
+
 // The plain point-implicit chemistry step at fixed (T, p):
 // (I - dt*J) d(rhoY) = dt*omega, dY = d(rhoY)/rho -- the reference's explicit
 // EPI (src/FiniteRateChem.cu:211-234) applied to the chemical source alone.
