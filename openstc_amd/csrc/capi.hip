@@ -120,6 +120,7 @@ struct ostc_ctx {
   // the kernels of independent zones can share the GPU
   std::vector<hipStream_t> zstream;
   std::vector<hipEvent_t> zdone;
+  std::vector<hipEvent_t> zmid;  // a zone's fluxes are done
   hipEvent_t fork = nullptr;
 };
 
@@ -444,6 +445,7 @@ int ostc_ctx_create(const ostc_param *param, ostc_ctx **out) {
 int ostc_ctx_destroy(ostc_ctx *ctx) {
   if (!ctx) return OSTC_OK;
   for (auto e : ctx->zdone) (void)hipEventDestroy(e);
+  for (auto e : ctx->zmid) (void)hipEventDestroy(e);
   for (auto q : ctx->zstream) (void)hipStreamDestroy(q);
   if (ctx->fork) (void)hipEventDestroy(ctx->fork);
   if (ctx->dev) (void)hipFree(ctx->dev);
@@ -863,6 +865,9 @@ static int zone_streams(ostc_ctx *c, int n) {
     hipEvent_t e = nullptr;
     HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     c->zdone.push_back(e);
+    hipEvent_t m = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&m, hipEventDisableTiming));
+    c->zmid.push_back(m);
   }
   return OSTC_OK;
 }
@@ -870,7 +875,8 @@ static int zone_streams(ostc_ctx *c, int n) {
 // Everything of one iteration that touches only zone z (src/Driver.cu:
 // 224-257): source, fluxes, local dt, implicit treatment, update,
 // limit_flow, boundary conditions.
-static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen, hipStream_t st) {
+static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen, hipStream_t st,
+                          hipEvent_t fluxes_done = nullptr) {
   const CellOpts &o = c->opts;
   const bool visc = c->host.viscous_scheme != 0;
   void *s = (void *)st;
@@ -899,6 +905,7 @@ static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen
     prof_mark(P_DT, st, false);
     if (r) return r;
   }
+  if (fluxes_done) HIP_TRY(hipEventRecord(fluxes_done, st));
   if (o.implicit_method == 0) prof_mark(P_EXPL_IMPL, st, true);
   {
     // implicit treatment, then the update with the T-from-e solve
@@ -947,7 +954,12 @@ int ostc_steady_step(ostc_ctx *c, ostc_zone **zones, int n, const ostc_inner_fac
     HIP_TRY(hipEventRecord(c->fork, st));
     for (int b = 0; b < n; ++b) {
       HIP_TRY(hipStreamWaitEvent(c->zstream[b], c->fork, 0));
-      if (int r = zone_iteration(c, zones[b], step, output_screen, c->zstream[b])) return r;
+      // zones in pairs, the second starting when the first's fluxes are
+      // done: its source kernel (LDS-bound, vector registers to spare) then
+      // meets the first zone's HBM-bound block solves rather than another
+      // source kernel
+      if (b & 1) HIP_TRY(hipStreamWaitEvent(c->zstream[b], c->zmid[b - 1], 0));
+      if (int r = zone_iteration(c, zones[b], step, output_screen, c->zstream[b], c->zmid[b])) return r;
       HIP_TRY(hipEventRecord(c->zdone[b], c->zstream[b]));
     }
     for (int b = 0; b < n; ++b) HIP_TRY(hipStreamWaitEvent(st, c->zdone[b], 0));
