@@ -360,7 +360,11 @@ int ostc_zone_counters(ostc_zone *zone, long long out[4], int reset, void *strea
  * write enters it), and in non-reacting zones the first DPLUR sweep forms
  * dq0 from dq as it loads it.  inner faces of block b are
  * inner[off_b .. off_b+n_inner[b]).  Returns the residual sums in
- * res_sums[4] when step%output_screen==0 or step==1 (else leaves them). */
+ * res_sums[4] when step%output_screen==0 or step==1 (else leaves them).
+ * With n > 1 (and the stage profiler off) each zone's chain runs on a
+ * library-owned stream forked from `stream` and joined back before the
+ * inner-face exchange; zone 2k+1 starts once zone 2k's fluxes are done.
+ * The result is the same as running the zones in order on `stream`. */
 int ostc_steady_step(ostc_ctx *ctx, ostc_zone **zones, int n, const ostc_inner_face *inner, const int *n_inner,
                      int step, int output_screen, double res_sums[4], int *res_computed, void *stream);
 
