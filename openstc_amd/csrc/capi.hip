@@ -340,7 +340,7 @@ void make_chem64(const ostc_param &p, ChemF64 &out) {
     }
   for (int l = 0; l < kMaxSpec; ++l)
     for (int r = 0; r < p.n_reac && r < kMaxReac; ++r)
-      if (out.net[r] & (1u << l)) out.jr[l][out.jn[l]++] = (unsigned char)r;
+      if (out.net[r] & (1u << l)) out.jr[l][out.jn[l]++] = r;
   unsigned any = 0;
   for (int r = 0; r < p.n_reac && r < kMaxReac; ++r) any |= out.part[r];
   for (int l = 0; l < kMaxSpec; ++l) out.rank[l] = -1;

@@ -209,10 +209,11 @@ __global__ __launch_bounds__(256) void k_local_dt(ZoneView<R> z, const ostc_para
 // CR: 0 no chemistry; > 0 the two-pass chemistry for mechanisms of at most
 // CR reactions (rates of progress in 2 CR doubles of LDS per thread: 80 KB
 // per workgroup at CR = 20, two workgroups per CU); < 0 the one-pass form (any
-// reaction count).  Three waves per SIMD are asked for (the kernels without
-// the LDS reach them).
+// reaction count).  The LDS variants ask for two waves per SIMD, the most
+// their LDS allows, which leaves 256 VGPRs per lane (the EPI Jacobian's sums
+// stay in registers); the others ask for three.
 template <typename R, int NS, int NT, int CR, bool DT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_source(ZoneView<R> z,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && NS > 0 ? 2 : 3))) void k_source(ZoneView<R> z,
                                                                                         const ostc_param *__restrict__ p) {
   constexpr bool CHEM = CR != 0;
   static_assert(!DT || NT > 0, "the fused time step needs the SST branch's metric and mut");

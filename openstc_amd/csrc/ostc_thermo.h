@@ -51,9 +51,10 @@ struct ChemF64 {
   int n_act;
   int act[kMaxSpec], rank[kMaxSpec];
   // per species m, the reactions whose net coefficient of m is nonzero (the
-  // terms of row m of the EPI Jacobian), in order
+  // terms of row m of the EPI Jacobian), in order (ints, not bytes: a scalar
+  // load reads whole dwords, a byte would take a vector load)
   int jn[kMaxSpec];
-  unsigned char jr[kMaxSpec][kMaxReac];
+  int jr[kMaxSpec][kMaxReac];
 };
 constexpr size_t kChemF64Offset = (kThermoF32Offset + sizeof(ThermoF32) + 255) / 256 * 256;
 constexpr size_t kParamBlockBytes = kChemF64Offset + sizeof(ChemF64);
@@ -382,11 +383,15 @@ __device__ __forceinline__ void gibbs_div_rt(double t, double log_t, double t_in
   }
 }
 
-// integer power for stoichiometric exponents (0..3); pow(c, 0) == 1
-__device__ __forceinline__ double ipow(double c, int n) {
-  double r = 1.0;
-  for (int q = 0; q < n; ++q) r *= c;
-  return r;
+// (int)x for a non-negative integer x < 2^52 held in a double (the
+// stoichiometric coefficients), from its bit pattern: integer operations
+// only, so a wave-uniform x stays in scalar registers (a float-to-int
+// conversion would run on the vector unit)
+__device__ __forceinline__ int small_int_of(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  if (u == 0) return 0;
+  const int e = (int)(u >> 52) - 1023;  // x = 1.f * 2^e, 0 <= e < 52
+  return (int)(((u & 0xFFFFFFFFFFFFFull) | 0x10000000000000ull) >> (52 - e));
 }
 
 // Forward / backward rate of progress (mol/cm3/s) of reaction i at
@@ -445,8 +450,9 @@ __device__ __forceinline__ void reaction_rates(const ostc_param *__restrict__ p,
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     if (part & (1u << l)) {
-      for (int q = 0; q < (int)nf[l]; ++q) a *= c[l];
-      for (int q = 0; q < (int)nb[l]; ++q) b *= c[l];
+      const int ef = small_int_of(nf[l]), eb = small_int_of(nb[l]);
+      for (int q = 0; q < ef; ++q) a *= c[l];
+      for (int q = 0; q < eb; ++q) b *= c[l];
     }
   }
   q1 = a;
@@ -599,7 +605,45 @@ __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p
       }
     }
   }
-  if constexpr (JAC == 1) {
+  if constexpr (JAC == 1 && NS <= 9) {
+    // reaction-major: each reaction's rates and coefficient rows are read
+    // once and added to every row m it changes, all NS*NS sums in
+    // registers; per entry the same terms in the same (reaction) order as
+    // the row-major form below
+    double rry[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) rry[n] = y[n] > 1e-25 ? 1.0 / (rho * y[n]) : 0.0;
+    double zz[NS * NS];
+#pragma unroll
+    for (int q = 0; q < NS * NS; ++q) zz[q] = 0;
+#pragma unroll 1
+    for (int r = 0; r < nr; ++r) {
+      const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
+      const unsigned net = ct->net[r];
+      double nf[NS], nb[NS], v[NS];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        nf[n] = ct->nu_f[r * kMaxSpec + n];
+        nb[n] = ct->nu_b[r * kMaxSpec + n];
+        v[n] = nf[n] * q1 - nb[n] * q2;
+      }
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        if (net & (1u << m)) {
+          const double dm = nb[m] - nf[m];
+#pragma unroll
+          for (int n = 0; n < NS; ++n) zz[m * NS + n] += dm * v[n];
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      double row[NS];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) row[n] = y[n] > 1e-25 ? p->mw[m] * (zz[m * NS + n] * rry[n]) * 1e+3 : 0.0;
+      row_out(m, row);
+    }
+  } else if constexpr (JAC == 1) {
     double rry[NS];
 #pragma unroll
     for (int n = 0; n < NS; ++n) rry[n] = y[n] > 1e-25 ? 1.0 / (rho * y[n]) : 0.0;
