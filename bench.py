@@ -51,22 +51,16 @@ METRIC = "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem
 MEASURED_TRAFFIC = {
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2305.7e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4178.5e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 7700.3e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 5663.0e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3635.6e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4034.1e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3071.2e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.5e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "local_dt"): 537.3e6,
     ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 514.7e6,
     ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1174.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 2137.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "source"): 1363.2e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 739.8e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 840.6e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 743.9e6,
     ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.6e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "local_dt"): 144.8e6,
     ("sst_flat_plate_dplur", "dplur_sweep"): 478.6e6,
     ("sst_flat_plate_dplur", "dplur_sweep_first"): 506.3e6,
     ("sst_flat_plate_dplur", "flux_ij"): 402.8e6,
@@ -146,6 +140,10 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
     na = sum(1 for l in range(ns) if any(p.stoi_f[r * L.MAX_SPEC + l] or p.stoi_b[r * L.MAX_SPEC + l]
                                           for r in range(p.n_reac)))
     npw = (na + 7) // 8
+    # EPI under DPLUR on the 9-species kernels (at most 20 reactions): the
+    # source kernel also forms the local time step and stores the chemistry
+    # block's LU factors, which dq0 then only reads
+    fac = epi and ns == 9 and p.n_reac <= 20 and p.implicit_method == 1 and na > 0
     out = {}
 
     def put(stage, n_bulk, n_f64=0, n_i32=0):
@@ -169,13 +167,22 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
         src = 2 + ns + nv
     else:
         src = nv
-    put("source", src, njac)
+    if fac:
+        # the time step: c, gamma (+ metric 9, u, v, w, mu without SST) ->
+        # spectral radii 3, dt; the block's factors in place of the Jacobian
+        src += 2 + (0 if rans else 9 + 3 + 1) + 4
+        put("source", src, na * na, npw)
+    else:
+        put("source", src, njac)
     # local time step: metric 9, u,v,w, rho, c [mu] [mu_t] [gamma] -> spectral radii 3, dt
     put("local_dt", 9 + 3 + 2 + (1 if visc else 0) + (1 if rans else 0) + (1 if ns else 0) + 4)
     # DQ0 (reacting zones; non-reacting zones form dq0 in the first sweep):
     # dq, dt, spectral radii 3, [J_t 2] -> dq0; the reacting species' block of
-    # the Jacobian read, its LU factors + packed pivots written
-    if epi:
+    # the Jacobian read, its LU factors + packed pivots written (fac: the
+    # factors and pivots read)
+    if fac:
+        put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, na * na, npw)
+    elif epi:
         put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, 2 * na * na, npw)
     else:
         put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, njac)

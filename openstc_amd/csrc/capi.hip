@@ -347,6 +347,7 @@ void make_chem64(const ostc_param &p, ChemF64 &out) {
   for (int l = 0; l < p.n_spec && l < kMaxSpec; ++l)
     if (any & (1u << l)) {
       out.act[out.n_act] = l;
+      out.act_mask |= 1u << l;
       out.rank[l] = out.n_act++;
     }
   for (int i = 0; i < p.n_spec && i < kMaxSpec; ++i) {
@@ -889,8 +890,11 @@ static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen
   if (r) return r;
   bool wrote = false;
   // SST zones without chemistry: the source kernel also forms the local
-  // time step (it holds the metric, density, viscosity and mut)
-  const bool dt_in_source = source_has_dt(o);
+  // time step (it holds the metric, density, viscosity and mut); EPI zones
+  // under DPLUR (9-species kernels): it also factors the chemistry block, so
+  // that dq0 only solves with it
+  const bool lu_in_source = source_factors_chem(o);
+  const bool dt_in_source = source_has_dt(o) || lu_in_source;
   prof_mark(P_SOURCE, st, true);
   r = ZONE_CALL(z, source_terms(z->vf, c->dev, o, st, false, &wrote, dt_in_source),
                 source_terms(z->vd, c->dev, o, st, false, &wrote, dt_in_source));
@@ -912,14 +916,14 @@ static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen
     const bool dplur = o.implicit_method == 1;
     if (fp64(z)) {
       double *res = nullptr;
-      r = implicit_treatment(z->vd, c->dev, o, st, &res);
+      r = implicit_treatment(z->vd, c->dev, o, st, &res, lu_in_source);
       if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
       if (r) return r;
       prof_mark(P_UPDATE, st, true);
       r = update_cv_bv(z->vd, c->dev, o, res, st, z->lf_slot);
     } else {
       float *res = nullptr;
-      r = implicit_treatment(z->vf, c->dev, o, st, &res);
+      r = implicit_treatment(z->vf, c->dev, o, st, &res, lu_in_source);
       if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
       if (r) return r;
       prof_mark(P_UPDATE, st, true);

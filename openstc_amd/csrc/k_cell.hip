@@ -157,10 +157,15 @@ __global__ __launch_bounds__(256) void k_init_mut(ZoneView<R> z, const ostc_para
 
 // local_time_step (src/TimeAdvanceFunc.cuh:21-71) of one cell from its metric
 // m, velocity, density, laminar and eddy viscosity; the viscous radius is not
-// stored (the reference never reads visc_spectr_rad)
+// stored (the reference never reads visc_spectr_rad).  Returns the time step
+// and the sum of the three inviscid spectral radii it stored.
+template <typename R>
+struct CellDt {
+  R dt, sri;
+};
 template <typename R, int NS, int NT>
-__device__ __forceinline__ void local_dt_cell(const ZoneView<R> &z, const ostc_param *__restrict__ p, int64_t id,
-                                              const R *m, R u, R v, R w, R rho, R mul, R mut) {
+__device__ __forceinline__ CellDt<R> local_dt_cell(const ZoneView<R> &z, const ostc_param *__restrict__ p,
+                                                   int64_t id, const R *m, R u, R v, R w, R rho, R mul, R mut) {
   const int64_t P = z.plane;
   const R gx = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
   const R ge = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
@@ -183,7 +188,9 @@ __device__ __forceinline__ void local_dt_cell(const ZoneView<R> &z, const ostc_p
   R srv = gx * gx + ge * ge;
   if (three) srv += gz * gz;
   srv *= c1 * c2;
-  z.dt[id] = (R)p->cfl / (sri + srv);
+  const R dt = (R)p->cfl / (sri + srv);
+  z.dt[id] = dt;
+  return {dt, sri};
 }
 
 template <typename R, int NS, int NT>
@@ -196,6 +203,95 @@ __global__ __launch_bounds__(256) void k_local_dt(ZoneView<R> z, const ostc_para
   for (int q = 0; q < 9; ++q) m[q] = z.metric[q * P + id];
   local_dt_cell<R, NS, NT>(z, p, id, m, z.bv[P + id], z.bv[2 * P + id], z.bv[3 * P + id], z.bv[id], z.mul[id],
                            NT > 0 ? z.mut[id] : (R)0);
+}
+
+// The chemistry block of DPLUR, (diag I - dt J), in the space of the species
+// that react.  A species that takes part in no reaction (N2 of H2-air, the
+// inert species appended to smaller mechanisms) has a zero row and column in
+// J: a factorisation of the whole block never picks its row as a pivot (its
+// column entries are zero), its L and U entries stay zero and its U diagonal
+// stays diag, every update it enters subtracts an exact zero, and the solve
+// gives it b / diag.  So k_dq0 factors only the NA x NA block of the NA
+// species that react (ChemF64::act, in their order) and the sweeps solve that
+// block: the same values as the whole 9 x 9 factors (the sign of a zero
+// aside), with NA^2 doubles per cell instead of 81.  NA == NS takes the whole
+// block in the species' own order (the 20-species bucket, whose appended
+// inert species stay in it as zero rows and columns).  Pivots are kept as
+// positions in the block, four bits each (eight bits above 16 species),
+// packed into 32-bit planes of chem_piv.
+template <int NA>
+struct PivPack {
+  static constexpr int bits = NA > 16 ? 8 : 4, per = 32 / bits, words = (NA + per - 1) / per;
+};
+template <int NA>
+__device__ __forceinline__ void piv_pack(const int *piv, unsigned *w) {
+  using PP = PivPack<NA>;
+#pragma unroll
+  for (int q = 0; q < PP::words; ++q) w[q] = 0;
+#pragma unroll
+  for (int r = 0; r < NA; ++r) w[r / PP::per] |= (unsigned)piv[r] << (PP::bits * (r % PP::per));
+}
+template <int NA>
+__device__ __forceinline__ void piv_unpack(const unsigned *w, int *piv) {
+  using PP = PivPack<NA>;
+#pragma unroll
+  for (int r = 0; r < NA; ++r) piv[r] = (int)((w[r / PP::per] >> (PP::bits * (r % PP::per))) & ((1u << PP::bits) - 1));
+}
+// the r-th species of the block
+template <int NS, int NA>
+__device__ __forceinline__ int block_species(const ChemF64 *ct, int r) {
+  if constexpr (NA == NS)
+    return r;
+  else
+    return ct->act[r];
+}
+
+// The DPLUR chemistry block of one cell, (diag I - dt J) with diag = 1 + dt *
+// (sum of the spectral radii), from its whole EPI Jacobian a (NS x NS,
+// row-major, overwritten): factored as a whole, then stored as the factors
+// of the block of the species that react and their pivots, the layout
+// k_dq0 writes.  Every row and column of a species that reacts in no
+// reaction is zero in J, so the whole factors restricted to the species
+// that react are the block's own (the sign of a zero aside), and a pivot is
+// always a reacting species' row; its position in the block is the number
+// of reacting species before it (ChemF64::act_mask).  Up to 9 species: the
+// pivots take four bits each, PivPack's layout for blocks of up to 16.
+template <typename R, int NS>
+__device__ __forceinline__ void epi_factor_store(const ZoneView<R> &z, const ChemF64 *__restrict__ ct, int64_t id,
+                                                 R dt, R sri, double *a) {
+  static_assert(NS <= 9, "whole-block factors in registers up to 9 species");
+  const int64_t P = z.plane;
+  const R diag = (R)1 + dt * sri;
+  const double dtd = (double)dt, dg = (double)diag;
+  double d[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) d[q] = dg - dtd * a[q * NS + q];
+#pragma unroll
+  for (int q = 0; q < NS * NS; ++q) a[q] = -dtd * a[q];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) a[q * NS + q] = d[q];
+  int piv[NS];
+  lu_factor_any<NS>(a, piv);
+  const int na = ct->n_act;
+  const unsigned amask = ct->act_mask;
+  unsigned w0 = 0, w1 = 0;
+#pragma unroll
+  for (int m = 0; m < NS; ++m) {
+    const int rm = ct->rank[m];
+    if (rm < 0) continue;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const int rn = ct->rank[n];
+      if (rn >= 0) z.chem_lu[(int64_t)(rm * na + rn) * P + id] = a[m * NS + n];
+    }
+    const unsigned v = (unsigned)__popc(amask & ((1u << piv[m]) - 1u));
+    if (rm < 8)
+      w0 |= v << (4 * rm);
+    else
+      w1 |= v << (4 * (rm - 8));
+  }
+  z.chem_piv[id] = (int)w0;
+  if (na > 8) z.chem_piv[P + id] = (int)w1;
 }
 
 // ------------------------------------------------------------------ source
@@ -211,12 +307,17 @@ __global__ __launch_bounds__(256) void k_local_dt(ZoneView<R> z, const ostc_para
 // per workgroup at CR = 20, two workgroups per CU); < 0 the one-pass form (any
 // reaction count).  The LDS variants ask for two waves per SIMD, the most
 // their LDS allows, which leaves 256 VGPRs per lane (the EPI Jacobian's sums
-// stay in registers); the others ask for three.
-template <typename R, int NS, int NT, int CR, bool DT>
+// stay in registers); the others ask for three.  FAC (with DT, the two-pass
+// chemistry and up to 9 species): with the EPI source the Jacobian is not
+// stored; the kernel forms and factors the DPLUR chemistry block itself
+// (epi_factor_store), so that k_dq0 only solves with it.
+template <typename R, int NS, int NT, int CR, bool DT, bool FAC = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && NS > 0 ? 2 : 3))) void k_source(ZoneView<R> z,
                                                                                         const ostc_param *__restrict__ p) {
   constexpr bool CHEM = CR != 0;
-  static_assert(!DT || NT > 0, "the fused time step needs the SST branch's metric and mut");
+  static_assert(!FAC || (DT && CR > 0 && NS > 0 && NS <= 9), "the block factors need the time step and the two-pass chemistry");
+  // DT: the cell's time step and the sum of its spectral radii
+  R dt_c = 0, sri_c = 0;
   // CR > 0: the rates of progress of the 256 cells of the workgroup
   __shared__ double qs_mem[CR > 0 && NS > 0 ? 2 * CR * 256 : 1];
   constexpr int NV = 5 + NS + NT;
@@ -287,7 +388,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && N
       z.lambda_t[id] = mut * z.cp[id] / (R)p->Prt;
     else
       z.lambda_t[id] = mut * (R)(kGammaAir * kRu / kMwAir / (kGammaAir - 1)) / (R)p->Prt;
-    if constexpr (DT) local_dt_cell<R, NS, NT>(z, p, id, m, u[id], v[id], w[id], rho, z.mul[id], mut);
+    if constexpr (DT) {
+      const CellDt<R> q = local_dt_cell<R, NS, NT>(z, p, id, m, u[id], v[id], w[id], rho, z.mul[id], mut);
+      dt_c = q.dt;
+      sri_c = q.sri;
+    }
+  } else if constexpr (DT) {
+    R m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = z.metric[q * P + id];
+    const CellDt<R> q = local_dt_cell<R, NS, NT>(z, p, id, m, z.bv[P + id], z.bv[2 * P + id], z.bv[3 * P + id],
+                                                 z.bv[id], z.mul[id], (R)0);
+    dt_c = q.dt;
+    sri_c = q.sri;
   }
   if constexpr (CHEM && NS > 0) {
     double y[NS], omega[NS];
@@ -308,10 +421,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && N
       const int nr = min(p->n_reac, CR);
       const auto om = [&](int l, double v) { z.dq[(5 + l) * P + id] = (R)((double)J * v); };
       if (meth == 1) {
-        chem_source_lds<NS, 1>(p, nr, CR, rho, t, y, qs, 256, om, [&](int m, const double *row) {
+        if constexpr (FAC) {
+          chem_source_lds<NS, 3>(p, nr, CR, rho, t, y, qs, 256, om,
+                                 [&](int, double *a) { epi_factor_store<R, NS>(z, chem64(p), id, dt_c, sri_c, a); });
+        } else {
+          chem_source_lds<NS, 1>(p, nr, CR, rho, t, y, qs, 256, om, [&](int m, const double *row) {
 #pragma unroll
-          for (int n = 0; n < NS; ++n) jac[(m * NS + n) * P + id] = row[n];
-        });
+            for (int n = 0; n < NS; ++n) jac[(m * NS + n) * P + id] = row[n];
+          });
+        }
       } else if (meth == 2) {
         chem_source_lds<NS, 2>(p, nr, CR, rho, t, y, qs, 256, om,
                                [&](int m, const double *d) { jac[m * P + id] = *d; });
@@ -380,51 +498,12 @@ __global__ __launch_bounds__(256) void k_explicit_implicit(ZoneView<R> z, const 
   }
 }
 
-// The chemistry block of DPLUR, (diag I - dt J), in the space of the species
-// that react.  A species that takes part in no reaction (N2 of H2-air, the
-// inert species appended to smaller mechanisms) has a zero row and column in
-// J: a factorisation of the whole block never picks its row as a pivot (its
-// column entries are zero), its L and U entries stay zero and its U diagonal
-// stays diag, every update it enters subtracts an exact zero, and the solve
-// gives it b / diag.  So k_dq0 factors only the NA x NA block of the NA
-// species that react (ChemF64::act, in their order) and the sweeps solve that
-// block: the same values as the whole 9 x 9 factors (the sign of a zero
-// aside), with NA^2 doubles per cell instead of 81.  NA == NS takes the whole
-// block in the species' own order (the 20-species bucket, whose appended
-// inert species stay in it as zero rows and columns).  Pivots are kept as
-// positions in the block, four bits each (eight bits above 16 species),
-// packed into 32-bit planes of chem_piv.
-template <int NA>
-struct PivPack {
-  static constexpr int bits = NA > 16 ? 8 : 4, per = 32 / bits, words = (NA + per - 1) / per;
-};
-template <int NA>
-__device__ __forceinline__ void piv_pack(const int *piv, unsigned *w) {
-  using PP = PivPack<NA>;
-#pragma unroll
-  for (int q = 0; q < PP::words; ++q) w[q] = 0;
-#pragma unroll
-  for (int r = 0; r < NA; ++r) w[r / PP::per] |= (unsigned)piv[r] << (PP::bits * (r % PP::per));
-}
-template <int NA>
-__device__ __forceinline__ void piv_unpack(const unsigned *w, int *piv) {
-  using PP = PivPack<NA>;
-#pragma unroll
-  for (int r = 0; r < NA; ++r) piv[r] = (int)((w[r / PP::per] >> (PP::bits * (r % PP::per))) & ((1u << PP::bits) - 1));
-}
-// the r-th species of the block
-template <int NS, int NA>
-__device__ __forceinline__ int block_species(const ChemF64 *ct, int r) {
-  if constexpr (NA == NS)
-    return r;
-  else
-    return ct->act[r];
-}
-
 // compute_DQ_0 (src/DPLURHPP.cuh:13-76): dq0 = dq / diag with the chemistry
 // (EPI_for_dq0, src/FiniteRateChem.cu:236-253, LU factors kept for the sweeps)
-// and SST (src/SST.cu:196-204) blocks treated point-implicitly.
-template <typename R, int NS, int NT, bool CHEM, int NA = NS>
+// and SST (src/SST.cu:196-204) blocks treated point-implicitly.  LU_READY:
+// the source kernel has stored the chemistry block's factors (k_source with
+// FAC); they are read instead of formed.
+template <typename R, int NS, int NT, bool CHEM, int NA = NS, bool LU_READY = false>
 __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__restrict__ p) {
   const auto c = cell_index(z, 0);
   if (!c.ok) return;
@@ -441,22 +520,33 @@ __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__
       const ChemF64 *ct = chem64(p);
       double a[NA * NA], b[NA];
       int piv[NA];
+      if constexpr (LU_READY) {
 #pragma unroll
-      for (int r = 0; r < NA; ++r) {
-        const int m = block_species<NS, NA>(ct, r);
+        for (int q = 0; q < NA * NA; ++q) a[q] = z.chem_lu[q * P + id];
+        unsigned w[PivPack<NA>::words];
 #pragma unroll
-        for (int c2 = 0; c2 < NA; ++c2)
-          a[r * NA + c2] = -dtd * z.chem_jac[(m * NS + block_species<NS, NA>(ct, c2)) * P + id];
-        a[r * NA + r] = dg - dtd * z.chem_jac[(m * NS + m) * P + id];
-        b[r] = (double)z.dq[(5 + m) * P + id];
+        for (int q = 0; q < PivPack<NA>::words; ++q) w[q] = (unsigned)z.chem_piv[q * P + id];
+        piv_unpack<NA>(w, piv);
+#pragma unroll
+        for (int r = 0; r < NA; ++r) b[r] = (double)z.dq[(5 + block_species<NS, NA>(ct, r)) * P + id];
+      } else {
+#pragma unroll
+        for (int r = 0; r < NA; ++r) {
+          const int m = block_species<NS, NA>(ct, r);
+#pragma unroll
+          for (int c2 = 0; c2 < NA; ++c2)
+            a[r * NA + c2] = -dtd * z.chem_jac[(m * NS + block_species<NS, NA>(ct, c2)) * P + id];
+          a[r * NA + r] = dg - dtd * z.chem_jac[(m * NS + m) * P + id];
+          b[r] = (double)z.dq[(5 + m) * P + id];
+        }
+        lu_factor_any<NA>(a, piv);
+#pragma unroll
+        for (int q = 0; q < NA * NA; ++q) z.chem_lu[q * P + id] = a[q];
+        unsigned w[PivPack<NA>::words];
+        piv_pack<NA>(piv, w);
+#pragma unroll
+        for (int q = 0; q < PivPack<NA>::words; ++q) z.chem_piv[q * P + id] = (int)w[q];
       }
-      lu_factor_any<NA>(a, piv);
-#pragma unroll
-      for (int q = 0; q < NA * NA; ++q) z.chem_lu[q * P + id] = a[q];
-      unsigned w[PivPack<NA>::words];
-      piv_pack<NA>(piv, w);
-#pragma unroll
-      for (int q = 0; q < PivPack<NA>::words; ++q) z.chem_piv[q * P + id] = (int)w[q];
       lu_solve_any<NA>(a, piv, b);
 #pragma unroll
       for (int r = 0; r < NA; ++r) z.dq0[(5 + block_species<NS, NA>(ct, r)) * P + id] = (R)b[r];
@@ -1433,12 +1523,17 @@ int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, 
                  bool *wrote, bool with_dt) {
   const bool chem = o.mix_model == OSTC_FR;
   *wrote = false;
-  if (with_dt && !source_has_dt(o)) return OSTC_ERR_ARG;
+  if (with_dt && !source_has_dt(o) && !source_factors_chem(o)) return OSTC_ERR_ARG;
   if (o.nt == 0 && !chem && !force) return OSTC_OK;
   const auto g = cell_grid(z.mx, z.my, z.mz);
   if (chem) {
     // mechanisms of up to 20 reactions take the two-pass chemistry
-    if (o.n_reac <= 20)
+    if (with_dt) {
+      if (o.nt == 2)
+        hipLaunchKernelGGL((k_source<R, 9, 2, 20, true, true>), g.grid, g.block, 0, st, z, dp);
+      else
+        hipLaunchKernelGGL((k_source<R, 9, 0, 20, true, true>), g.grid, g.block, 0, st, z, dp);
+    } else if (o.n_reac <= 20)
       OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, 20, false>), g.grid, g.block, 0, st, z, dp));
     else
       OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, -1, false>), g.grid, g.block, 0, st, z, dp));
@@ -1464,7 +1559,8 @@ int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &
 }
 
 template <typename R>
-int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result) {
+int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result,
+                       bool lu_ready) {
   const auto g = cell_grid(z.mx, z.my, z.mz);
   const bool chem = o.mix_model == OSTC_FR;
   *result = z.dq;
@@ -1487,7 +1583,13 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
   if (!fuse_dq0) {
     prof_mark(P_DQ0, st, true);
     if (chem) {
-      if (o.ns == 9 && o.chem == 1) {
+      if (o.ns == 9 && o.chem == 1 && lu_ready) {
+        // EPI block factored by the source kernel: the solve only
+        if (o.nt == 2)
+          OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 2, true, NA_, true>), g.grid, g.block, 0, st, z, dp));
+        else
+          OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 0, true, NA_, true>), g.grid, g.block, 0, st, z, dp));
+      } else if (o.ns == 9 && o.chem == 1) {
         // EPI block: factored in the space of the species that react
         if (o.nt == 2)
           OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 2, true, NA_>), g.grid, g.block, 0, st, z, dp));
@@ -1641,7 +1743,7 @@ int wall_distance(const ZoneView<R> &z, const double *wall_xyz, int n_wall, hipS
                                 bool *, bool);                                                                  \
   template int local_time_step<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t);      \
   template int implicit_treatment<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t,    \
-                                     R **);                                                                     \
+                                     R **, bool);                                                               \
   template int update_cv_bv<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, const R *,            \
                                hipStream_t, int);                                                               \
   template int limit_flow<R>(const ZoneView<R> &, const ostc_param *, const CellOpts &, hipStream_t, int);      \

@@ -47,16 +47,27 @@ template <typename R>
 int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
 // writes dq (first writer; zeros when the model has no source and force is set);
 // with_dt: also the local time step (spectr, dt), for the zones where
-// source_has_dt holds -- the time step then skips local_time_step
+// source_has_dt or source_factors_chem holds -- the time step then skips
+// local_time_step; in the second case the source also stores the factors of
+// the DPLUR chemistry block instead of the EPI Jacobian, and
+// implicit_treatment is called with lu_ready
 template <typename R>
 int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, bool force,
                  bool *wrote, bool with_dt = false);
 inline bool source_has_dt(const CellOpts &o) { return o.nt > 0 && o.mix_model != OSTC_FR; }
+// EPI chemistry under DPLUR on the 9-species kernels with the two-pass
+// chemistry (at most 20 reactions)
+inline bool source_factors_chem(const CellOpts &o) {
+  return o.mix_model == OSTC_FR && o.chem == 1 && o.implicit_method == 1 && o.ns == 9 && o.n_reac <= 20 &&
+         o.n_act > 0;
+}
 template <typename R>
 int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
-// leaves the final increment in *result (one of z.dq/z.dq0/z.dqk)
+// leaves the final increment in *result (one of z.dq/z.dq0/z.dqk); lu_ready:
+// the source kernel stored the chemistry block's factors (source_factors_chem)
 template <typename R>
-int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result);
+int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result,
+                       bool lu_ready = false);
 // check_slot >= 0: also classify every cell for limit_flow into repair-list
 // slot (check_slot & 1); then call limit_flow with the same slot.  Time steps
 // alternate the slot.

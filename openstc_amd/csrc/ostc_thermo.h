@@ -50,6 +50,7 @@ struct ChemF64 {
   // row and column of the EPI Jacobian are zero), n_act their count
   int n_act;
   int act[kMaxSpec], rank[kMaxSpec];
+  unsigned act_mask;  // bit k: species k reacts (rank[k] = popcount of the bits below k)
   // per species m, the reactions whose net coefficient of m is nonzero (the
   // terms of row m of the EPI Jacobian), in order (ints, not bytes: a scalar
   // load reads whole dwords, a byte would take a vector load)
@@ -543,16 +544,19 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
 // reaction's rates of progress, kept in LDS (q1 of reaction r at
 // qs[r * stride], q2 at qs[(nr_max + r) * stride]; stride = the workgroup's
 // thread count, qs already offset to this thread), then omega and the
-// Jacobian row by row.  Each finished row goes to row_out(m, row) at once, so
-// only one row of the 9x9 matrix is live -- against the whole matrix for the
-// one-pass form, which holds a kernel to one wave per SIMD -- and every loop
-// over the reactions is a real loop with one copy of the rate law (rates of
-// progress in registers need the reaction loops unrolled: 20 copies of the
-// rate law, ~0.5 MB of code through a 64 KB instruction cache).  Sums run over
-// the reactions in the same order as the one-pass form and, as it does, over
-// all species (one that takes no part in a reaction adds exact zeros); row m
-// visits only the reactions that change species m (ChemF64::jr).  JAC: 1 EPI rows, 2 DA diagonal (row_out(m, &d) with the diagonal
-// entry), 0 none; omega_out(k, omega_k) receives the production rates.
+// Jacobian.  Every loop over the reactions is a real loop with one copy of
+// the rate law (rates of progress in registers need the reaction loops
+// unrolled: 20 copies of the rate law, ~0.5 MB of code through a 64 KB
+// instruction cache).  Sums run over the reactions in the same order as the
+// one-pass form and, as it does, over all species (one that takes no part in
+// a reaction adds exact zeros).  Up to 9 species the Jacobian is summed
+// reaction-major with the whole matrix in registers (the kernels that call
+// this leave 256 VGPRs per lane); above 9, row by row, row m visiting only
+// the reactions that change species m (ChemF64::jr).  JAC: 1 EPI rows, each
+// finished row to row_out(m, row); 3 (up to 9 species) the whole EPI matrix,
+// row-major, in one call row_out(0, jac) that may overwrite it; 2 DA
+// diagonal (row_out(m, &d) with the diagonal entry); 0 none.
+// omega_out(k, omega_k) receives the production rates.
 template <int NS, int JAC, class RowOut, class OmegaOut>
 __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p, int nr, int nr_max, double rho,
                                                 double t, const double *y, double *qs, int stride,
@@ -605,7 +609,8 @@ __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p
       }
     }
   }
-  if constexpr (JAC == 1 && NS <= 9) {
+  static_assert(JAC != 3 || NS <= 9, "the whole-matrix output is for up to 9 species");
+  if constexpr ((JAC == 1 || JAC == 3) && NS <= 9) {
     // reaction-major: each reaction's rates and coefficient rows are read
     // once and added to every row m it changes, all NS*NS sums in
     // registers; per entry the same terms in the same (reaction) order as
@@ -638,10 +643,15 @@ __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p
     }
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
-      double row[NS];
 #pragma unroll
-      for (int n = 0; n < NS; ++n) row[n] = y[n] > 1e-25 ? p->mw[m] * (zz[m * NS + n] * rry[n]) * 1e+3 : 0.0;
-      row_out(m, row);
+      for (int n = 0; n < NS; ++n)
+        zz[m * NS + n] = y[n] > 1e-25 ? p->mw[m] * (zz[m * NS + n] * rry[n]) * 1e+3 : 0.0;
+    }
+    if constexpr (JAC == 3) {
+      row_out(0, zz);
+    } else {
+#pragma unroll
+      for (int m = 0; m < NS; ++m) row_out(m, zz + m * NS);
     }
   } else if constexpr (JAC == 1) {
     double rry[NS];
