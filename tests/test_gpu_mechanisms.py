@@ -176,3 +176,53 @@ def test_mixing_layer_7_species_n2_inside_fp64(mech_dir_n2_inside):
         pr.orc.step()
     compare(pr, 0, 1e-9, 1e-12, "7 species (N2 second) FP64 after 5 iterations")
     pr.close()
+
+
+def write_n2_collider(tmp):
+    """The shipped 9-species mechanism with a 20th reaction in which N2 is a
+    specific collision partner, N2+H2=N2+H+H (the H2 dissociation rate): N2
+    takes part in a reaction with no net change, so all nine species enter
+    the EPI block -- N2 with a zero row and a nonzero column -- and the
+    block's pivots take two packed words."""
+    d = os.path.join(tmp, "chemkin9_n2")
+    os.makedirs(d, exist_ok=True)
+    src = open(os.path.join(cases.DATA, "combustion_mechanism.inp")).read().splitlines()
+    end = max(i for i, line in enumerate(src) if line.strip().upper() == "END")
+    out = src[:end] + ["N2+H2=N2+H+H    4.58E+19   -1.40    1.04E+05"] + src[end:]
+    with open(os.path.join(d, "combustion_mechanism.inp"), "w") as f:
+        f.write("\n".join(out) + "\n")
+    for name in ("therm.dat", "tran.dat"):
+        with open(os.path.join(cases.DATA, name)) as a, open(os.path.join(d, name), "w") as b:
+            b.write(a.read())
+    return d
+
+
+@pytest.fixture(scope="module")
+def mech_dir_n2_collider(tmp_path_factory):
+    return write_n2_collider(str(tmp_path_factory.mktemp("mech_n2c")))
+
+
+def test_mixing_layer_all_nine_react_fp64(mech_dir_n2_collider):
+    mech, info = cases.load_mechanism(True, mech_dir_n2_collider)
+    assert mech.n_spec == 9 and mech.n_reac == 20
+    case = cases.mixing_layer(mx=24, my=16, mz=8, cfl=0.2, mech_dir=mech_dir_n2_collider)
+    say("mixing_layer 24x16x8, 9 species / 20 reactions (N2 a collider), FP64, 5 iterations")
+    pr = Pair(case, precision=L.FP64)
+    pr.init()
+    for _ in range(5):
+        pr.gpu.step()
+        pr.orc.step()
+    compare(pr, 0, 1e-9, 1e-12, "9 species (N2 a collider) FP64 after 5 iterations")
+    pr.close()
+
+
+def test_mixing_layer_all_nine_react_fp32(mech_dir_n2_collider):
+    case = cases.mixing_layer(mx=24, my=16, mz=8, cfl=0.2, mech_dir=mech_dir_n2_collider)
+    say("mixing_layer 24x16x8, 9 species / 20 reactions (N2 a collider), FP32 bulk vs FP64 oracle, 10 iterations")
+    pr = Pair(case, precision=L.FP32)
+    pr.init()
+    for _ in range(10):
+        pr.gpu.step()
+        pr.orc.step()
+    compare(pr, 0, 2e-5, 2e-6, "9 species (N2 a collider) FP32 after 10 iterations")
+    pr.close()
