@@ -49,23 +49,27 @@ METRIC = "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem
 # Re-measured whenever the kernel changes; a stage without an entry reports
 # null.
 MEASURED_TRAFFIC = {
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2305.7e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4178.5e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3635.6e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4034.1e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3071.2e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.5e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 514.7e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1174.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 739.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 840.6e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 743.9e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.6e6,
-    ("sst_flat_plate_dplur", "dplur_sweep"): 478.6e6,
-    ("sst_flat_plate_dplur", "dplur_sweep_first"): 506.3e6,
-    ("sst_flat_plate_dplur", "flux_ij"): 402.8e6,
-    ("sst_flat_plate_dplur", "flux_k"): 402.2e6,
-    ("sst_flat_plate_dplur", "source"): 358.1e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2308.5e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4179.6e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 4350.6e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 4979.5e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3645.1e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4038.1e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3073.1e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 512.1e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1174.9e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 1182.1e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "source"): 1265.3e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 738.6e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 838.9e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 744.7e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.9e6,
+    ("sst_flat_plate_dplur", "dplur_sweep"): 480.6e6,
+    ("sst_flat_plate_dplur", "dplur_sweep_first"): 502.1e6,
+    ("sst_flat_plate_dplur", "flux_ij"): 402.2e6,
+    ("sst_flat_plate_dplur", "flux_k"): 402.5e6,
+    ("sst_flat_plate_dplur", "source"): 360.0e6,
     ("sst_flat_plate_dplur", "update"): 273.5e6,
 }
 
