@@ -1,7 +1,8 @@
 // Boundary conditions (src/BoundCond.cuh:45-412) and the on-device inner-face
 // halo exchange (src/DataCommunication.cuh:27-98).  Each face is one launch
-// over its 2-D tangential range; the ngg ghost layers are filled by the same
-// thread.
+// over its face points and layers, one thread per (point, layer).
+#include <algorithm>
+
 #include "ostc_kernels.h"
 #include "ostc_thermo.h"
 
@@ -452,10 +453,23 @@ template <typename R>
 int inner_halo(const ZoneView<R> &z, const ZoneView<R> &t, const ostc_param *dp, const CellOpts &o,
                const ostc_inner_face &f, hipStream_t st) {
   unsigned tpb[3], bpg[3];
-  for (int q = 0; q < 3; ++q) {
-    tpb[q] = f.n_point[q] <= 2 * z.g + 1 ? 1 : 16;
-    bpg[q] = (f.n_point[q] - 1) / tpb[q] + 1;
+  const int fd = f.face;
+  if (fd >= 0 && fd < 3 && f.n_point[fd] <= 2 * z.g + 1) {
+    // all layers of a face point in one workgroup, on adjacent lanes: along
+    // i (an i face) a row's layers then share their cache lines; i first
+    // among the other two directions, where it is one of them
+    tpb[fd] = f.n_point[fd];
+    int budget = 256 / (int)tpb[fd];
+    for (int q = 0; q < 3; ++q) {
+      if (q == fd) continue;
+      const int want = q == 0 ? 64 : 16;
+      tpb[q] = (unsigned)std::max(1, std::min(std::min(want, budget), f.n_point[q]));
+      budget = std::max(1, budget / (int)tpb[q]);
+    }
+  } else {
+    for (int q = 0; q < 3; ++q) tpb[q] = f.n_point[q] <= 2 * z.g + 1 ? 1 : 16;
   }
+  for (int q = 0; q < 3; ++q) bpg[q] = (f.n_point[q] - 1) / tpb[q] + 1;
   const dim3 grid(bpg[0], bpg[1], bpg[2]), block(tpb[0], tpb[1], tpb[2]);
   if (o.ns == 0 && o.nt == 0)
     hipLaunchKernelGGL((k_inner<R, 0, 0>), grid, block, 0, st, z, t, dp, f);
