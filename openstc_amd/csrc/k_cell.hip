@@ -59,12 +59,18 @@ __global__ __launch_bounds__(256) void k_cv_from_bv(ZoneView<R> z, const ostc_pa
   z.vel[id] = sqrt(vel2);
 }
 
-// update_physical_properties (src/FieldOperation.cuh:75-109); interior + 1
+// update_physical_properties (src/FieldOperation.cuh:75-109); interior + 1.
+// One thread per point of the (mx+2) x (my+2) x (mz+2) box, numbered i
+// fastest, so that every wave but the last is full (rows of mx+2 points in
+// 64-lane workgroup rows would leave most of a fifth wave idle at mx = 256).
 template <typename R, int NS>
 __global__ __launch_bounds__(256) void k_props(ZoneView<R> z, const ostc_param *__restrict__ p) {
-  const auto c = cell_index(z, 1);
-  if (!c.ok) return;
-  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const unsigned ni = (unsigned)z.mx + 2, nj = (unsigned)z.my + 2, nk = (unsigned)z.mz + 2;
+  const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ni * nj * nk) return;
+  const unsigned row = q / ni;
+  const int ci = (int)(q - row * ni) - 1, cj = (int)(row % nj) - 1, ck = (int)(row / nj) - 1;
+  const int64_t id = z.idx(ci, cj, ck), P = z.plane;
   const R t = z.bv[5 * P + id];
   if constexpr (NS > 0) {
     R cp[NS], y[NS];
@@ -1491,13 +1497,15 @@ int cv_from_bv(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hi
 
 template <typename R>
 int physical_properties(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st) {
-  const auto g = cell_grid(z.mx + 2, z.my + 2, z.mz + 2);
+  const long long n = (long long)(z.mx + 2) * (z.my + 2) * (z.mz + 2);
+  if (n >= (1ll << 31)) return OSTC_ERR_ARG;  // the kernel numbers the points in 32 bits
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
   if (o.ns == 0)
-    hipLaunchKernelGGL((k_props<R, 0>), g.grid, g.block, 0, st, z, dp);
+    hipLaunchKernelGGL((k_props<R, 0>), grid, block, 0, st, z, dp);
   else if (o.ns == 9)
-    hipLaunchKernelGGL((k_props<R, 9>), g.grid, g.block, 0, st, z, dp);
+    hipLaunchKernelGGL((k_props<R, 9>), grid, block, 0, st, z, dp);
   else if (o.ns == 20)
-    hipLaunchKernelGGL((k_props<R, 20>), g.grid, g.block, 0, st, z, dp);
+    hipLaunchKernelGGL((k_props<R, 20>), grid, block, 0, st, z, dp);
   else
     return OSTC_ERR_ARG;
   return last_err();
