@@ -498,7 +498,10 @@ int ostc_zone_create(ostc_ctx *ctx, const ostc_zone_desc *d, ostc_zone **out) {
     z->planes[f] = pl[f];
     z->fsize[f] = (f == OSTC_F_X || f == OSTC_F_Y || f == OSTC_F_Z || f == OSTC_F_CHEM_JAC) ? 8 : z->esize;
     if (pl[f] == 0) continue;
-    const size_t bytes = (size_t)z->plane * pl[f] * z->fsize[f];
+    size_t bytes = (size_t)z->plane * pl[f] * z->fsize[f];
+    // the LU factors share these planes and may be stored in tiles of 64
+    // cells, which reach up to 63 cells' entries beyond them
+    if (f == OSTC_F_CHEM_JAC) bytes += (size_t)64 * pl[f] * z->fsize[f];
     hipError_t e = hipMalloc(&z->ptr[f], bytes);
     if (e == hipSuccess) e = hipMemset(z->ptr[f], 0, bytes);
     if (e != hipSuccess) {

@@ -288,7 +288,7 @@ __device__ __forceinline__ void epi_factor_store(const ZoneView<R> &z, const Che
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const int rn = ct->rank[n];
-      if (rn >= 0) z.chem_lu[(int64_t)(rm * na + rn) * P + id] = a[m * NS + n];
+      if (rn >= 0) z.chem_lu[lu_tiled(id, rm * na + rn, na * na)] = a[m * NS + n];
     }
     const unsigned v = (unsigned)__popc(amask & ((1u << piv[m]) - 1u));
     if (rm < 8)
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__
       int piv[NA];
       if constexpr (LU_READY) {
 #pragma unroll
-        for (int q = 0; q < NA * NA; ++q) a[q] = z.chem_lu[q * P + id];
+        for (int q = 0; q < NA * NA; ++q) a[q] = z.chem_lu[lu_tiled(id, q, NA * NA)];
         unsigned w[PivPack<NA>::words];
 #pragma unroll
         for (int q = 0; q < PivPack<NA>::words; ++q) w[q] = (unsigned)z.chem_piv[q * P + id];
@@ -638,7 +638,7 @@ __device__ __forceinline__ void dplur_finish(const ZoneView<R> &z, const ostc_pa
 // in place.  A kernel of its own: one streaming pass over the 9x9 FP64
 // factors with every load issued at once, instead of loads that the sweep's
 // neighbour exchange would leave exposed at the end of each plane.
-template <typename R, int NS, int NA>
+template <typename R, int NS, int NA, bool TILED = false>
 __global__ __launch_bounds__(256) void k_dplur_chem(ZoneView<R> z, const ostc_param *__restrict__ p,
                                                     R *__restrict__ dout) {
   const auto c = cell_index(z, 0);
@@ -651,7 +651,7 @@ __global__ __launch_bounds__(256) void k_dplur_chem(ZoneView<R> z, const ostc_pa
   int piv[NA];
   unsigned w[PivPack<NA>::words];
 #pragma unroll
-  for (int q = 0; q < NA * NA; ++q) a[q] = z.chem_lu[q * P + id];
+  for (int q = 0; q < NA * NA; ++q) a[q] = z.chem_lu[TILED ? lu_tiled(id, q, NA * NA) : q * P + id];
 #pragma unroll
   for (int q = 0; q < PivPack<NA>::words; ++q) w[q] = (unsigned)z.chem_piv[q * P + id];
   piv_unpack<NA>(w, piv);
@@ -1656,9 +1656,13 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
     if (last_err()) return OSTC_ERR_HIP;
     if (chem && o.chem == 1 && o.ns > 0) {
       prof_mark(P_DPLUR_CHEM, st, true);
-      if (o.ns == 9)
-        OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dplur_chem<R, 9, NA_>), g.grid, g.block, 0, st, z, dp, out));
-      else if (o.ns == 20)
+      if (o.ns == 9) {
+        // factors stored by the source kernel are tiled (lu_tiled)
+        if (lu_ready)
+          OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dplur_chem<R, 9, NA_, true>), g.grid, g.block, 0, st, z, dp, out));
+        else
+          OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dplur_chem<R, 9, NA_>), g.grid, g.block, 0, st, z, dp, out));
+      } else if (o.ns == 20)
         hipLaunchKernelGGL((k_dplur_chem<R, 20, 20>), g.grid, g.block, 0, st, z, dp, out);
       else
         return OSTC_ERR_ARG;

@@ -81,6 +81,17 @@ struct ZoneView {
   }
 };
 
+// Tiled storage of the DPLUR chemistry block's LU factors (the zones whose
+// source kernel stores them, where they do not share the Jacobian's planes):
+// entry q (of nq) of the cell at padded index id, in tiles of 64 consecutive
+// cells -- a wave row -- each tile holding its cells' entry q as one run of 64
+// doubles.  A wave's loads of the whole block read one contiguous stretch of
+// nq * 512 B instead of nq runs a plane apart.  Indices reach
+// ceil(P / 64) * 64 * nq, beyond P * nq by up to 63 * nq.
+__host__ __device__ __forceinline__ int64_t lu_tiled(int64_t id, int q, int nq) {
+  return (id & ~(int64_t)63) * nq + (int64_t)q * 64 + (id & 63);
+}
+
 template <typename T>
 __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
 template <typename T>
