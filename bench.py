@@ -50,17 +50,11 @@ METRIC = "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem
 # null.
 MEASURED_TRAFFIC = {
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2308.5e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4179.6e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 4350.6e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 4979.5e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3645.1e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4038.1e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3073.1e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.8e6,
     ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 512.1e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1174.9e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 1182.1e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "source"): 1265.3e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 738.6e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 838.9e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 744.7e6,
@@ -145,8 +139,9 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
                                           for r in range(p.n_reac)))
     npw = (na + 7) // 8
     # EPI under DPLUR on the 9-species kernels (at most 20 reactions): the
-    # source kernel also forms the local time step and stores the chemistry
-    # block's LU factors, which dq0 then only reads
+    # source kernel forms the SST source and the local time step only, and
+    # the chemistry follows the fluxes in the dq0 stage, which also stores
+    # the chemistry block's LU factors
     fac = epi and ns == 9 and p.n_reac <= 20 and p.implicit_method == 1 and na > 0
     out = {}
 
@@ -172,10 +167,15 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
     else:
         src = nv
     if fac:
-        # the time step: c, gamma (+ metric 9, u, v, w, mu without SST) ->
-        # spectral radii 3, dt; the block's factors in place of the Jacobian
-        src += 2 + (0 if rans else 9 + 3 + 1) + 4
-        put("source", src, na * na, npw)
+        # no chemistry: SST (as above, without T and Y) or nothing, dq
+        # written; the time step: c, gamma (+ metric 9, u, v, w, rho, mu
+        # without SST) -> spectral radii 3, dt
+        if rans:
+            src = (4 + 2 + 9 + 1 + 1 + 1 + 2) + (2 + nv + 2)
+        else:
+            src = 9 + 3 + 1 + 1 + nv
+        src += 2 + 4
+        put("source", src)
     else:
         put("source", src, njac)
     # local time step: metric 9, u,v,w, rho, c [mu] [mu_t] [gamma] -> spectral radii 3, dt
@@ -185,7 +185,10 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
     # the Jacobian read, its LU factors + packed pivots written (fac: the
     # factors and pivots read)
     if fac:
-        put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, na * na, npw)
+        # the chemistry after the fluxes with dq0: rho, T, Y, J, dq, dt,
+        # spectral radii 3, [J_t 2] -> dq0 (the species rows written twice and
+        # read once in between), the block's factors + packed pivots
+        put("dplur_dq0", 2 + ns + 1 + nv + 1 + 3 + (2 if rans else 0) + nv + 2 * ns, na * na, npw)
     elif epi:
         put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, 2 * na * na, npw)
     else:

@@ -894,8 +894,9 @@ static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen
   bool wrote = false;
   // SST zones without chemistry: the source kernel also forms the local
   // time step (it holds the metric, density, viscosity and mut); EPI zones
-  // under DPLUR (9-species kernels): it also factors the chemistry block, so
-  // that dq0 only solves with it
+  // under DPLUR (9-species kernels): so does the SST-only source kernel, and
+  // the chemistry source follows the fluxes, fused with dq0 and the
+  // factorisation of the chemistry block (no Jacobian stored)
   const bool lu_in_source = source_factors_chem(o);
   const bool dt_in_source = source_has_dt(o) || lu_in_source;
   prof_mark(P_SOURCE, st, true);

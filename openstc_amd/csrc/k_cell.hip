@@ -253,20 +253,11 @@ __device__ __forceinline__ int block_species(const ChemF64 *ct, int r) {
 }
 
 // The DPLUR chemistry block of one cell, (diag I - dt J) with diag = 1 + dt *
-// (sum of the spectral radii), from its whole EPI Jacobian a (NS x NS,
-// row-major, overwritten): factored as a whole, then stored as the factors
-// of the block of the species that react and their pivots, the layout
-// k_dq0 writes.  Every row and column of a species that reacts in no
-// reaction is zero in J, so the whole factors restricted to the species
-// that react are the block's own (the sign of a zero aside), and a pivot is
-// always a reacting species' row; its position in the block is the number
-// of reacting species before it (ChemF64::act_mask).  Up to 9 species: the
-// pivots take four bits each, PivPack's layout for blocks of up to 16.
+// (sum of the spectral radii), formed from its whole EPI Jacobian a (NS x NS,
+// row-major) in place and factored as a whole (pivots as row indices).
 template <typename R, int NS>
-__device__ __forceinline__ void epi_factor_store(const ZoneView<R> &z, const ChemF64 *__restrict__ ct, int64_t id,
-                                                 R dt, R sri, double *a) {
+__device__ __forceinline__ void epi_factor_whole(R dt, R sri, double *a, int *piv) {
   static_assert(NS <= 9, "whole-block factors in registers up to 9 species");
-  const int64_t P = z.plane;
   const R diag = (R)1 + dt * sri;
   const double dtd = (double)dt, dg = (double)diag;
   double d[NS];
@@ -276,8 +267,23 @@ __device__ __forceinline__ void epi_factor_store(const ZoneView<R> &z, const Che
   for (int q = 0; q < NS * NS; ++q) a[q] = -dtd * a[q];
 #pragma unroll
   for (int q = 0; q < NS; ++q) a[q * NS + q] = d[q];
-  int piv[NS];
   lu_factor_any<NS>(a, piv);
+}
+
+// The whole block's factors a and pivots stored as the factors of the block
+// of the species that react (tiled, lu_tiled) and their pivots, what the
+// sweeps' block solve reads.  Every row and column of a species that reacts
+// in no reaction is zero in J, so the whole factors restricted to the
+// species that react are the block's own (the sign of a zero aside), and a
+// pivot is always a reacting species' row; its position in the block is the
+// number of reacting species before it (ChemF64::act_mask).  Up to 9
+// species: the pivots take four bits each, PivPack's layout for blocks of up
+// to 16.
+template <typename R, int NS>
+__device__ __forceinline__ void epi_store_factors(const ZoneView<R> &z, const ChemF64 *__restrict__ ct, int64_t id,
+                                                  const double *a, const int *piv) {
+  static_assert(NS <= 9, "whole-block factors in registers up to 9 species");
+  const int64_t P = z.plane;
   const int na = ct->n_act;
   const unsigned amask = ct->act_mask;
   unsigned w0 = 0, w1 = 0;
@@ -313,17 +319,12 @@ __device__ __forceinline__ void epi_factor_store(const ZoneView<R> &z, const Che
 // per workgroup at CR = 20, two workgroups per CU); < 0 the one-pass form (any
 // reaction count).  The LDS variants ask for two waves per SIMD, the most
 // their LDS allows, which leaves 256 VGPRs per lane (the EPI Jacobian's sums
-// stay in registers); the others ask for three.  FAC (with DT, the two-pass
-// chemistry and up to 9 species): with the EPI source the Jacobian is not
-// stored; the kernel forms and factors the DPLUR chemistry block itself
-// (epi_factor_store), so that k_dq0 only solves with it.
-template <typename R, int NS, int NT, int CR, bool DT, bool FAC = false>
+// stay in registers); the others ask for three.
+template <typename R, int NS, int NT, int CR, bool DT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && NS > 0 ? 2 : 3))) void k_source(ZoneView<R> z,
                                                                                         const ostc_param *__restrict__ p) {
   constexpr bool CHEM = CR != 0;
-  static_assert(!FAC || (DT && CR > 0 && NS > 0 && NS <= 9), "the block factors need the time step and the two-pass chemistry");
-  // DT: the cell's time step and the sum of its spectral radii
-  R dt_c = 0, sri_c = 0;
+
   // CR > 0: the rates of progress of the 256 cells of the workgroup
   __shared__ double qs_mem[CR > 0 && NS > 0 ? 2 * CR * 256 : 1];
   constexpr int NV = 5 + NS + NT;
@@ -394,19 +395,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && N
       z.lambda_t[id] = mut * z.cp[id] / (R)p->Prt;
     else
       z.lambda_t[id] = mut * (R)(kGammaAir * kRu / kMwAir / (kGammaAir - 1)) / (R)p->Prt;
-    if constexpr (DT) {
-      const CellDt<R> q = local_dt_cell<R, NS, NT>(z, p, id, m, u[id], v[id], w[id], rho, z.mul[id], mut);
-      dt_c = q.dt;
-      sri_c = q.sri;
-    }
+    if constexpr (DT) local_dt_cell<R, NS, NT>(z, p, id, m, u[id], v[id], w[id], rho, z.mul[id], mut);
   } else if constexpr (DT) {
     R m[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) m[q] = z.metric[q * P + id];
-    const CellDt<R> q = local_dt_cell<R, NS, NT>(z, p, id, m, z.bv[P + id], z.bv[2 * P + id], z.bv[3 * P + id],
-                                                 z.bv[id], z.mul[id], (R)0);
-    dt_c = q.dt;
-    sri_c = q.sri;
+    local_dt_cell<R, NS, NT>(z, p, id, m, z.bv[P + id], z.bv[2 * P + id], z.bv[3 * P + id], z.bv[id], z.mul[id],
+                             (R)0);
   }
   if constexpr (CHEM && NS > 0) {
     double y[NS], omega[NS];
@@ -427,15 +422,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && N
       const int nr = min(p->n_reac, CR);
       const auto om = [&](int l, double v) { z.dq[(5 + l) * P + id] = (R)((double)J * v); };
       if (meth == 1) {
-        if constexpr (FAC) {
-          chem_source_lds<NS, 3>(p, nr, CR, rho, t, y, qs, 256, om,
-                                 [&](int, double *a) { epi_factor_store<R, NS>(z, chem64(p), id, dt_c, sri_c, a); });
-        } else {
-          chem_source_lds<NS, 1>(p, nr, CR, rho, t, y, qs, 256, om, [&](int m, const double *row) {
+        chem_source_lds<NS, 1>(p, nr, CR, rho, t, y, qs, 256, om, [&](int m, const double *row) {
 #pragma unroll
-            for (int n = 0; n < NS; ++n) jac[(m * NS + n) * P + id] = row[n];
-          });
-        }
+          for (int n = 0; n < NS; ++n) jac[(m * NS + n) * P + id] = row[n];
+        });
       } else if (meth == 2) {
         chem_source_lds<NS, 2>(p, nr, CR, rho, t, y, qs, 256, om,
                                [&](int m, const double *d) { jac[m * P + id] = *d; });
@@ -506,10 +496,8 @@ __global__ __launch_bounds__(256) void k_explicit_implicit(ZoneView<R> z, const 
 
 // compute_DQ_0 (src/DPLURHPP.cuh:13-76): dq0 = dq / diag with the chemistry
 // (EPI_for_dq0, src/FiniteRateChem.cu:236-253, LU factors kept for the sweeps)
-// and SST (src/SST.cu:196-204) blocks treated point-implicitly.  LU_READY:
-// the source kernel has stored the chemistry block's factors (k_source with
-// FAC); they are read instead of formed.
-template <typename R, int NS, int NT, bool CHEM, int NA = NS, bool LU_READY = false>
+// and SST (src/SST.cu:196-204) blocks treated point-implicitly.
+template <typename R, int NS, int NT, bool CHEM, int NA = NS>
 __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__restrict__ p) {
   const auto c = cell_index(z, 0);
   if (!c.ok) return;
@@ -526,33 +514,22 @@ __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__
       const ChemF64 *ct = chem64(p);
       double a[NA * NA], b[NA];
       int piv[NA];
-      if constexpr (LU_READY) {
 #pragma unroll
-        for (int q = 0; q < NA * NA; ++q) a[q] = z.chem_lu[lu_tiled(id, q, NA * NA)];
-        unsigned w[PivPack<NA>::words];
+      for (int r = 0; r < NA; ++r) {
+        const int m = block_species<NS, NA>(ct, r);
 #pragma unroll
-        for (int q = 0; q < PivPack<NA>::words; ++q) w[q] = (unsigned)z.chem_piv[q * P + id];
-        piv_unpack<NA>(w, piv);
-#pragma unroll
-        for (int r = 0; r < NA; ++r) b[r] = (double)z.dq[(5 + block_species<NS, NA>(ct, r)) * P + id];
-      } else {
-#pragma unroll
-        for (int r = 0; r < NA; ++r) {
-          const int m = block_species<NS, NA>(ct, r);
-#pragma unroll
-          for (int c2 = 0; c2 < NA; ++c2)
-            a[r * NA + c2] = -dtd * z.chem_jac[(m * NS + block_species<NS, NA>(ct, c2)) * P + id];
-          a[r * NA + r] = dg - dtd * z.chem_jac[(m * NS + m) * P + id];
-          b[r] = (double)z.dq[(5 + m) * P + id];
-        }
-        lu_factor_any<NA>(a, piv);
-#pragma unroll
-        for (int q = 0; q < NA * NA; ++q) z.chem_lu[q * P + id] = a[q];
-        unsigned w[PivPack<NA>::words];
-        piv_pack<NA>(piv, w);
-#pragma unroll
-        for (int q = 0; q < PivPack<NA>::words; ++q) z.chem_piv[q * P + id] = (int)w[q];
+        for (int c2 = 0; c2 < NA; ++c2)
+          a[r * NA + c2] = -dtd * z.chem_jac[(m * NS + block_species<NS, NA>(ct, c2)) * P + id];
+        a[r * NA + r] = dg - dtd * z.chem_jac[(m * NS + m) * P + id];
+        b[r] = (double)z.dq[(5 + m) * P + id];
       }
+      lu_factor_any<NA>(a, piv);
+#pragma unroll
+      for (int q = 0; q < NA * NA; ++q) z.chem_lu[q * P + id] = a[q];
+      unsigned w[PivPack<NA>::words];
+      piv_pack<NA>(piv, w);
+#pragma unroll
+      for (int q = 0; q < PivPack<NA>::words; ++q) z.chem_piv[q * P + id] = (int)w[q];
       lu_solve_any<NA>(a, piv, b);
 #pragma unroll
       for (int r = 0; r < NA; ++r) z.dq0[(5 + block_species<NS, NA>(ct, r)) * P + id] = (R)b[r];
@@ -579,6 +556,64 @@ __global__ __launch_bounds__(256) void k_dq0(ZoneView<R> z, const ostc_param *__
       z.dq0[(NS + 6) * P + id] = z.dq[(NS + 6) * P + id] / diag;
     }
   }
+}
+
+// The finite-rate chemistry of an EPI zone under DPLUR placed after the flux
+// stages (the time step of ostc_steady_step on the 9-species kernels, at most
+// 20 reactions; the SST source and the local time step ran before the fluxes
+// in k_source, which wrote zeros to the species rows of dq).  Per cell: the
+// source J * omega (src/FiniteRateChem.cu:7-63) added to the species rows of
+// dq, which hold the flux residual -- the reference adds the two the other
+// way round, so the sum can differ in its last bit --; the EPI Jacobian
+// (reaction-major, in registers); the block (diag I - dt J) factored whole
+// and its reacting species' factors and pivots stored for the sweeps; dq0 of
+// every row (compute_DQ_0 and EPI_for_dq0, src/DPLURHPP.cuh:13-76,
+// src/FiniteRateChem.cu:236-253) from the same factors in registers.  The
+// Jacobian is never stored, and k_dq0's pass over it goes.
+template <typename R, int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_chem_dq0(
+    ZoneView<R> z, const ostc_param *__restrict__ p) {
+  constexpr int NS = 9, CR = 20;
+  __shared__ double qs_mem[2 * CR * 256];
+  const auto c = cell_index(z, 0);
+  if (!c.ok) return;
+  const int64_t id = z.idx(c.i, c.j, c.k), P = z.plane;
+  const R dt = z.dt[id];
+  const R sri = z.spectr[id] + z.spectr[P + id] + z.spectr[2 * P + id];
+  const R diag = (R)1 + dt * sri;
+  // flow and turbulence rows as k_dq0 forms them
+#pragma unroll
+  for (int l = 0; l < 5; ++l) z.dq0[l * P + id] = z.dq[l * P + id] / diag;
+  if constexpr (NT > 0) {
+    if (p->turb_implicit == 1) {
+      z.dq0[(NS + 5) * P + id] = z.dq[(NS + 5) * P + id] / (diag - dt * z.turb_jac[id]);
+      z.dq0[(NS + 6) * P + id] = z.dq[(NS + 6) * P + id] / (diag - dt * z.turb_jac[P + id]);
+    } else {
+      z.dq0[(NS + 5) * P + id] = z.dq[(NS + 5) * P + id] / diag;
+      z.dq0[(NS + 6) * P + id] = z.dq[(NS + 6) * P + id] / diag;
+    }
+  }
+  const R J = z.jac[id];
+  double y[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) y[l] = (double)z.sv[l * P + id];
+  const double rho = (double)z.bv[id], t = (double)z.bv[5 * P + id];
+  double *qs = qs_mem + threadIdx.x + 64 * threadIdx.y;
+  const int nr = min(p->n_reac, CR);
+  // the species rows of dq with the source added, at the bulk precision,
+  // parked in dq0's species rows while the Jacobian holds the registers
+  const auto om = [&](int l, double v) { z.dq0[(5 + l) * P + id] = z.dq[(5 + l) * P + id] + (R)((double)J * v); };
+  chem_source_lds<NS, 3>(p, nr, CR, rho, t, y, qs, 256, om, [&](int, double *a) {
+    int piv[NS];
+    epi_factor_whole<R, NS>(dt, sri, a, piv);
+    epi_store_factors<R, NS>(z, chem64(p), id, a, piv);
+    double b[NS];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) b[m] = (double)z.dq0[(5 + m) * P + id];
+    lu_solve_any<NS>(a, piv, b);
+#pragma unroll
+    for (int m = 0; m < NS; ++m) z.dq0[(5 + m) * P + id] = (R)b[m];
+  });
 }
 
 // Diagonal solve and write of one cell after a DPLUR sweep has gathered its
@@ -1537,10 +1572,12 @@ int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, 
   if (chem) {
     // mechanisms of up to 20 reactions take the two-pass chemistry
     if (with_dt) {
+      // source_factors_chem: the SST source and the time step only; the
+      // chemistry follows the fluxes (k_chem_dq0)
       if (o.nt == 2)
-        hipLaunchKernelGGL((k_source<R, 9, 2, 20, true, true>), g.grid, g.block, 0, st, z, dp);
+        hipLaunchKernelGGL((k_source<R, 9, 2, 0, true>), g.grid, g.block, 0, st, z, dp);
       else
-        hipLaunchKernelGGL((k_source<R, 9, 0, 20, true, true>), g.grid, g.block, 0, st, z, dp);
+        hipLaunchKernelGGL((k_source<R, 9, 0, 0, true>), g.grid, g.block, 0, st, z, dp);
     } else if (o.n_reac <= 20)
       OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, 20, false>), g.grid, g.block, 0, st, z, dp));
     else
@@ -1592,11 +1629,11 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
     prof_mark(P_DQ0, st, true);
     if (chem) {
       if (o.ns == 9 && o.chem == 1 && lu_ready) {
-        // EPI block factored by the source kernel: the solve only
+        // the chemistry after the fluxes, with dq0 (source_factors_chem)
         if (o.nt == 2)
-          OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 2, true, NA_, true>), g.grid, g.block, 0, st, z, dp));
+          hipLaunchKernelGGL((k_chem_dq0<R, 2>), g.grid, g.block, 0, st, z, dp);
         else
-          OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dq0<R, 9, 0, true, NA_, true>), g.grid, g.block, 0, st, z, dp));
+          hipLaunchKernelGGL((k_chem_dq0<R, 0>), g.grid, g.block, 0, st, z, dp);
       } else if (o.ns == 9 && o.chem == 1) {
         // EPI block: factored in the space of the species that react
         if (o.nt == 2)

@@ -48,9 +48,9 @@ int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o
 // writes dq (first writer; zeros when the model has no source and force is set);
 // with_dt: also the local time step (spectr, dt), for the zones where
 // source_has_dt or source_factors_chem holds -- the time step then skips
-// local_time_step; in the second case the source also stores the factors of
-// the DPLUR chemistry block instead of the EPI Jacobian, and
-// implicit_treatment is called with lu_ready
+// local_time_step; in the second case the source kernel leaves out the
+// chemistry, and implicit_treatment, called with lu_ready, adds it after the
+// fluxes together with dq0 and the DPLUR chemistry block's factors
 template <typename R>
 int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, bool force,
                  bool *wrote, bool with_dt = false);
@@ -63,8 +63,9 @@ inline bool source_factors_chem(const CellOpts &o) {
 }
 template <typename R>
 int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
-// leaves the final increment in *result (one of z.dq/z.dq0/z.dqk); lu_ready:
-// the source kernel stored the chemistry block's factors (source_factors_chem)
+// leaves the final increment in *result (one of z.dq/z.dq0/z.dqk); lu_ready
+// (source_factors_chem zones whose source_terms ran with_dt): the chemistry
+// source, its block's factors and dq0 are formed together after the fluxes
 template <typename R>
 int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result,
                        bool lu_ready = false);
