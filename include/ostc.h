@@ -354,11 +354,16 @@ int ostc_residual_sums_noalloc(ostc_ctx *ctx, ostc_zone **zones, int n, double *
 int ostc_zone_counters(ostc_zone *zone, long long out[4], int reset, void *stream);
 
 /* One steady iteration of Driver::steady_simulation over n zones
- * (src/Driver.cu:218-281) in the reference's stage order.  Two stages are
- * fused with the same per-cell arithmetic: in SST zones without chemistry the
+ * (src/Driver.cu:218-281) in the reference's stage order.  Stages are fused
+ * with the same per-cell arithmetic: in SST zones without chemistry the
  * source kernel also forms the local time step (nothing the flux stages
  * write enters it), and in non-reacting zones the first DPLUR sweep forms
- * dq0 from dq as it loads it.  inner faces of block b are
+ * dq0 from dq as it loads it.  In EPI zones under DPLUR on the 9-species
+ * kernels (at most 20 reactions) the finite-rate source follows the fluxes
+ * in one kernel with dq0 and the chemistry block's factors: the species
+ * source is added to the flux residual (the reference adds the fluxes to
+ * it; the sum can differ in its last bit), and the EPI Jacobian is not
+ * stored (OSTC_F_CHEM_JAC then holds the block's factors).  inner faces of block b are
  * inner[off_b .. off_b+n_inner[b]).  Returns the residual sums in
  * res_sums[4] when step%output_screen==0 or step==1 (else leaves them).
  * With n > 1 (and the stage profiler off) each zone's chain runs on a
