@@ -329,6 +329,21 @@ ostc_param pad_species(const ostc_param &u, int bucket) {
 
 void make_chem64(const ostc_param &p, ChemF64 &out) {
   std::memset(&out, 0, sizeof(out));
+  for (int r = 0; r < p.n_reac && r < kMaxReac; ++r) {
+    ReacF64 &q = out.reac[r];
+    q.A = p.A[r];
+    q.b = p.b[r];
+    q.Ea = p.Ea[r];
+    q.A2 = p.A2[r];
+    q.b2 = p.b2[r];
+    q.Ea2 = p.Ea2[r];
+    q.troe_alpha = p.troe_alpha[r];
+    q.troe_t1 = p.troe_t1[r];
+    q.troe_t2 = p.troe_t2[r];
+    q.troe_t3 = p.troe_t3[r];
+    q.type = p.reac_type[r];
+    q.order = p.reac_order[r];
+  }
   for (int i = 0; i < p.n_spec && i < kMaxSpec; ++i) out.inv_mw[i] = p.mw[i] > 0 ? 1.0 / p.mw[i] : 0.0;
   for (int r = 0; r < p.n_reac && r < kMaxReac; ++r)
     for (int l = 0; l < p.n_spec && l < kMaxSpec; ++l) {

@@ -36,7 +36,17 @@ void make_thermo32(const ostc_param &p, ThermoF32 &out);
 // the species that take part (forward or backward coefficient nonzero) and of
 // those whose net coefficient is nonzero -- wave-uniform tests that skip the
 // zero terms of the production sums and of the EPI Jacobian.
+// One reaction's rate constants in one record (the parameter block keeps
+// them in per-quantity arrays): a wave-uniform reaction index then reads them
+// from one base address with wide scalar loads.
+struct ReacF64 {
+  double A, b, Ea, A2, b2, Ea2;                 // Arrhenius: main, second (low-pressure / reverse)
+  double troe_alpha, troe_t1, troe_t2, troe_t3;
+  int type, order;                               // reaction type, reaction order (for Kc)
+  int pad[2];
+};
 struct ChemF64 {
+  ReacF64 reac[kMaxReac];
   double inv_mw[kMaxSpec];
   double nu_f[kMaxReac * kMaxSpec], nu_b[kMaxReac * kMaxSpec];
   unsigned part[kMaxReac], net[kMaxReac];
@@ -405,9 +415,10 @@ __device__ __forceinline__ void reaction_rates(const ostc_param *__restrict__ p,
                                                double log_t, double temp_t, double temp_t_inv, const double *c,
                                                const double *g, const double *nf, const double *nb, unsigned part,
                                                double &q1, double &q2) {
-  const int type = p->reac_type[i];
-  double kf = arrhenius(t_inv, log_t, p->A[i], p->b[i], p->Ea[i]);
-  if (type == 3) kf += arrhenius(t_inv, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
+  const ReacF64 &rc = chem64(p)->reac[i];
+  const int type = rc.type;
+  double kf = arrhenius(t_inv, log_t, rc.A, rc.b, rc.Ea);
+  if (type == 3) kf += arrhenius(t_inv, log_t, rc.A2, rc.b2, rc.Ea2);
   if (type > 3) {
     double cc = 0;
 #pragma unroll
@@ -415,13 +426,13 @@ __device__ __forceinline__ void reaction_rates(const ostc_param *__restrict__ p,
     if (type == 4) {
       kf *= cc;
     } else {
-      const double kf_low = arrhenius(t_inv, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
+      const double kf_low = arrhenius(t_inv, log_t, rc.A2, rc.b2, rc.Ea2);
       const double kf_high = kf;
       const double pr = kf_low * cc / kf_high;
       double fv = 1.0;
       if (type > 5) {
-        double f_cent = (1 - p->troe_alpha[i]) * exp(-t / p->troe_t3[i]) + p->troe_alpha[i] * exp(-t / p->troe_t1[i]);
-        if (type == 7) f_cent += exp(-p->troe_t2[i] * t_inv);
+        double f_cent = (1 - rc.troe_alpha) * exp(-t / rc.troe_t3) + rc.troe_alpha * exp(-t / rc.troe_t1);
+        if (type == 7) f_cent += exp(-rc.troe_t2 * t_inv);
         const double logFc = log10(f_cent);
         const double cc1 = -0.4 - 0.67 * logFc;
         const double n = 0.75 - 1.27 * logFc;
@@ -438,14 +449,14 @@ __device__ __forceinline__ void reaction_rates(const ostc_param *__restrict__ p,
 #pragma unroll
     for (int l = 0; l < NS; ++l)
       if (part & (1u << l)) d_gibbs += g[l] * (nb[l] - nf[l]);
-    const int order = p->reac_order[i];
+    const int order = rc.order;
     // kb = kf / Kc with Kc = temp_t^order * exp(-d_gibbs)
     const double base = order >= 0 ? temp_t_inv : temp_t;
     double pw = 1.0;
     for (int q = 0; q < (order >= 0 ? order : -order); ++q) pw *= base;
     kb = kf * exp(d_gibbs) * pw;
   } else {
-    kb = arrhenius(t_inv, log_t, p->A2[i], p->b2[i], p->Ea2[i]);
+    kb = arrhenius(t_inv, log_t, rc.A2, rc.b2, rc.Ea2);
   }
   double a = kf, b = kb;
 #pragma unroll
