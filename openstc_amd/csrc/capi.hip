@@ -353,6 +353,10 @@ void make_chem64(const ostc_param &p, ChemF64 &out) {
       if (f != 0 || b != 0) out.part[r] |= 1u << l;
       if (f != b) out.net[r] |= 1u << l;
     }
+  for (int r = 0; r < p.n_reac && r < kMaxReac; ++r) {
+    out.reac[r].part = out.part[r];
+    out.reac[r].net = out.net[r];
+  }
   for (int l = 0; l < kMaxSpec; ++l)
     for (int r = 0; r < p.n_reac && r < kMaxReac; ++r)
       if (out.net[r] & (1u << l)) out.jr[l][out.jn[l]++] = r;

@@ -43,7 +43,7 @@ struct ReacF64 {
   double A, b, Ea, A2, b2, Ea2;                 // Arrhenius: main, second (low-pressure / reverse)
   double troe_alpha, troe_t1, troe_t2, troe_t3;
   int type, order;                               // reaction type, reaction order (for Kc)
-  int pad[2];
+  unsigned part, net;                            // ChemF64::part / net of the reaction
 };
 struct ChemF64 {
   ReacF64 reac[kMaxReac];
@@ -510,7 +510,7 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
       nf[l] = ct->nu_f[r * kMaxSpec + l];
       nb[l] = ct->nu_b[r * kMaxSpec + l];
     }
-    const unsigned part = ct->part[r], net = ct->net[r];
+    const unsigned part = ct->reac[r].part, net = ct->reac[r].net;
     double q1, q2;
     reaction_rates<NS>(p, r, t, t_inv, log_t, temp_t, temp_t_inv, c, g, nf, nb, part, q1, q2);
     (void)part;
@@ -589,7 +589,7 @@ __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p
         nb[l] = ct->nu_b[r * kMaxSpec + l];
       }
       double q1, q2;
-      reaction_rates<NS>(p, r, t, t_inv, log_t, temp_t, temp_t_inv, c, g, nf, nb, ct->part[r], q1, q2);
+      reaction_rates<NS>(p, r, t, t_inv, log_t, temp_t, temp_t_inv, c, g, nf, nb, ct->reac[r].part, q1, q2);
       qs[r * stride] = q1;
       qs[(nr_max + r) * stride] = q2;
     }
@@ -635,7 +635,7 @@ __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p
 #pragma unroll 1
     for (int r = 0; r < nr; ++r) {
       const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
-      const unsigned net = ct->net[r];
+      const unsigned net = ct->reac[r].net;
       double nf[NS], nb[NS], v[NS];
 #pragma unroll
       for (int n = 0; n < NS; ++n) {
