@@ -50,18 +50,18 @@ METRIC = "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem
 # null.
 MEASURED_TRAFFIC = {
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2304.3e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4277.4e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 5044.3e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 1473.9e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4276.7e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 5043.3e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 1472.6e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3637.1e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4035.6e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3072.0e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.5e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "properties"): 730.5e6,
     ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 513.4e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1195.1e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 1389.4e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "source"): 250.0e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1195.2e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 1388.9e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "source"): 250.6e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 741.0e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 840.6e6,
     ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 743.3e6,
