@@ -319,8 +319,10 @@ __device__ __forceinline__ void epi_store_factors(const ZoneView<R> &z, const Ch
 // per workgroup at CR = 20, two workgroups per CU); < 0 the one-pass form (any
 // reaction count).  The LDS variants ask for two waves per SIMD, the most
 // their LDS allows, which leaves 256 VGPRs per lane (the EPI Jacobian's sums
-// stay in registers); the others ask for three.
-template <typename R, int NS, int NT, int CR, bool DT>
+// stay in registers); the others ask for three.  TURB_ROWS: only the
+// turbulence rows of dq are written (zones whose fluxes write the others
+// first: no chemistry here, its source follows the fluxes).
+template <typename R, int NS, int NT, int CR, bool DT, bool TURB_ROWS = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && NS > 0 ? 2 : 3))) void k_source(ZoneView<R> z,
                                                                                         const ostc_param *__restrict__ p) {
   constexpr bool CHEM = CR != 0;
@@ -450,8 +452,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CR > 0 && N
 #pragma unroll
     for (int l = 0; l < NS; ++l) dq[5 + l] = (R)((double)J * omega[l]);
   }
+  static_assert(!TURB_ROWS || !CHEM, "the species rows carry the chemistry source");
 #pragma unroll
-  for (int l = 0; l < NV; ++l) z.dq[l * P + id] = dq[l];
+  for (int l = TURB_ROWS ? 5 + NS : 0; l < NV; ++l) z.dq[l * P + id] = dq[l];
 }
 
 // ------------------------------------------------------------------ implicit
@@ -1575,9 +1578,9 @@ int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, 
       // source_factors_chem: the SST source and the time step only; the
       // chemistry follows the fluxes (k_chem_dq0)
       if (o.nt == 2)
-        hipLaunchKernelGGL((k_source<R, 9, 2, 0, true>), g.grid, g.block, 0, st, z, dp);
+        hipLaunchKernelGGL((k_source<R, 9, 2, 0, true, true>), g.grid, g.block, 0, st, z, dp);
       else
-        hipLaunchKernelGGL((k_source<R, 9, 0, 0, true>), g.grid, g.block, 0, st, z, dp);
+        hipLaunchKernelGGL((k_source<R, 9, 0, 0, true, true>), g.grid, g.block, 0, st, z, dp);
     } else if (o.n_reac <= 20)
       OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, 20, false>), g.grid, g.block, 0, st, z, dp));
     else

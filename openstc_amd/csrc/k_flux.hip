@@ -339,7 +339,10 @@ __device__ __forceinline__ void viscous_face(const ZoneView<R> &z, const ostc_pa
   }
 }
 
-template <typename R, int NS, int NT, int DIR, bool INV, bool VISC, bool ACCUM>
+// ACC: 0 the first writer of dq, 1 accumulating into it, 2 the first writer
+// of the flow and species rows, accumulating into the turbulence rows (which
+// the SST source wrote before)
+template <typename R, int NS, int NT, int DIR, bool INV, bool VISC, int ACC>
 __global__ __launch_bounds__(256) void sweep_kernel(ZoneView<R> z, const ostc_param *__restrict__ p, int L,
                                                     int TW) {
   constexpr int NV = 5 + NS + NT;
@@ -442,7 +445,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(ZoneView<R> z, const ostc_pa
 #pragma unroll
     for (int l = 0; l < NV; ++l) {
       const R d = fc[l * nfaces + qr] - fc[l * nfaces + ql];
-      if (ACCUM)
+      if (ACC == 1 || (ACC == 2 && l >= 5 + NS))
         z.dq[id + l * z.plane] -= d;
       else
         z.dq[id + l * z.plane] = -d;
@@ -941,7 +944,7 @@ constexpr int kSweepLds = 53248;
 
 template <typename R, int NS, int NT>
 static int launch_sweeps(const ZoneView<R> &z, const ostc_param *dp, bool inv, bool visc, bool first_writer,
-                         hipStream_t st) {
+                         hipStream_t st, bool keep_turb) {
   constexpr int NV = 5 + NS + NT;
   const int ext[3] = {z.mx, z.my, z.mz};
   for (int dir = 0; dir < 3; ++dir) {
@@ -984,25 +987,39 @@ static int launch_sweeps(const ZoneView<R> &z, const ostc_param *dp, bool inv, b
     }
     const size_t lds = (size_t)TW * (L + 1) * NV * sizeof(R);
     const bool acc = !(first_writer && dir == 0);
+    // the first writer of the flow and species rows, the turbulence rows
+    // accumulated (the SST source wrote them)
+    const bool turb_acc = !acc && keep_turb && NT > 0;
     prof_mark(P_FLUX_I + dir, st, true);
 #define OSTC_SWEEP(D, I, V, A) \
   hipLaunchKernelGGL((sweep_kernel<R, NS, NT, D, I, V, A>), grid, dim3(256), lds, st, z, dp, L, TW)
 #define OSTC_SWEEP_D(D)                               \
   do {                                                \
     if (inv && visc) {                                \
-      if (acc) OSTC_SWEEP(D, true, true, true);       \
-      else OSTC_SWEEP(D, true, true, false);          \
+      if (acc) OSTC_SWEEP(D, true, true, 1);          \
+      else OSTC_SWEEP(D, true, true, 0);              \
     } else if (inv) {                                 \
-      if (acc) OSTC_SWEEP(D, true, false, true);      \
-      else OSTC_SWEEP(D, true, false, false);         \
+      if (acc) OSTC_SWEEP(D, true, false, 1);         \
+      else OSTC_SWEEP(D, true, false, 0);             \
     } else {                                          \
-      if (acc) OSTC_SWEEP(D, false, true, true);      \
-      else OSTC_SWEEP(D, false, true, false);         \
+      if (acc) OSTC_SWEEP(D, false, true, 1);         \
+      else OSTC_SWEEP(D, false, true, 0);             \
     }                                                 \
   } while (0)
-    if (dir == 0) OSTC_SWEEP_D(0);
-    else if (dir == 1) OSTC_SWEEP_D(1);
-    else OSTC_SWEEP_D(2);
+    if (turb_acc) {
+      if (inv && visc)
+        OSTC_SWEEP(0, true, true, 2);
+      else if (inv)
+        OSTC_SWEEP(0, true, false, 2);
+      else
+        OSTC_SWEEP(0, false, true, 2);
+    } else if (dir == 0) {
+      OSTC_SWEEP_D(0);
+    } else if (dir == 1) {
+      OSTC_SWEEP_D(1);
+    } else {
+      OSTC_SWEEP_D(2);
+    }
 #undef OSTC_SWEEP_D
 #undef OSTC_SWEEP
     prof_mark(P_FLUX_I + dir, st, false);
@@ -1013,18 +1030,20 @@ static int launch_sweeps(const ZoneView<R> &z, const ostc_param *dp, bool inv, b
 
 template <typename R>
 int flux_sweeps(const ZoneView<R> &z, const ostc_param *dp, int ns, int nt, bool inv, bool visc, bool first_writer,
-                hipStream_t st) {
+                hipStream_t st, bool keep_turb) {
+  if (keep_turb && ns == 0) return OSTC_ERR_ARG;
   if (ns == 0 && nt == 0) return launch_march<R, 0>(z, dp, inv, visc, first_writer, st);
   if (ns == 0 && nt == 2) return launch_march<R, 2>(z, dp, inv, visc, first_writer, st);
-  if (ns == 9 && nt == 0) return launch_sweeps<R, 9, 0>(z, dp, inv, visc, first_writer, st);
-  if (ns == 9 && nt == 2) return launch_sweeps<R, 9, 2>(z, dp, inv, visc, first_writer, st);
-  if (ns == 20 && nt == 0) return launch_sweeps<R, 20, 0>(z, dp, inv, visc, first_writer, st);
-  if (ns == 20 && nt == 2) return launch_sweeps<R, 20, 2>(z, dp, inv, visc, first_writer, st);
+  if (ns == 9 && nt == 0) return launch_sweeps<R, 9, 0>(z, dp, inv, visc, first_writer, st, keep_turb);
+  if (ns == 9 && nt == 2) return launch_sweeps<R, 9, 2>(z, dp, inv, visc, first_writer, st, keep_turb);
+  if (ns == 20 && nt == 0) return launch_sweeps<R, 20, 0>(z, dp, inv, visc, first_writer, st, keep_turb);
+  if (ns == 20 && nt == 2) return launch_sweeps<R, 20, 2>(z, dp, inv, visc, first_writer, st, keep_turb);
   return OSTC_ERR_ARG;
 }
 
-template int flux_sweeps<float>(const ZoneView<float> &, const ostc_param *, int, int, bool, bool, bool, hipStream_t);
+template int flux_sweeps<float>(const ZoneView<float> &, const ostc_param *, int, int, bool, bool, bool, hipStream_t,
+                                bool);
 template int flux_sweeps<double>(const ZoneView<double> &, const ostc_param *, int, int, bool, bool, bool,
-                                 hipStream_t);
+                                 hipStream_t, bool);
 
 }  // namespace ostc

@@ -26,8 +26,11 @@ inline CellGrid cell_grid(int ni, int nj, int nk) {
 
 // k_flux.hip
 template <typename R>
+// first_writer: the i direction writes dq instead of accumulating into it;
+// keep_turb (mixtures, with first_writer): it accumulates into the
+// turbulence rows, which the SST source wrote, and writes the others
 int flux_sweeps(const ZoneView<R> &z, const ostc_param *dp, int ns, int nt, bool inv, bool visc, bool first_writer,
-                hipStream_t st);
+                hipStream_t st, bool keep_turb = false);
 
 // k_cell.hip
 struct CellOpts {
@@ -49,7 +52,9 @@ int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o
 // with_dt: also the local time step (spectr, dt), for the zones where
 // source_has_dt or source_factors_chem holds -- the time step then skips
 // local_time_step; in the second case the source kernel leaves out the
-// chemistry, and implicit_treatment, called with lu_ready, adds it after the
+// chemistry and writes only the turbulence rows of dq (the fluxes then write
+// the others: flux_sweeps with first_writer and keep_turb), and
+// implicit_treatment, called with lu_ready, adds the chemistry after the
 // fluxes together with dq0 and the DPLUR chemistry block's factors
 template <typename R>
 int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, bool force,
