@@ -923,9 +923,10 @@ static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen
                 source_terms(z->vd, c->dev, o, st, false, &wrote, dt_in_source));
   prof_mark(P_SOURCE, st, false);
   if (r) return r;
-  // EPI zones with the chemistry after the fluxes: the source wrote only the
-  // turbulence rows of dq (none without SST); the i fluxes write the others
-  const bool flux_first = !wrote || lu_in_source, keep_turb = lu_in_source && o.nt > 0;
+  // a source kernel that forms the time step writes only the turbulence
+  // rows of dq (none without SST; the chemistry, if any, follows the
+  // fluxes): the first flux direction writes the others
+  const bool flux_first = !wrote || dt_in_source, keep_turb = dt_in_source && o.nt > 0;
   r = ZONE_CALL(z, flux_sweeps(z->vf, c->dev, o.ns, o.nt, true, visc, flux_first, st, keep_turb),
                 flux_sweeps(z->vd, c->dev, o.ns, o.nt, true, visc, flux_first, st, keep_turb));
   if (r) return r;

@@ -1586,12 +1586,13 @@ int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, 
     else
       OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, -1, false>), g.grid, g.block, 0, st, z, dp));
   } else if (with_dt) {
+    // the SST rows of dq only: the fluxes write the others (keep_turb)
     if (o.ns == 0)
-      hipLaunchKernelGGL((k_source<R, 0, 2, 0, true>), g.grid, g.block, 0, st, z, dp);
+      hipLaunchKernelGGL((k_source<R, 0, 2, 0, true, true>), g.grid, g.block, 0, st, z, dp);
     else if (o.ns == 9)
-      hipLaunchKernelGGL((k_source<R, 9, 2, 0, true>), g.grid, g.block, 0, st, z, dp);
+      hipLaunchKernelGGL((k_source<R, 9, 2, 0, true, true>), g.grid, g.block, 0, st, z, dp);
     else
-      hipLaunchKernelGGL((k_source<R, 20, 2, 0, true>), g.grid, g.block, 0, st, z, dp);
+      hipLaunchKernelGGL((k_source<R, 20, 2, 0, true, true>), g.grid, g.block, 0, st, z, dp);
   } else {
     OSTC_DISPATCH_NSNT(o, hipLaunchKernelGGL((k_source<R, NS_, NT_, 0, false>), g.grid, g.block, 0, st, z, dp));
   }

@@ -753,7 +753,10 @@ __device__ __forceinline__ void face_flux_v(const ostc_param *__restrict__ p, co
 
 constexpr int kSegUseful = 60;  // written lanes 2..61 of a 64-lane segment
 
-template <typename R, int NT, int MARCH, bool INV, bool VISC, bool ACCUM>
+// ACC (MARCH = 1): 0 the first writer of dq, 1 accumulating into it, 2 the
+// first writer of the flow rows, accumulating into the turbulence rows
+// (which the SST source wrote before); MARCH = 2 always accumulates
+template <typename R, int NT, int MARCH, bool INV, bool VISC, int ACC>
 __global__ __launch_bounds__(256) void k_flux_march(ZoneView<R> z, const ostc_param *__restrict__ p, int mlen) {
   constexpr int NV = 5 + NT, NG = 4 + NT;
   constexpr int X = 3 - MARCH;  // the direction whose central differences are loaded
@@ -874,7 +877,7 @@ __global__ __launch_bounds__(256) void k_flux_march(ZoneView<R> z, const ostc_pa
           const R dmv = gm[l] - gprev[l];
           R v;
           if constexpr (MARCH == 1) {
-            const R base = ACCUM ? z.dq[q + l * P] - diprev[l] : -diprev[l];
+            const R base = (ACC == 1 || (ACC == 2 && l >= 5)) ? z.dq[q + l * P] - diprev[l] : -diprev[l];
             v = base - dmv;
           } else {
             v = z.dq[q + l * P] - dmv;
@@ -903,7 +906,7 @@ __global__ __launch_bounds__(256) void k_flux_march(ZoneView<R> z, const ostc_pa
 
 template <typename R, int NT>
 static int launch_march(const ZoneView<R> &z, const ostc_param *dp, bool inv, bool visc, bool first_writer,
-                        hipStream_t st) {
+                        hipStream_t st, bool keep_turb) {
   const int segs = (z.mx + kSegUseful - 1) / kSegUseful;
   // rows marched per wave (launch tuning; 3 extra rows are loaded per chunk)
   static const int mlen = [] {
@@ -919,16 +922,20 @@ static int launch_march(const ZoneView<R> &z, const ostc_param *dp, bool inv, bo
     const int X = 3 - m;
     const int extX = X == 1 ? z.my : z.mz, extM = m == 1 ? z.my : z.mz;
     const dim3 grid((unsigned)segs, (unsigned)((extX + 3) / 4), (unsigned)((extM + mlen - 1) / mlen));
-    const bool acc = !(first_writer && m == 1);
+    const int acc = !(first_writer && m == 1) ? 1 : (keep_turb && NT > 0 ? 2 : 0);
     prof_mark(m == 1 ? P_FLUX_IJ : P_FLUX_K, st, true);
     if (m == 1) {
-      if (inv && visc) { if (acc) OSTC_MARCH(1, true, true, true); else OSTC_MARCH(1, true, true, false); }
-      else if (inv) { if (acc) OSTC_MARCH(1, true, false, true); else OSTC_MARCH(1, true, false, false); }
-      else { if (acc) OSTC_MARCH(1, false, true, true); else OSTC_MARCH(1, false, true, false); }
+      if (acc == 2) {
+        if (inv && visc) OSTC_MARCH(1, true, true, 2);
+        else if (inv) OSTC_MARCH(1, true, false, 2);
+        else OSTC_MARCH(1, false, true, 2);
+      } else if (inv && visc) { if (acc) OSTC_MARCH(1, true, true, 1); else OSTC_MARCH(1, true, true, 0); }
+      else if (inv) { if (acc) OSTC_MARCH(1, true, false, 1); else OSTC_MARCH(1, true, false, 0); }
+      else { if (acc) OSTC_MARCH(1, false, true, 1); else OSTC_MARCH(1, false, true, 0); }
     } else {
-      if (inv && visc) OSTC_MARCH(2, true, true, true);
-      else if (inv) OSTC_MARCH(2, true, false, true);
-      else OSTC_MARCH(2, false, true, true);
+      if (inv && visc) OSTC_MARCH(2, true, true, 1);
+      else if (inv) OSTC_MARCH(2, true, false, 1);
+      else OSTC_MARCH(2, false, true, 1);
     }
     prof_mark(m == 1 ? P_FLUX_IJ : P_FLUX_K, st, false);
     if (hipGetLastError() != hipSuccess) return OSTC_ERR_HIP;
@@ -1031,9 +1038,8 @@ static int launch_sweeps(const ZoneView<R> &z, const ostc_param *dp, bool inv, b
 template <typename R>
 int flux_sweeps(const ZoneView<R> &z, const ostc_param *dp, int ns, int nt, bool inv, bool visc, bool first_writer,
                 hipStream_t st, bool keep_turb) {
-  if (keep_turb && ns == 0) return OSTC_ERR_ARG;
-  if (ns == 0 && nt == 0) return launch_march<R, 0>(z, dp, inv, visc, first_writer, st);
-  if (ns == 0 && nt == 2) return launch_march<R, 2>(z, dp, inv, visc, first_writer, st);
+  if (ns == 0 && nt == 0) return launch_march<R, 0>(z, dp, inv, visc, first_writer, st, keep_turb);
+  if (ns == 0 && nt == 2) return launch_march<R, 2>(z, dp, inv, visc, first_writer, st, keep_turb);
   if (ns == 9 && nt == 0) return launch_sweeps<R, 9, 0>(z, dp, inv, visc, first_writer, st, keep_turb);
   if (ns == 9 && nt == 2) return launch_sweeps<R, 9, 2>(z, dp, inv, visc, first_writer, st, keep_turb);
   if (ns == 20 && nt == 0) return launch_sweeps<R, 20, 0>(z, dp, inv, visc, first_writer, st, keep_turb);

@@ -27,8 +27,8 @@ inline CellGrid cell_grid(int ni, int nj, int nk) {
 // k_flux.hip
 template <typename R>
 // first_writer: the i direction writes dq instead of accumulating into it;
-// keep_turb (mixtures, with first_writer): it accumulates into the
-// turbulence rows, which the SST source wrote, and writes the others
+// keep_turb (with first_writer): it accumulates into the turbulence rows,
+// which the SST source wrote, and writes the others
 int flux_sweeps(const ZoneView<R> &z, const ostc_param *dp, int ns, int nt, bool inv, bool visc, bool first_writer,
                 hipStream_t st, bool keep_turb = false);
 
@@ -49,7 +49,9 @@ int physical_properties(const ZoneView<R> &z, const ostc_param *dp, const CellOp
 template <typename R>
 int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
 // writes dq (first writer; zeros when the model has no source and force is set);
-// with_dt: also the local time step (spectr, dt), for the zones where
+// with_dt: also the local time step (spectr, dt) and only the turbulence
+// rows of dq (flux_sweeps then runs with first_writer and keep_turb), for
+// the zones where
 // source_has_dt or source_factors_chem holds -- the time step then skips
 // local_time_step; in the second case the source kernel leaves out the
 // chemistry and writes only the turbulence rows of dq (the fluxes then write
