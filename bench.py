@@ -49,28 +49,28 @@ METRIC = "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem
 # Re-measured whenever the kernel changes; a stage without an entry reports
 # null.
 MEASURED_TRAFFIC = {
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2304.3e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4276.7e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 5043.3e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 1472.6e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3637.1e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4035.6e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3072.0e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.5e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2306.1e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4277.4e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 5043.5e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 1057.6e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3580.3e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4036.5e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3072.2e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.0e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "properties"): 730.5e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 513.4e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1195.2e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 1388.9e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "source"): 250.6e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 741.0e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 840.6e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 743.3e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.8e6,
-    ("sst_flat_plate_dplur", "dplur_sweep"): 479.4e6,
-    ("sst_flat_plate_dplur", "dplur_sweep_first"): 501.8e6,
-    ("sst_flat_plate_dplur", "flux_ij"): 402.3e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 513.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1195.4e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 1389.0e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "source"): 144.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 672.0e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 839.8e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 742.7e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.7e6,
+    ("sst_flat_plate_dplur", "dplur_sweep"): 482.6e6,
+    ("sst_flat_plate_dplur", "dplur_sweep_first"): 502.9e6,
+    ("sst_flat_plate_dplur", "flux_ij"): 358.6e6,
     ("sst_flat_plate_dplur", "flux_k"): 402.2e6,
-    ("sst_flat_plate_dplur", "source"): 357.7e6,
+    ("sst_flat_plate_dplur", "source"): 299.7e6,
     ("sst_flat_plate_dplur", "update"): 273.5e6,
 }
 
@@ -156,19 +156,30 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
         out[stage] = n_bulk * esize + n_f64 * 8 + n_i32 * 4
 
     # flux, one pass over the cells per direction (inviscid + viscous fused):
-    # bv 6, sv, metric 9, J, [mu, lambda, (mu_t, lambda_t, d), rho*D_k], dq RMW
+    # bv 6, sv, metric 9, J, [mu, lambda, (mu_t, lambda_t, d), rho*D_k], dq RMW.
+    # The first direction writes the rows of dq no source kernel wrote
+    # before it: a source kernel that forms the time step (SST zones without
+    # chemistry, EPI zones with the chemistry after the fluxes) writes only
+    # the turbulence rows, and zones without a source term none.
+    dt_src = (rans and p.mix_model != L.FR) or fac
+    no_src = not rans and not chem
+    first_reads = nt if dt_src else (0 if no_src else nv)
     flux = 6 + nsc + 9 + 1 + ((2 + (3 if rans else 0) + ns) if visc else 0) + 2 * nv
-    for s in ("flux_i", "flux_j", "flux_k", "flux_ij"):
+    for s in ("flux_j", "flux_k"):
         put(s, flux)
+    for s in ("flux_i", "flux_ij"):
+        put(s, flux - nv + first_reads)
     # source: SST reads rho,u,v,w, k,w, metric 9, J, mu, d (+T, cp for a
     # mixture) and writes mu_t, lambda_t, dq (first writer), J_t 2; the SST
     # source of an air zone also forms the local time step (reads c, writes
     # spectral radii 3 and dt); chemistry reads rho, T, Y and writes dq and
     # the FP64 Jacobian
     if rans:
+        # an air zone's SST source also forms the time step and writes only
+        # the turbulence rows of dq
         src = (4 + 2 + 9 + 1 + 1 + 1 + (ns + 2 if ns else 0)) + (2 + nv + 2)
         if ns == 0:
-            src += 1 + 3 + 1
+            src += 1 + 3 + 1 - (nv - nt)
     elif chem:
         src = 2 + ns + nv
     else:
@@ -178,9 +189,9 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
         # written; the time step: c, gamma (+ metric 9, u, v, w, rho, mu
         # without SST) -> spectral radii 3, dt
         if rans:
-            src = (4 + 2 + 9 + 1 + 1 + 1 + 2) + (2 + nv + 2)
+            src = (4 + 2 + 9 + 1 + 1 + 1 + 2) + (2 + nt + 2)
         else:
-            src = 9 + 3 + 1 + 1 + nv
+            src = 9 + 3 + 1 + 1
         src += 2 + 4
         put("source", src)
     else:
