@@ -916,8 +916,8 @@ static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen
   // under DPLUR (9-species kernels): so does the SST-only source kernel, and
   // the chemistry source follows the fluxes, fused with dq0 and the
   // factorisation of the chemistry block (no Jacobian stored)
-  const bool lu_in_source = source_factors_chem(o);
-  const bool dt_in_source = source_has_dt(o) || lu_in_source;
+  const bool chem_late = chem_after_fluxes(o);
+  const bool dt_in_source = source_has_dt(o) || chem_late;
   prof_mark(P_SOURCE, st, true);
   r = ZONE_CALL(z, source_terms(z->vf, c->dev, o, st, false, &wrote, dt_in_source),
                 source_terms(z->vd, c->dev, o, st, false, &wrote, dt_in_source));
@@ -943,14 +943,14 @@ static int zone_iteration(ostc_ctx *c, ostc_zone *z, int step, int output_screen
     const bool dplur = o.implicit_method == 1;
     if (fp64(z)) {
       double *res = nullptr;
-      r = implicit_treatment(z->vd, c->dev, o, st, &res, lu_in_source);
+      r = implicit_treatment(z->vd, c->dev, o, st, &res, chem_late);
       if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
       if (r) return r;
       prof_mark(P_UPDATE, st, true);
       r = update_cv_bv(z->vd, c->dev, o, res, st, z->lf_slot);
     } else {
       float *res = nullptr;
-      r = implicit_treatment(z->vf, c->dev, o, st, &res, lu_in_source);
+      r = implicit_treatment(z->vf, c->dev, o, st, &res, chem_late);
       if (!dplur) prof_mark(P_EXPL_IMPL, st, false);
       if (r) return r;
       prof_mark(P_UPDATE, st, true);

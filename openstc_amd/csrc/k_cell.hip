@@ -1569,13 +1569,13 @@ int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, 
                  bool *wrote, bool with_dt) {
   const bool chem = o.mix_model == OSTC_FR;
   *wrote = false;
-  if (with_dt && !source_has_dt(o) && !source_factors_chem(o)) return OSTC_ERR_ARG;
+  if (with_dt && !source_has_dt(o) && !chem_after_fluxes(o)) return OSTC_ERR_ARG;
   if (o.nt == 0 && !chem && !force) return OSTC_OK;
   const auto g = cell_grid(z.mx, z.my, z.mz);
   if (chem) {
     // mechanisms of up to 20 reactions take the two-pass chemistry
     if (with_dt) {
-      // source_factors_chem: the SST source and the time step only; the
+      // chem_after_fluxes: the SST source and the time step only; the
       // chemistry follows the fluxes (k_chem_dq0)
       if (o.nt == 2)
         hipLaunchKernelGGL((k_source<R, 9, 2, 0, true, true>), g.grid, g.block, 0, st, z, dp);
@@ -1609,7 +1609,7 @@ int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &
 
 template <typename R>
 int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result,
-                       bool lu_ready) {
+                       bool chem_late) {
   const auto g = cell_grid(z.mx, z.my, z.mz);
   const bool chem = o.mix_model == OSTC_FR;
   *result = z.dq;
@@ -1632,8 +1632,8 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
   if (!fuse_dq0) {
     prof_mark(P_DQ0, st, true);
     if (chem) {
-      if (o.ns == 9 && o.chem == 1 && lu_ready) {
-        // the chemistry after the fluxes, with dq0 (source_factors_chem)
+      if (o.ns == 9 && o.chem == 1 && chem_late) {
+        // the chemistry after the fluxes, with dq0 (chem_after_fluxes)
         if (o.nt == 2)
           hipLaunchKernelGGL((k_chem_dq0<R, 2>), g.grid, g.block, 0, st, z, dp);
         else
@@ -1699,7 +1699,7 @@ int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpt
       prof_mark(P_DPLUR_CHEM, st, true);
       if (o.ns == 9) {
         // factors stored by the source kernel are tiled (lu_tiled)
-        if (lu_ready)
+        if (chem_late)
           OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dplur_chem<R, 9, NA_, true>), g.grid, g.block, 0, st, z, dp, out));
         else
           OSTC_DISPATCH_NA(o, hipLaunchKernelGGL((k_dplur_chem<R, 9, NA_>), g.grid, g.block, 0, st, z, dp, out));

@@ -52,11 +52,11 @@ int initialize_mut(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o
 // with_dt: also the local time step (spectr, dt) and only the turbulence
 // rows of dq (flux_sweeps then runs with first_writer and keep_turb), for
 // the zones where
-// source_has_dt or source_factors_chem holds -- the time step then skips
+// source_has_dt or chem_after_fluxes holds -- the time step then skips
 // local_time_step; in the second case the source kernel leaves out the
 // chemistry and writes only the turbulence rows of dq (the fluxes then write
 // the others: flux_sweeps with first_writer and keep_turb), and
-// implicit_treatment, called with lu_ready, adds the chemistry after the
+// implicit_treatment, called with chem_late, adds the chemistry after the
 // fluxes together with dq0 and the DPLUR chemistry block's factors
 template <typename R>
 int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, bool force,
@@ -64,18 +64,18 @@ int source_terms(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, 
 inline bool source_has_dt(const CellOpts &o) { return o.nt > 0 && o.mix_model != OSTC_FR; }
 // EPI chemistry under DPLUR on the 9-species kernels with the two-pass
 // chemistry (at most 20 reactions)
-inline bool source_factors_chem(const CellOpts &o) {
+inline bool chem_after_fluxes(const CellOpts &o) {
   return o.mix_model == OSTC_FR && o.chem == 1 && o.implicit_method == 1 && o.ns == 9 && o.n_reac <= 20 &&
          o.n_act > 0;
 }
 template <typename R>
 int local_time_step(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st);
-// leaves the final increment in *result (one of z.dq/z.dq0/z.dqk); lu_ready
-// (source_factors_chem zones whose source_terms ran with_dt): the chemistry
+// leaves the final increment in *result (one of z.dq/z.dq0/z.dqk); chem_late
+// (chem_after_fluxes zones whose source_terms ran with_dt): the chemistry
 // source, its block's factors and dq0 are formed together after the fluxes
 template <typename R>
 int implicit_treatment(const ZoneView<R> &z, const ostc_param *dp, const CellOpts &o, hipStream_t st, R **result,
-                       bool lu_ready = false);
+                       bool chem_late = false);
 // check_slot >= 0: also classify every cell for limit_flow into repair-list
 // slot (check_slot & 1); then call limit_flow with the same slot.  Time steps
 // alternate the slot.
