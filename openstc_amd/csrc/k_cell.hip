@@ -603,19 +603,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const double rho = (double)z.bv[id], t = (double)z.bv[5 * P + id];
   double *qs = qs_mem + threadIdx.x + 64 * threadIdx.y;
   const int nr = min(p->n_reac, CR);
-  // the species rows of dq with the source added, at the bulk precision,
-  // parked in dq0's species rows while the Jacobian holds the registers
-  const auto om = [&](int l, double v) { z.dq0[(5 + l) * P + id] = z.dq[(5 + l) * P + id] + (R)((double)J * v); };
-  chem_source_lds<NS, 3>(p, nr, CR, rho, t, y, qs, 256, om, [&](int, double *a) {
-    int piv[NS];
-    epi_factor_whole<R, NS>(dt, sri, a, piv);
-    epi_store_factors<R, NS>(z, chem64(p), id, a, piv);
-    double b[NS];
+  // the block first (factored and stored, its factors kept in registers),
+  // then the production rates: the species rows of dq with the source added,
+  // at the bulk precision, then the solve
+  int piv[NS];
+  double b[NS];
+  const auto om = [&](int l, double v) { b[l] = (double)(z.dq[(5 + l) * P + id] + (R)((double)J * v)); };
+  chem_source_lds<NS, 4>(p, nr, CR, rho, t, y, qs, 256, om, [&](int phase, double *a) {
+    if (phase == 0) {
+      epi_factor_whole<R, NS>(dt, sri, a, piv);
+      epi_store_factors<R, NS>(z, chem64(p), id, a, piv);
+    } else {
+      lu_solve_any<NS>(a, piv, b);
 #pragma unroll
-    for (int m = 0; m < NS; ++m) b[m] = (double)z.dq0[(5 + m) * P + id];
-    lu_solve_any<NS>(a, piv, b);
-#pragma unroll
-    for (int m = 0; m < NS; ++m) z.dq0[(5 + m) * P + id] = (R)b[m];
+      for (int m = 0; m < NS; ++m) z.dq0[(5 + m) * P + id] = (R)b[m];
+    }
   });
 }
 

@@ -568,6 +568,71 @@ __device__ __forceinline__ void chem_source(const ostc_param *__restrict__ p, in
 // row-major, in one call row_out(0, jac) that may overwrite it; 2 DA
 // diagonal (row_out(m, &d) with the diagonal entry); 0 none.
 // omega_out(k, omega_k) receives the production rates.
+// JAC == 4 of chem_source_lds (up to 9 species): the EPI matrix first, to
+// row_out(0, jac) (which may factor it in place), then the production rates
+// to omega_out, then row_out(1, jac) with the matrix row_out(0) left: the
+// caller keeps the block's factors in registers across the production sums
+// (the same sums in the same order as the other modes).
+template <int NS, class RowOut, class OmegaOut>
+__device__ __forceinline__ void chem_jac_block(const ostc_param *__restrict__ p, const ChemF64 *ct, int nr,
+                                               int nr_max, double rho, const double *y, const double *qs, int stride,
+                                               OmegaOut omega_out, RowOut row_out) {
+  static_assert(NS <= 9, "the whole matrix in registers up to 9 species");
+  double zz[NS * NS];
+  {
+    double rry[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) rry[n] = y[n] > 1e-25 ? 1.0 / (rho * y[n]) : 0.0;
+#pragma unroll
+    for (int q = 0; q < NS * NS; ++q) zz[q] = 0;
+#pragma unroll 1
+    for (int r = 0; r < nr; ++r) {
+      const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
+      const unsigned net = ct->reac[r].net;
+      double nf[NS], nb[NS], v[NS];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        nf[n] = ct->nu_f[r * kMaxSpec + n];
+        nb[n] = ct->nu_b[r * kMaxSpec + n];
+        v[n] = nf[n] * q1 - nb[n] * q2;
+      }
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        if (net & (1u << m)) {
+          const double dm = nb[m] - nf[m];
+#pragma unroll
+          for (int n = 0; n < NS; ++n) zz[m * NS + n] += dm * v[n];
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+#pragma unroll
+      for (int n = 0; n < NS; ++n)
+        zz[m * NS + n] = y[n] > 1e-25 ? p->mw[m] * (zz[m * NS + n] * rry[n]) * 1e+3 : 0.0;
+    }
+  }
+  row_out(0, zz);
+  {
+    double cr[NS], d[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) cr[i] = d[i] = 0;
+#pragma unroll 1
+    for (int r = 0; r < nr; ++r) {
+      const double q1 = qs[r * stride], q2 = qs[(nr_max + r) * stride];
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        const double nf = ct->nu_f[r * kMaxSpec + i], nb = ct->nu_b[r * kMaxSpec + i];
+        cr[i] += q2 * nf + q1 * nb;
+        d[i] += q1 * nf + q2 * nb;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i) omega_out(i, cr[i] * (p->mw[i] * 1e+3) - d[i] * (p->mw[i] * 1e+3));
+  }
+  row_out(1, zz);
+}
+
 template <int NS, int JAC, class RowOut, class OmegaOut>
 __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p, int nr, int nr_max, double rho,
                                                 double t, const double *y, double *qs, int stride,
@@ -593,6 +658,10 @@ __device__ __forceinline__ void chem_source_lds(const ostc_param *__restrict__ p
       qs[r * stride] = q1;
       qs[(nr_max + r) * stride] = q2;
     }
+  }
+  if constexpr (JAC == 4) {
+    chem_jac_block<NS>(p, ct, nr, nr_max, rho, y, qs, stride, omega_out, row_out);
+    return;
   }
   {
     // dense over the species, as the one-pass form: a species that takes no
