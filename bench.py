@@ -49,28 +49,28 @@ METRIC = "Mcell-iter/s (flux+SST+chem+DPLUR), %HBM peak; surrogate vs plain chem
 # Re-measured whenever the kernel changes; a stage without an entry reports
 # null.
 MEASURED_TRAFFIC = {
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2306.1e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4277.4e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 5043.5e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 1057.6e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3580.3e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4036.5e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3072.2e6,
-    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.0e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_sweep"): 2304.5e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_chem"): 4276.7e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "dplur_dq0"): 4582.7e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "source"): 1061.4e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_i"): 3578.1e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_j"): 4034.7e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "flux_k"): 3071.3e6,
+    ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "update"): 1767.5e6,
     ("h2_jet_vitiated_crossflow_fr_sst_4x256x192x128", "properties"): 730.5e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 513.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1195.4e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 1389.0e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_sweep"): 514.4e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_chem"): 1195.2e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "dplur_dq0"): 1257.6e6,
     ("h2air_mixing_layer_fr_epi_dplur", "source"): 144.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 672.0e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 839.8e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 742.7e6,
-    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.7e6,
-    ("sst_flat_plate_dplur", "dplur_sweep"): 482.6e6,
-    ("sst_flat_plate_dplur", "dplur_sweep_first"): 502.9e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_i"): 671.2e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_j"): 838.4e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "flux_k"): 743.6e6,
+    ("h2air_mixing_layer_fr_epi_dplur", "update"): 442.8e6,
+    ("sst_flat_plate_dplur", "dplur_sweep"): 480.9e6,
+    ("sst_flat_plate_dplur", "dplur_sweep_first"): 499.3e6,
     ("sst_flat_plate_dplur", "flux_ij"): 358.6e6,
-    ("sst_flat_plate_dplur", "flux_k"): 402.2e6,
-    ("sst_flat_plate_dplur", "source"): 299.7e6,
+    ("sst_flat_plate_dplur", "flux_k"): 402.3e6,
+    ("sst_flat_plate_dplur", "source"): 300.8e6,
     ("sst_flat_plate_dplur", "update"): 273.5e6,
 }
 
@@ -204,9 +204,8 @@ def alg_bytes_per_cell(p, esize: int) -> dict:
     # factors and pivots read)
     if fac:
         # the chemistry after the fluxes with dq0: rho, T, Y, J, dq, dt,
-        # spectral radii 3, [J_t 2] -> dq0 (the species rows written twice and
-        # read once in between), the block's factors + packed pivots
-        put("dplur_dq0", 2 + ns + 1 + nv + 1 + 3 + (2 if rans else 0) + nv + 2 * ns, na * na, npw)
+        # spectral radii 3, [J_t 2] -> dq0, the block's factors + packed pivots
+        put("dplur_dq0", 2 + ns + 1 + nv + 1 + 3 + (2 if rans else 0) + nv, na * na, npw)
     elif epi:
         put("dplur_dq0", nv + 1 + 3 + (2 if rans else 0) + nv, 2 * na * na, npw)
     else:
